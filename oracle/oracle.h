@@ -1,0 +1,195 @@
+/*
+ * oracle.h -- CPU restatement of the LPM-C implicit load-step path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP product path:
+ * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.
+ * Nothing in the product (liblpm.so) links or calls it.
+ *
+ * Every function restates one reference function on flat arrays (row-major
+ * [particle][slot] instead of the reference's double** rows) and cites the
+ * reference file:line it follows.  Arithmetic is written in the reference's
+ * evaluation order (left-to-right C semantics, x*x for pow(x,2) as gcc -O2 folds
+ * it), and the library is built with -ffp-contract=off so that no FMA is formed.
+ * Execution is serial, which is the race-free specification of the reference's
+ * OpenMP loops.
+ *
+ * Pins (what each function is checked against; see tests/test_oracle_*.py):
+ *   orc_create_lattice      closed-form particle counts and spacings
+ *   orc_neighbor_search     closed-form lattice shells; brute force vs cell list
+ *   orc_afem_connectivity   closed form |conn| = 61 (SC/FCC interior), 31 (hex);
+ *                           K_pointer rule 9k-3 / 4k-1; brute-force set definition
+ *   orc_kntv                closed forms of Kn/Tv per lattice
+ *   orc_tangent_fd          symmetry, translation invariance, analytic Jacobian
+ *   orc_tangent_analytic    central finite differences of orc_internal_force
+ *   orc_bond_force_*        zero force at the reference configuration; affine patch
+ *                           test (interior stress = C:eps); radial-return closed form
+ *   orc_solve_cg            dense direct solve (numpy) on small systems
+ *   orc_damage_*            uniform-field closed form; brute force shell sort
+ * MKL's internal iterate sequence (dcg) is closed source: parity of solverCG is
+ * checked on converged solutions only ("parity unpinned" for iterates).
+ */
+#ifndef LPM_ORACLE_H
+#define LPM_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Model state.  All arrays are owned by the caller (numpy in the tests).
+ * Bond arrays are [n][nbmax]; slot arrays are [n][nbmax][nslot]. */
+typedef struct orc_model
+{
+    /* sizes and lattice constants */
+    int dim, lattice, n, nbmax, nb1max, nb2max, connmax, ntype;
+    double radius, cut1, cut2, volume;
+
+    /* material and damage parameters */
+    int plmode;
+    double J2_H, J2_xi;
+    double damage_L, damage_threshold, damagec_A, critical_bstrain;
+    int nbreak;
+
+    /* per-type and per-particle inputs */
+    int *type;           /* n */
+    double *Ce;          /* ntype*3: C11, C12, C44 */
+    double *sigmay;      /* n */
+    double *emod;        /* n or NULL: per-particle modulus factor (extension) */
+    double *crit;        /* n*nbmax or NULL: per-bond critical stretch (extension) */
+
+    /* geometry */
+    double *x, *x0, *xt; /* n*3: xyz, xyz_initial, xyz_temp */
+    int *nbr, *nsign;    /* n*nbmax, -1 padded */
+    int *nbr1, *nbr2;    /* n*nb1max, n*nb2max */
+    int *nb, *nb_initial;
+    double *L0;          /* n*nbmax: distance_initial */
+    double *cs0;         /* n*nbmax*3: csx/csy/csz_initial */
+    int *conn, *nb_conn; /* n*connmax, n */
+    int *kptr;           /* (n+1)*2: K_pointer */
+
+    /* bond fields */
+    double *Kn, *Tv, *dL, *dL_ave, *ddL, *ddLp, *F, *Ftemp, *bond_stress;
+    double *cs;          /* n*nbmax*3: csx/csy/csz */
+    double *dLp;         /* n*nbmax*3 slots */
+    double *dmgD;        /* n*nbmax*2 slots */
+    double *dmg_broken, *dmg_w;
+
+    /* particle fields */
+    double *dLtot, *TdLtot, *ddLtot, *TddLtot; /* n*2 */
+    double *beta;        /* n*6*3 slots */
+    double *alpha, *beta_eq; /* n*3 slots */
+    double *dlambda, *stress, *seq, *sm, *triax; /* n, n*6, n, n, n */
+    int *pl_flag;
+    double *dnl, *dloc;  /* n*2 slots: damage_nonlocal, damage_local */
+    double *damage_visual;
+    double *Pin;         /* n*3 */
+    double *Pex, *Pex_temp, *residual, *disp; /* n*dim */
+    int *dispBC, *fixidx; /* n*dim */
+    double *reaction;    /* n*dim capacity */
+    int nreaction;
+
+    /* tangent in the reference's 1-based upper CSR */
+    int64_t nK;
+    double *K;
+    int *IK, *JK;
+} orc_model;
+
+typedef struct orc_dispbc
+{
+    int type;
+    char flag;
+    double step;
+} orc_dispbc;
+
+typedef struct orc_forcebc
+{
+    int type;
+    double step[3];
+} orc_forcebc;
+
+/* Lattice description filled by orc_lattice_params (reference src/initialization.c). */
+typedef struct orc_lattice
+{
+    int dim, lattice, nbmax, nb1max, nb2max, nafem;
+    double radius, cut1, cut2, volume;
+} orc_lattice;
+
+/* Solver / loop settings; reference values are rtol2=1e-8, atol2=1e-12, maxit=dim*n,
+ * newton_tol=TOLITER, newton_maxit=MAXITER. */
+typedef struct orc_solver
+{
+    double rtol2, atol2;
+    int maxit;
+    double newton_tol;
+    int newton_maxit;
+    int tangent;         /* 0: forward FD (reference), 1: analytic elastic Jacobian */
+    int nthreads;        /* >1 only for timing (CPU baseline); 1 is the specification */
+} orc_solver;
+
+/* Per-step outputs of orc_run_steps. */
+typedef struct orc_stepout
+{
+    double reaction;     /* writeReaction value after the step */
+    double disp;         /* writeDisp value after the step */
+    int newton_iters;    /* Newton iterations over all re-solves of the step */
+    int cg_iters;        /* CG iterations summed over the step */
+    int resolves;        /* re-entries after bond breakage */
+    int broken;          /* bonds broken during the step */
+    double res_norm;     /* final residual norm */
+    double tol_mult;     /* tolerance multiplier of the last solve */
+} orc_stepout;
+
+/* ---- geometry ------------------------------------------------------------- */
+void orc_lattice_params(int lattice, double radius, double box_z, orc_lattice *out);
+int orc_create_lattice(int lattice, double radius, const double *box, int eulerflag,
+                       double a1, double a2, double a3, double *x, int cap);
+
+/* ---- neighbours ----------------------------------------------------------- */
+int orc_neighbor_search(orc_model *m);
+int orc_neighbor_search_cells(orc_model *m);
+int64_t orc_afem_connectivity(orc_model *m);
+
+/* ---- bond constants, kinematics, constitutive ----------------------------- */
+void orc_kntv(orc_model *m);
+void orc_computedL(orc_model *m);
+void orc_bond_force_elastic(orc_model *m, int ii);
+void orc_bond_force_j2(orc_model *m, int ii);
+void orc_bond_force_incremental(orc_model *m, int ii);
+void orc_compute_stress(orc_model *m);
+void orc_switch_state(orc_model *m, int flag);
+void orc_bond_force_general(orc_model *m, int mode);
+void orc_internal_force(orc_model *m, const double *x, double *pin);
+
+/* ---- tangent, residual, boundary conditions -------------------------------- */
+void orc_tangent_fd(orc_model *m);
+void orc_tangent_analytic(orc_model *m);
+void orc_update_rr(orc_model *m);
+void orc_set_disp_bc(orc_model *m, int nbd, const orc_dispbc *bc);
+void orc_set_force_bc(orc_model *m, int nbf, const orc_forcebc *bc);
+void orc_bc_stiffness_update(orc_model *m);
+double orc_nrm2(int n, const double *x);
+
+/* ---- linear solve ----------------------------------------------------------- */
+void orc_spmv_upper(const orc_model *m, const double *x, double *y);
+int orc_solve_cg(orc_model *m, const orc_solver *s, double *final_rr);
+
+/* ---- damage and cracks ------------------------------------------------------ */
+int orc_damage_pwise_nonlocal(orc_model *m);
+int orc_damage_brittle(orc_model *m, int *broken_i, int *broken_j, int cap);
+void orc_update_crack(orc_model *m);
+int orc_shell_sort_select(int k, const double *bstrain, int nbreak, int *order);
+
+/* ---- outputs ------------------------------------------------------------------- */
+double orc_reaction(const orc_model *m, char axis, int t1);
+double orc_mean_disp(const orc_model *m, char axis, int t1);
+
+/* ---- load-step loop ------------------------------------------------------------- */
+int orc_run_steps(orc_model *m, const orc_solver *s, int nsteps,
+                  const int *nbd, const orc_dispbc *dbc, const int *nbf, const orc_forcebc *fbc,
+                  int maxbc, char out_axis, int out_type, orc_stepout *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

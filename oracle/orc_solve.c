@@ -1,0 +1,432 @@
+/*
+ * Oracle: linear solve, damage, crack bookkeeping, outputs and the load-step loop.
+ * TEST INFRASTRUCTURE ONLY (see oracle.h).
+ */
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "oracle.h"
+
+#define O_EPS 1e-6
+#define O_PI 3.14159265358979323846
+
+static inline int64_t B(const orc_model *m, int i, int j) { return (int64_t)i * m->nbmax + j; }
+
+/* y = K x for the symmetric matrix stored as the reference's 1-based upper CSR
+ * (the operation of mkl_sparse_d_mv with SPARSE_MATRIX_TYPE_SYMMETRIC/UPPER,
+ * src/solver.c:336-347,384). */
+void orc_spmv_upper(const orc_model *m, const double *x, double *y)
+{
+    const int nr = m->dim * m->n;
+    memset(y, 0, sizeof(double) * nr);
+    for (int r = 0; r < nr; r++)
+    {
+        double acc = 0.0;
+        for (int p = m->IK[r] - 1; p < m->IK[r + 1] - 1; p++)
+        {
+            int c = m->JK[p] - 1;
+            acc += m->K[p] * x[c];
+            if (c != r)
+                y[c] += m->K[p] * x[r];
+        }
+        y[r] += acc;
+    }
+}
+
+/* Parallel variant for the CPU timing baseline: rows are split into thread bands,
+ * each thread scatters into its own buffer, buffers are summed at the end. */
+static void spmv_upper_par(const orc_model *m, const double *x, double *y, double *scratch, int nt)
+{
+    const int nr = m->dim * m->n;
+#ifdef _OPENMP
+#pragma omp parallel num_threads(nt)
+    {
+        int t = omp_get_thread_num();
+        double *yt = scratch + (size_t)t * nr;
+        memset(yt, 0, sizeof(double) * nr);
+#pragma omp for schedule(static)
+        for (int r = 0; r < nr; r++)
+        {
+            double acc = 0.0;
+            for (int p = m->IK[r] - 1; p < m->IK[r + 1] - 1; p++)
+            {
+                int c = m->JK[p] - 1;
+                acc += m->K[p] * x[c];
+                if (c != r)
+                    yt[c] += m->K[p] * x[r];
+            }
+            yt[r] += acc;
+        }
+#pragma omp for schedule(static)
+        for (int r = 0; r < nr; r++)
+        {
+            double s = 0.0;
+            for (int u = 0; u < nt; u++)
+                s += scratch[(size_t)u * nr + r];
+            y[r] = s;
+        }
+    }
+#else
+    (void)scratch, (void)nt;
+    orc_spmv_upper(m, x, y);
+#endif
+}
+
+/* solverCG (reference src/solver.c:329-411): unpreconditioned conjugate gradients
+ * from disp = 0, stopping when |r|^2 <= rtol2*|r0|^2 + atol2 (MKL dcg's residual
+ * test with dpar[0]=rtol2, dpar[1]=atol2 on squared norms) or after maxit
+ * iterations; then xyz += disp, also on failure.  Returns the iteration count,
+ * negative if maxit was reached. */
+int orc_solve_cg(orc_model *m, const orc_solver *s, double *final_rr)
+{
+    const int nr = m->dim * m->n;
+    const int nt = s->nthreads > 1 ? s->nthreads : 1;
+    double *r = malloc(sizeof(double) * nr), *p = malloc(sizeof(double) * nr), *q = malloc(sizeof(double) * nr);
+    double *scratch = nt > 1 ? malloc(sizeof(double) * (size_t)nr * nt) : NULL;
+    double *x = m->disp;
+    memset(x, 0, sizeof(double) * nr);
+    memcpy(r, m->residual, sizeof(double) * nr);
+    memcpy(p, r, sizeof(double) * nr);
+    double rr = 0.0;
+    for (int k = 0; k < nr; k++)
+        rr += r[k] * r[k];
+    const double thr = s->rtol2 * rr + s->atol2;
+    int it = 0, conv = rr <= thr;
+    while (!conv && it < s->maxit)
+    {
+        if (nt > 1)
+            spmv_upper_par(m, p, q, scratch, nt);
+        else
+            orc_spmv_upper(m, p, q);
+        double pq = 0.0;
+        for (int k = 0; k < nr; k++)
+            pq += p[k] * q[k];
+        double alpha = rr / pq, rrn = 0.0;
+        for (int k = 0; k < nr; k++)
+        {
+            x[k] += alpha * p[k];
+            r[k] -= alpha * q[k];
+            rrn += r[k] * r[k];
+        }
+        it++;
+        if (rrn <= thr)
+        {
+            rr = rrn;
+            conv = 1;
+            break;
+        }
+        double beta = rrn / rr;
+        rr = rrn;
+        for (int k = 0; k < nr; k++)
+            p[k] = r[k] + beta * p[k];
+    }
+    if (final_rr)
+        *final_rr = rr;
+    for (int i = 0; i < m->n; i++)
+        for (int c = 0; c < m->dim; c++)
+            m->x[3 * i + c] += x[m->dim * i + c];
+    free(r), free(p), free(q), free(scratch);
+    return conv ? it : -it;
+}
+
+/* updateDuctileDamagePwiseNonlocal (reference src/constitutive.c:1757-1862):
+ * Gaussian-weighted average of dlambda*(1 + A*triaxiality) over all particles
+ * within 3L in the initial configuration (index order), bond breakage when either
+ * end exceeds the threshold, bond damage max(D_i, D_j), w = 1 - D. */
+int orc_damage_pwise_nonlocal(orc_model *m)
+{
+    const double L = m->damage_L;
+    for (int i = 0; i < m->n; i++)
+    {
+        double *dn = m->dnl + 2 * i;
+        if (dn[0] > m->damage_threshold)
+        {
+            if (dn[0] > 1.0)
+                dn[0] = 1.0;
+            continue;
+        }
+        double Dd = 0, A = 0;
+        for (int j = 0; j < m->n; j++)
+        {
+            double dx = m->x0[3 * j] - m->x0[3 * i], dy = m->x0[3 * j + 1] - m->x0[3 * i + 1],
+                   dz = m->x0[3 * j + 2] - m->x0[3 * i + 2];
+            double dis = sqrt(dx * dx + dy * dy + dz * dz);
+            if (dis < 3 * L)
+            {
+                double loc = 0;
+                double f = (1.0 + m->damagec_A * m->triax[j]);
+                if (f > 0.0)
+                    loc = m->dlambda[j] * (1.0 + m->damagec_A * m->triax[j]);
+                double phi = 1.0 / L / sqrt(2 * O_PI) * exp(-0.5 * dis * dis / L / L);
+                Dd += loc * phi * m->volume;
+                A += phi * m->volume;
+            }
+        }
+        if (Dd > 0.0)
+            dn[0] += 1.0 / A * Dd;
+    }
+    int k = 0;
+    for (int i = 0; i < m->n; i++)
+        for (int j = 0; j < m->nb_initial[i]; j++)
+        {
+            int64_t b = B(m, i, j);
+            int q = m->nbr[b];
+            if (m->dnl[2 * i] > m->damage_threshold || m->dnl[2 * q] > m->damage_threshold)
+                if (fabs(m->dmg_broken[b]) > O_EPS)
+                {
+                    m->dmg_broken[b] = 0.0;
+                    m->dmgD[2 * b] = 1.0;
+                    k++;
+                }
+        }
+    for (int i = 0; i < m->n; i++)
+        for (int j = 0; j < m->nb_initial[i]; j++)
+        {
+            int64_t b = B(m, i, j);
+            int q = m->nbr[b];
+            if (fabs(m->dmg_broken[b]) > O_EPS)
+            {
+                double a = m->dnl[2 * i], c = m->dnl[2 * q];
+                m->dmgD[2 * b] = a < c ? c : a;
+            }
+            m->dmg_w[b] = 1.0 - m->dmgD[2 * b];
+        }
+    return k;
+}
+
+/* The shell sort of updateBrittleDamage (reference src/constitutive.c:1486-1508):
+ * gaps k/2, k/4, ..., 1, insertion with strict '>' so that ties keep their gap
+ * order.  order[] is permuted alongside the keys; returns k. */
+int orc_shell_sort_select(int k, const double *bstrain, int nbreak, int *order)
+{
+    double *key = malloc(sizeof(double) * (k > 0 ? k : 1));
+    for (int i = 0; i < k; i++)
+        key[i] = bstrain[i], order[i] = i;
+    for (int r = k / 2; r >= 1; r = r / 2)
+        for (int i = r; i < k; ++i)
+        {
+            double tb = key[i];
+            int to = order[i];
+            int j = i - r;
+            while (j >= 0 && key[j] > tb)
+            {
+                key[j + r] = key[j];
+                order[j + r] = order[j];
+                j = j - r;
+            }
+            key[j + r] = tb;
+            order[j + r] = to;
+        }
+    free(key);
+    (void)nbreak;
+    return k;
+}
+
+/* updateBrittleDamage (reference src/constitutive.c:1437-1526): bonds with
+ * dL/L0 >= critical (per-bond critical stretch with the extension), collected in
+ * (i, slot) order; all break if there are at most nbreak, otherwise the nbreak
+ * largest after the reference's shell sort.  The reference's fixed 400-entry
+ * candidate buffer is replaced by an unbounded one.  Broken (i, j) pairs are
+ * written to broken_i/broken_j (up to cap); returns the candidate count k. */
+int orc_damage_brittle(orc_model *m, int *bi, int *bj, int cap)
+{
+    int k = 0, capc = 1024;
+    int *ci = malloc(sizeof(int) * capc), *cj = malloc(sizeof(int) * capc);
+    double *cb = malloc(sizeof(double) * capc);
+    for (int i = 0; i < m->n; i++)
+        for (int j = 0; j < m->nb_initial[i]; j++)
+        {
+            int64_t b = B(m, i, j);
+            double crit = m->crit ? m->crit[b] : m->critical_bstrain;
+            double ave = m->dL[b] / m->L0[b];
+            if (ave >= crit)
+            {
+                if (k == capc)
+                {
+                    capc *= 2;
+                    ci = realloc(ci, sizeof(int) * capc);
+                    cj = realloc(cj, sizeof(int) * capc);
+                    cb = realloc(cb, sizeof(double) * capc);
+                }
+                ci[k] = i, cj[k] = j, cb[k] = ave;
+                k++;
+            }
+        }
+    int nb_out = 0;
+    if (k > 0 && k <= m->nbreak)
+    {
+        for (int t = 0; t < k; t++)
+        {
+            int64_t b = B(m, ci[t], cj[t]);
+            m->dmgD[2 * b] = 1.0;
+            m->dmg_w[b] = 0.0;
+            m->dmg_broken[b] = 0.0;
+            if (nb_out < cap)
+                bi[nb_out] = ci[t], bj[nb_out] = m->nbr[b], nb_out++;
+        }
+    }
+    else if (k > m->nbreak)
+    {
+        int *ord = malloc(sizeof(int) * k);
+        orc_shell_sort_select(k, cb, m->nbreak, ord);
+        for (int t = k - m->nbreak; t < k; t++)
+        {
+            int64_t b = B(m, ci[ord[t]], cj[ord[t]]);
+            m->dmgD[2 * b] = 1.0;
+            m->dmg_w[b] = 0.0;
+            m->dmg_broken[b] = 0.0;
+            if (nb_out < cap)
+                bi[nb_out] = ci[ord[t]], bj[nb_out] = m->nbr[b], nb_out++;
+        }
+        free(ord);
+    }
+    free(ci), free(cj), free(cb);
+    return k;
+}
+
+/* updateCrack (reference src/constitutive.c:1399-1434): live-bond count, visual
+ * damage, F *= damage_w (a second time after the constitutive law), Pin from F,
+ * particles without bonds fixed. */
+void orc_update_crack(orc_model *m)
+{
+    const int d = m->dim;
+    for (int i = 0; i < m->n; i++)
+    {
+        m->nb[i] = m->nb_initial[i];
+        m->damage_visual[i] = 0.0;
+        double *P = m->Pin + 3 * i;
+        P[0] = 0.0, P[1] = 0.0, P[2] = 0.0;
+        for (int j = 0; j < m->nb_initial[i]; j++)
+        {
+            int64_t b = B(m, i, j);
+            if (m->dmg_broken[b] <= O_EPS)
+                m->nb[i] -= 1;
+            m->damage_visual[i] += m->dmg_broken[b];
+            m->F[b] *= m->dmg_w[b];
+            P[0] += m->cs[3 * b + 0] * m->F[b];
+            P[1] += m->cs[3 * b + 1] * m->F[b];
+            P[2] += m->cs[3 * b + 2] * m->F[b];
+        }
+        if (m->nb[i] < 1)
+        {
+            m->fixidx[d * i] = 0;
+            m->fixidx[d * i + 1] = 0;
+            if (d == 3)
+                m->fixidx[d * i + 2] = 0;
+        }
+        m->damage_visual[i] = 1 - m->damage_visual[i] / m->nb_initial[i];
+    }
+}
+
+/* writeReaction / writeDisp values (reference src/data_handler.c:278-335). */
+double orc_reaction(const orc_model *m, char axis, int t1)
+{
+    int c = axis == 'x' ? 0 : axis == 'y' ? 1 : 2;
+    double force = 0.;
+    for (int i = 0; i < m->n; i++)
+        if (m->type[i] == t1)
+            for (int j = 0; j < m->nb_initial[i]; j++)
+            {
+                int64_t b = B(m, i, j);
+                force += -m->cs[3 * b + c] * m->F[b];
+            }
+    return force;
+}
+
+double orc_mean_disp(const orc_model *m, char axis, int t1)
+{
+    int c = axis == 'x' ? 0 : axis == 'y' ? 1 : 2;
+    int num = 0;
+    double p = 0, p0 = 0;
+    for (int i = 0; i < m->n; i++)
+        if (m->type[i] == t1)
+        {
+            p += m->x[3 * i + c];
+            p0 += m->x0[3 * i + c];
+            num++;
+        }
+    return (p - p0) / num;
+}
+
+static int damage_general(orc_model *m)
+{
+    if (m->plmode == 0)
+        return orc_damage_pwise_nonlocal(m);
+    if (m->plmode == 6)
+        return orc_damage_brittle(m, NULL, NULL, 0);
+    return 0;
+}
+
+/* The load-step loop of the reference driver (src/lpmc_project.c:382-546) with the
+ * solver tolerances as parameters.  computeStrain (output only) is not run.
+ * Returns 0, or the index+1 of the first step whose Newton loop hit the cap. */
+int orc_run_steps(orc_model *m, const orc_solver *s, int nsteps,
+                  const int *nbd, const orc_dispbc *dbc, const int *nbf, const orc_forcebc *fbc,
+                  int maxbc, char out_axis, int out_type, orc_stepout *out)
+{
+    const int d = m->dim, n = m->n;
+    int first_fail = 0;
+    for (int st = 0; st < nsteps; st++)
+    {
+        orc_stepout o;
+        memset(&o, 0, sizeof(o));
+        memcpy(m->xt, m->x, sizeof(double) * 3 * n);
+        memcpy(m->Ftemp, m->F, sizeof(double) * (size_t)n * m->nbmax);
+        memcpy(m->Pex_temp, m->Pex, sizeof(double) * d * n);
+        if (s->tangent == 0)
+            orc_tangent_fd(m);
+        else
+            orc_tangent_analytic(m);
+        orc_set_disp_bc(m, nbd[st], dbc + (size_t)st * maxbc);
+        orc_set_force_bc(m, nbf[st], fbc + (size_t)st * maxbc);
+        orc_bond_force_general(m, 4);
+        for (;;)
+        {
+            orc_update_rr(m);
+            double nres = orc_nrm2(d * n, m->residual);
+            double nrea = orc_nrm2(m->nreaction, m->reaction);
+            double tolm = nres > nrea ? nres : nrea;
+            int ni = 0;
+            while (nres > s->newton_tol * tolm && ni < s->newton_maxit)
+            {
+                orc_switch_state(m, 0);
+                orc_bc_stiffness_update(m);
+                int it = orc_solve_cg(m, s, NULL);
+                o.cg_iters += it < 0 ? -it : it;
+                orc_bond_force_general(m, m->plmode);
+                orc_update_rr(m);
+                ni++;
+                nres = orc_nrm2(d * n, m->residual);
+            }
+            if (ni >= s->newton_maxit && !first_fail)
+                first_fail = st + 1;
+            o.newton_iters += ni;
+            o.res_norm = nres;
+            o.tol_mult = tolm;
+            int broken = damage_general(m);
+            orc_update_crack(m);
+            orc_switch_state(m, 1);
+            o.broken += broken;
+            o.reaction = orc_reaction(m, out_axis, out_type);
+            o.disp = orc_mean_disp(m, out_axis, out_type);
+            if (broken > 0)
+            {
+                o.resolves++;
+                if (s->tangent == 0)
+                    orc_tangent_fd(m);
+                else
+                    orc_tangent_analytic(m);
+                continue;
+            }
+            break;
+        }
+        if (out)
+            out[st] = o;
+    }
+    return first_fail;
+}

@@ -1,0 +1,356 @@
+"""Pins of the CPU oracle against closed forms, invariants and brute force.
+
+The reference ships no tests and no golden vectors, and it cannot be built here
+(its Intel MKL headers are absent), so each oracle function is pinned by what the
+mathematics fixes, chosen so that a dropped term, a wrong sign or index, or a
+transposed operand fails a check.
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+E0, NU0 = 200e3, 0.3
+
+
+def sc(O, L, ntype=1, **kw):
+    x = O.create_lattice(2, 0.5, [0, L, 0, L, 0, L])
+    c = O.elastic_constants(E0, NU0)
+    m = O.Model(x, 2, 0.5, np.zeros(x.shape[0], np.int32), [c] * ntype, **kw)
+    m.setup()
+    return m, c
+
+
+def hexm(O, W, H, **kw):
+    x = O.create_lattice(1, 0.5, [0, W, 0, H, 0, 1.0])
+    c = O.elastic_constants(70e3, 0.25)
+    m = O.Model(x, 1, 0.5, np.zeros(x.shape[0], np.int32), [c], **kw)
+    m.setup()
+    return m, c
+
+
+def fcc(O, L, **kw):
+    x = O.create_lattice(3, 0.5, [0, L, 0, L, 0, L])
+    c = O.elastic_constants(70e3, 0.3)
+    m = O.Model(x, 3, 0.5, np.zeros(x.shape[0], np.int32), [c], **kw)
+    m.setup()
+    return m, c
+
+
+# ---------------------------------------------------------------- geometry -------
+@pytest.mark.parametrize("L,n", [(19.035, 8000), (59.3, 216000), (99.195, 1000000)])
+def test_sc_particle_counts(oracle, L, n):
+    x = oracle.create_lattice(2, 0.5, [0, L, 0, L, 0, L])
+    assert x.shape[0] == n
+    k = round(n ** (1 / 3))
+    # x fastest, then y, then z; unit spacing; x starts at box[0]
+    g = x.reshape(k, k, k, 3)
+    assert np.all(np.diff(g[..., 0], axis=2) == 1.0)
+    assert np.all(np.diff(g[..., 1], axis=1) == 1.0)
+    assert np.all(np.diff(g[..., 2], axis=0) == 1.0)
+    assert g[0, 0, 0, 0] == 0.0
+
+
+def test_hex_and_fcc_counts(oracle):
+    assert oracle.create_lattice(1, 0.5, [0, 999.6, 0, 865.77, 0, 1]).shape[0] == 1000000
+    assert oracle.create_lattice(3, 0.5, [0, 55.907, 0, 55.907, 0, 55.907]).shape[0] == 256000
+
+
+# -------------------------------------------------------------- neighbours -------
+def shells(lattice):
+    if lattice == 2:
+        o1 = [v for v in itertools.product((-1, 0, 1), repeat=3) if sum(map(abs, v)) == 1]
+        o2 = [v for v in itertools.product((-1, 0, 1), repeat=3) if sum(map(abs, v)) == 2]
+        return np.array(o1, float), np.array(o2, float)
+    if lattice == 3:
+        h = 1.0 / np.sqrt(2.0)  # a/2 with a = 4r/sqrt(2), r = 0.5
+        o1 = [np.array(v) * h for v in itertools.product((-1, 0, 1), repeat=3) if sum(map(abs, v)) == 2]
+        o2 = [np.array(v) * 2 * h for v in itertools.product((-1, 0, 1), repeat=3) if sum(map(abs, v)) == 1]
+        return np.array(o1), np.array(o2)
+    ang = np.arange(6) * np.pi / 3
+    o1 = np.stack([np.cos(ang), np.sin(ang), 0 * ang], 1)
+    o2 = np.stack([np.sqrt(3) * np.cos(ang + np.pi / 6), np.sqrt(3) * np.sin(ang + np.pi / 6), 0 * ang], 1)
+    return o1, o2
+
+
+@pytest.mark.parametrize("kind", ["sc", "fcc", "hex"])
+def test_neighbor_shells_closed_form(oracle, kind):
+    m = {"sc": lambda: sc(oracle, 7.1)[0], "fcc": lambda: fcc(oracle, 4.0)[0],
+         "hex": lambda: hexm(oracle, 9.0, 8.0)[0]}[kind]()
+    lat = {"sc": 2, "fcc": 3, "hex": 1}[kind]
+    o1, o2 = shells(lat)
+    full = np.where(m.nb_initial == m.nbmax)[0]
+    assert len(full) > 0
+    for i in full[:: max(1, len(full) // 25)]:
+        nb = m.nb_initial[i]
+        js = m.nbr[i, :nb]
+        assert np.all(np.diff(js) > 0), "lists are in increasing particle index"
+        d = m.x0[js] - m.x0[i]
+        for o, s in ((o1, 0), (o2, 1)):
+            sel = m.nsign[i, :nb] == s
+            got = sorted(map(tuple, np.round(d[sel], 6)))
+            assert got == sorted(map(tuple, np.round(o, 6)))
+        np.testing.assert_array_equal(m.cs0[i, :nb], (m.x0[i] - m.x0[js]) / m.L0[i, :nb, None])
+    # every bond is mutual and has a mutual reverse slot
+    for i in range(m.n):
+        for j in m.nbr[i, :m.nb_initial[i]]:
+            assert i in m.nbr[j, :m.nb_initial[j]]
+
+
+@pytest.mark.parametrize("kind", ["sc", "fcc", "hex"])
+def test_cell_list_equals_brute_force(oracle, kind):
+    m = {"sc": lambda: sc(oracle, 6.3)[0], "fcc": lambda: fcc(oracle, 3.5)[0],
+         "hex": lambda: hexm(oracle, 7.0, 6.0)[0]}[kind]()
+    ref = {k: m.A[k].copy() for k in ("nbr", "nsign", "L0", "cs0", "nb_initial", "nbr1", "nbr2")}
+    m.neighbors(cells=True)
+    for k, v in ref.items():
+        np.testing.assert_array_equal(m.A[k], v, err_msg=k)
+
+
+@pytest.mark.parametrize("kind,interior", [("sc", 61), ("fcc", 61), ("hex", 31)])
+def test_conn_sets_and_pointers(oracle, kind, interior):
+    m = {"sc": lambda: sc(oracle, 7.1)[0], "fcc": lambda: fcc(oracle, 6.5)[0],
+         "hex": lambda: hexm(oracle, 9.0, 8.0)[0]}[kind]()
+    assert m.nb_conn.max() == interior
+    d2 = m.dim * m.dim - (1 if m.dim == 2 else 3)
+    for i in range(m.n):
+        # definition: i's bond partners and the same-shell partners of each partner
+        want = set()
+        for a in range(m.nb_initial[i]):
+            j, s = m.nbr[i, a], m.nsign[i, a]
+            want.add(j)
+            want |= {q for b, q in enumerate(m.nbr[j, :m.nb_initial[j]]) if m.nsign[j, b] == s}
+        got = m.conn[i, :m.nb_conn[i]]
+        assert list(got) == sorted(want)
+        k = int(np.sum(got >= i))
+        assert m.kptr[i, 0] == k
+        assert m.kptr[i + 1, 1] == m.kptr[i, 1] + m.dim * m.dim * k - (1 if m.dim == 2 else 3)
+    assert m.s.nK == m.kptr[m.n, 1]
+    assert d2 > 0
+
+
+def test_kntv_closed_form(oracle):
+    m, (c11, c12, c44) = sc(oracle, 5.2)
+    r = 0.5
+    s0 = m.nsign == 0
+    s1 = m.nsign == 1
+    assert np.allclose(m.Kn[s0], r * (c11 - c12 - c44), rtol=1e-15)
+    assert np.allclose(m.Kn[s1], r * c44, rtol=1e-15)
+    assert np.allclose(m.Tv[s0 | s1], r * (c12 - c44) / 18.0, rtol=1e-14)
+
+
+# --------------------------------------------------------- constitutive laws -----
+def deep_interior(m):
+    full = m.nb_initial == m.nbmax
+    return np.array([i for i in np.where(full)[0] if np.all(full[m.nbr[i, :m.nbmax]])])
+
+
+def voigt_C(c11, c12, c44):
+    C = np.zeros((6, 6))
+    C[:3, :3] = c12
+    np.fill_diagonal(C[:3, :3], c11)
+    C[3, 3] = C[4, 4] = C[5, 5] = c44
+    return C
+
+
+@pytest.mark.parametrize("kind", ["sc", "fcc"])
+def test_reference_config_and_affine_patch(oracle, kind):
+    m, c = sc(oracle, 7.1, sigmay=1e9) if kind == "sc" else fcc(oracle, 6.5, sigmay=1e9)
+    oracle.lib().orc_bond_force_general(m.p, 6)
+    assert np.abs(m.Pin).max() == 0.0, "no force at the reference configuration"
+    eps = 1e-7 * np.array([[1.0, 0.2, -0.3], [0.2, -0.5, 0.1], [-0.3, 0.1, 0.7]])
+    m.x[:] = m.x0 + m.x0 @ eps.T
+    oracle.lib().orc_bond_force_general(m.p, 6)
+    deep = deep_interior(m)
+    assert len(deep) > 0
+    ev = np.array([eps[0, 0], eps[1, 1], eps[2, 2], 2 * eps[1, 2], 2 * eps[0, 2], 2 * eps[0, 1]])
+    sig = voigt_C(*c) @ ev
+    np.testing.assert_allclose(m.stress[deep], np.broadcast_to(sig, (len(deep), 6)), rtol=0,
+                               atol=2e-6 * np.abs(sig).max())
+    # interior equilibrium, up to the cancellation in dis - L0 at strain 1e-7
+    assert np.abs(m.Pin.reshape(-1, 3)[deep]).max() < 1e-6 * np.abs(m.F).max()
+
+
+def test_hex_affine_patch_plane(oracle):
+    m, (c11, c12, c44) = hexm(oracle, 12.0, 10.0, sigmay=1e9)
+    eps = 1e-7 * np.array([[1.0, 0.4, 0], [0.4, -0.6, 0], [0, 0, 0]])
+    m.x[:] = m.x0 + m.x0 @ eps.T
+    oracle.lib().orc_bond_force_general(m.p, 6)
+    deep = deep_interior(m)
+    sig = np.array([c11 * eps[0, 0] + c12 * eps[1, 1], c12 * eps[0, 0] + c11 * eps[1, 1], 2 * c44 * eps[0, 1]])
+    got = m.stress[deep][:, [0, 1, 5]] * 1.0  # box_z = 1
+    np.testing.assert_allclose(got, np.broadcast_to(sig, got.shape), rtol=0, atol=2e-6 * np.abs(sig).max())
+
+
+def test_radial_return_closed_form(oracle):
+    H = 2000.0
+    m, c = sc(oracle, 7.1, sigmay=200.0, J2_H=H)
+    eps = 4e-3 * np.array([[1.0, 0, 0], [0, -0.5, 0], [0, 0, -0.5]])
+    m.x[:] = m.x0 + m.x0 @ eps.T
+    oracle.lib().orc_bond_force_general(m.p, 0)
+    deep = deep_interior(m)
+    ev = np.array([eps[0, 0], eps[1, 1], eps[2, 2], 0, 0, 0])
+    s = voigt_C(*c) @ ev
+    s[:3] -= s[:3].mean()
+    seq = np.sqrt(1.5 * (np.sum(s[:3] ** 2) + 2 * np.sum(s[3:] ** 2)))
+    dlam = (seq - 200.0) / (3 * c[2] + H)
+    assert dlam > 0
+    np.testing.assert_allclose(m.dlambda[deep], dlam, rtol=5e-3)
+    # plastic flow reduces the deviatoric stress to the hardened yield surface
+    assert np.all(m.pl_flag[deep] == 1)
+
+
+# ---------------------------------------------------------------- tangent --------
+def test_analytic_jacobian_matches_central_differences(oracle):
+    m, _ = sc(oracle, 3.2)
+    rng = np.random.default_rng(3)
+    m.dLp[:, :, 0] = 1e-4 * rng.standard_normal(m.dLp[:, :, 0].shape) * (m.nbr >= 0)
+    m.x[:] = m.x0 + 2e-3 * rng.standard_normal(m.x.shape)
+    x = m.x.copy()
+    n = m.n
+    Jn = np.zeros((3 * n, 3 * n))
+    h = 1e-6
+    f1, f2 = np.zeros(3 * n), np.zeros(3 * n)
+    for col in range(3 * n):
+        xp = x.copy().reshape(-1)
+        xp[col] += h
+        oracle.lib().orc_internal_force(m.p, oracle.ptr(xp.reshape(n, 3).copy()), oracle.ptr(f1))
+        xp[col] -= 2 * h
+        oracle.lib().orc_internal_force(m.p, oracle.ptr(xp.reshape(n, 3).copy()), oracle.ptr(f2))
+        Jn[:, col] = (f1 - f2) / (2 * h)
+    oracle.lib().orc_tangent_analytic(m.p)
+    Ka = oracle.upper_csr_dense(m)
+    Jsym = Jn.copy()
+    up = np.triu(np.ones_like(Jn), 1)
+    # off-diagonal blocks: 0.5 (J + J^T); diagonal blocks: the upper triangle of J
+    Jsym = 0.5 * (Jn + Jn.T)
+    for i in range(n):
+        b = Jn[3 * i:3 * i + 3, 3 * i:3 * i + 3]
+        Jsym[3 * i:3 * i + 3, 3 * i:3 * i + 3] = np.triu(b) + np.triu(b, 1).T
+    assert up.sum() > 0
+    err = np.abs(Ka - Jsym).max() / np.abs(Jsym).max()
+    assert err < 1e-7, err
+
+
+def test_fd_tangent_pattern_symmetry_and_translation(oracle):
+    m, _ = sc(oracle, 4.3)
+    rng = np.random.default_rng(5)
+    m.x[:] = m.x0 + 1e-3 * rng.standard_normal(m.x.shape)
+    oracle.lib().orc_tangent_fd(m.p)
+    Kfd = m.K.copy()
+    ik, jk = m.IK.copy(), m.JK.copy()
+    oracle.lib().orc_tangent_analytic(m.p)
+    np.testing.assert_array_equal(m.IK, ik)
+    np.testing.assert_array_equal(m.JK, jk)
+    assert np.abs(Kfd - m.K).max() / np.abs(m.K).max() < 1e-5
+    # 1-based upper CSR, ascending columns in each row, diagonal first
+    for r in range(m.dim * m.n):
+        cols = jk[ik[r] - 1:ik[r + 1] - 1] - 1
+        assert cols[0] == r and np.all(np.diff(cols) > 0)
+    D = oracle.upper_csr_dense(m)
+    for c in range(3):
+        t = np.zeros(3 * m.n)
+        t[c::3] = 1.0
+        assert np.abs(D @ t).max() < 1e-9 * np.abs(D).max()
+
+
+def test_bc_elimination_and_cg_vs_dense(oracle):
+    m, _ = sc(oracle, 4.3, ntype=3)
+    n = m.n
+    types = np.zeros(n, np.int32)
+    z = m.x0[:, 2]
+    types[z < z.min() + 0.1] = 2
+    types[z > z.max() - 0.1] = 1
+    m.type[:] = types
+    oracle.lib().orc_kntv(m.p)
+    oracle.lib().orc_tangent_analytic(m.p)
+    m.dispBC.reshape(n, 3)[types == 2, :] = 0  # bottom fully fixed
+    m.dispBC.reshape(n, 3)[types == 1, 2] = 0  # top z prescribed
+    rng = np.random.default_rng(1)
+    m.residual[:] = rng.standard_normal(3 * n) * m.dispBC
+    D0 = oracle.upper_csr_dense(m)
+    oracle.lib().orc_bc_stiffness_update(m.p)
+    D = oracle.upper_csr_dense(m)
+    bc = np.where(m.dispBC == 0)[0]
+    nd = np.linalg.norm(np.diag(D0))
+    for r in bc:
+        row = D[r].copy()
+        assert abs(row[r] - nd) <= 1e-14 * nd
+        row[r] = 0
+        assert not row.any() and not D[:, r][np.arange(3 * n) != r].any()
+    want = np.linalg.solve(D, m.residual)
+    x0 = m.x.copy()
+    s = oracle.solver(rtol2=1e-28, atol2=0.0, maxit=20 * 3 * n)
+    it = oracle.lib().orc_solve_cg(m.p, s, None)
+    assert it > 0
+    np.testing.assert_allclose(m.disp, want, rtol=0, atol=1e-10 * np.abs(want).max())
+    np.testing.assert_allclose(m.x - x0, m.disp.reshape(n, 3), rtol=0, atol=1e-15)
+
+
+# ---------------------------------------------------------------- damage ---------
+def test_nonlocal_damage_uniform_field(oracle):
+    m, _ = sc(oracle, 6.3, damage_L=0.5)
+    m.dlambda[:] = 3e-3
+    m.triax[:] = 0.25
+    oracle.lib().orc_damage_pwise_nonlocal(m.p)
+    # a uniform rate field is reproduced exactly by the normalised Gaussian average
+    np.testing.assert_allclose(m.dnl[:, 0], 3e-3, rtol=1e-14)
+    live = m.nbr >= 0
+    np.testing.assert_allclose(m.dmg_w[live], 1 - 3e-3, rtol=1e-14)
+
+
+def test_nonlocal_damage_weights_brute_force(oracle):
+    m, _ = sc(oracle, 5.2, damage_L=0.7, damagec_A=0.5)
+    rng = np.random.default_rng(7)
+    m.dlambda[:] = rng.uniform(0, 1e-2, m.n)
+    m.triax[:] = rng.uniform(-0.5, 0.5, m.n)
+    oracle.lib().orc_damage_pwise_nonlocal(m.p)
+    L = 0.7
+    X = m.x0
+    d = np.sqrt(((X[None] - X[:, None]) ** 2).sum(-1))
+    w = np.where(d < 3 * L, np.exp(-0.5 * d * d / L / L), 0.0)
+    rate = m.dlambda * (1 + 0.5 * m.triax)
+    np.testing.assert_allclose(m.dnl[:, 0], (w @ rate) / w.sum(1), rtol=1e-12)
+
+
+def test_shell_sort_selection(oracle):
+    rng = np.random.default_rng(11)
+    for k in (1, 2, 7, 40, 401):
+        v = rng.uniform(0, 1, k)
+        order = np.zeros(k, np.int32)
+        oracle.lib().orc_shell_sort_select(k, oracle.ptr(v), 5, oracle.ptr(order))
+        assert np.all(np.diff(v[order]) >= 0)
+        assert sorted(order) == list(range(k))
+    # with ties the order of equal keys is the shell sort's, not a stable sort's:
+    # check against an independent restatement of the reference's gap sequence
+    v = np.repeat(rng.uniform(0, 1, 30), 2)[rng.permutation(60)]
+    order = np.zeros(60, np.int32)
+    oracle.lib().orc_shell_sort_select(60, oracle.ptr(v), 5, oracle.ptr(order))
+    idx = list(range(60))
+    gap = 30
+    while gap >= 1:
+        for i in range(gap, 60):
+            t = idx[i]
+            j = i - gap
+            while j >= 0 and v[idx[j]] > v[t]:
+                idx[j + gap] = idx[j]
+                j -= gap
+            idx[j + gap] = t
+        gap //= 2
+    assert list(order) == idx
+
+
+def test_brittle_breaks_largest(oracle):
+    m, _ = hexm(oracle, 8.0, 7.0, plmode=6, nbreak=4, critical_bstrain=1e-3)
+    rng = np.random.default_rng(2)
+    live = m.nbr >= 0
+    m.dL[:] = 0
+    m.dL[live] = m.L0[live] * rng.uniform(0, 2e-3, live.sum())
+    strain = m.dL / np.where(live, m.L0, 1)
+    cand = np.argwhere(live & (strain >= 1e-3))
+    k = oracle.lib().orc_damage_brittle(m.p, None, None, 0)
+    assert k == len(cand)
+    broken = np.argwhere(live & (m.dmg_broken == 0))
+    assert len(broken) == 4
+    top = np.sort(strain[live])[-4:]
+    assert np.allclose(np.sort(strain[tuple(broken.T)]), top)
