@@ -1,0 +1,192 @@
+/*
+ * lpmx.h -- C-ABI of the MI355X device layer behind the drop-in lpm.h API.
+ *
+ * Plain pointers and sizes only.  One process drives one GPU (device 0 unless
+ * LPMX_DEVICE is set); the library keeps a single device-resident model, like the
+ * reference keeps its state in globals, and is not reentrant.  Every entry point
+ * returns LPMX_OK (0) or a negative status; lpmx_last_error() describes the last
+ * failure.  No entry point falls back to host computation: without a usable
+ * gfx950 device the calls fail with LPMX_ERR_NOGPU.
+ *
+ * Array layouts at this boundary are the reference's, flattened row-major:
+ * positions [n][3], bond arrays [n][nneighbors], DOF vectors [n*dim],
+ * per-particle state [n].  Inside the device the bond arrays are stored as
+ * slot planes ([slot][particle]) and the tangent as 3x3 (2x2) blocks in a
+ * sliced-ELL layout of 64 rows per slice; the getters transpose back.
+ */
+#ifndef LPMX_H
+#define LPMX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum
+{
+    LPMX_OK = 0,
+    LPMX_ERR_HIP = -1,      /* a HIP runtime call failed */
+    LPMX_ERR_ARG = -2,      /* invalid argument or size */
+    LPMX_ERR_STATE = -3,    /* call out of order (no model, no pattern, ...) */
+    LPMX_ERR_CAPACITY = -4, /* a neighbour or conn list exceeded its capacity */
+    LPMX_ERR_NOGPU = -5,    /* no gfx950 device available */
+    LPMX_ERR_NOCONV = -6    /* a solve or Newton loop hit its iteration cap */
+};
+
+const char *lpmx_last_error(void);
+
+/* ---- model --------------------------------------------------------------------- */
+typedef struct lpmx_model_desc
+{
+    int dim;                  /* 2 or 3 */
+    int lattice;              /* 0 square, 1 hexagon, 2 SC, 3 FCC, 4 BCC */
+    int n;                    /* particles */
+    int nneighbors, nneighbors1, nneighbors2, nneighbors_AFEM;
+    double radius, neighbor1_cutoff, neighbor2_cutoff, particle_volume;
+} lpmx_model_desc;
+
+typedef struct lpmx_params
+{
+    int plmode;               /* 0 J2 mixed-linear, 6 elastic-brittle */
+    double J2_H, J2_xi;
+    double damage_L, damage_threshold, damagec_A, critical_bstrain;
+    int nbreak;
+} lpmx_params;
+
+/* Create the device model from positions (xyz = xyz_initial = xyz_in, [n][3]). */
+int lpmx_model_create(const lpmx_model_desc *desc, const double *xyz_in);
+int lpmx_model_destroy(void);
+int lpmx_set_params(const lpmx_params *p);
+
+/* ---- setup on the device ------------------------------------------------------- */
+/* searchNormalNeighbor: cell-list build of the bond lists (reference neighbor.c:9-46). */
+int lpmx_neighbor_build(void);
+/* searchAFEMNeighbor: conn sets, tangent pattern; *len_K = K_pointer[n][1]. */
+int lpmx_connectivity_build(int64_t *len_K);
+/* Particle types, C11/C12/C44 per type, yield stress, optional per-particle modulus
+ * factor (NULL: none), then calcKnTv on the device. */
+int lpmx_set_materials(int ntype, const int *type, const double *Ce, const double *sigmay, const double *emod);
+/* Optional per-bond critical stretch [n][nneighbors] of the brittle law (NULL: global). */
+int lpmx_set_bond_critical(const double *crit);
+
+/* ---- field transfer (reference layouts) ------------------------------------------ */
+enum lpmx_field
+{
+    LPMX_F_XYZ = 0, LPMX_F_XYZ_INITIAL, LPMX_F_XYZ_TEMP,                 /* double [n][3] */
+    LPMX_F_NEIGHBORS, LPMX_F_NSIGN,                                      /* int [n][nb] */
+    LPMX_F_NB, LPMX_F_NB_INITIAL, LPMX_F_NB_CONN, LPMX_F_TYPE, LPMX_F_PL_FLAG, /* int [n] */
+    LPMX_F_CONN,                                                         /* int [n][nafem+1] */
+    LPMX_F_K_POINTER,                                                    /* int [n+1][2] */
+    LPMX_F_DISTANCE_INITIAL, LPMX_F_CSX_INITIAL, LPMX_F_CSY_INITIAL, LPMX_F_CSZ_INITIAL,
+    LPMX_F_KN, LPMX_F_TV, LPMX_F_DL, LPMX_F_DL_AVE, LPMX_F_DDL, LPMX_F_DDLP, LPMX_F_F, LPMX_F_F_TEMP,
+    LPMX_F_CSX, LPMX_F_CSY, LPMX_F_CSZ, LPMX_F_BOND_STRESS, LPMX_F_DISTANCE,
+    LPMX_F_DAMAGE_BROKEN, LPMX_F_DAMAGE_W,                               /* double [n][nb] */
+    LPMX_F_DLP0, LPMX_F_DLP1, LPMX_F_DLP2, LPMX_F_DAMAGE_D0, LPMX_F_DAMAGE_D1, /* double [n][nb] */
+    LPMX_F_DL_TOTAL, LPMX_F_TDL_TOTAL, LPMX_F_DDL_TOTAL, LPMX_F_TDDL_TOTAL,   /* double [n][2] */
+    LPMX_F_J2_BETA0, LPMX_F_J2_BETA1, LPMX_F_J2_BETA2, LPMX_F_STRESS,    /* double [n][6] */
+    LPMX_F_J2_ALPHA0, LPMX_F_J2_ALPHA1, LPMX_F_J2_ALPHA2,                 /* double [n] */
+    LPMX_F_J2_DLAMBDA, LPMX_F_J2_STRESSEQ, LPMX_F_J2_STRESSM, LPMX_F_J2_TRIAXIALITY,
+    LPMX_F_DAMAGE_NONLOCAL0, LPMX_F_DAMAGE_NONLOCAL1, LPMX_F_DAMAGE_VISUAL, LPMX_F_SIGMAY,
+    LPMX_F_PIN,                                                          /* double [n*3] */
+    LPMX_F_PEX, LPMX_F_PEX_TEMP, LPMX_F_RESIDUAL, LPMX_F_DISP,           /* double [n*dim] */
+    LPMX_F_DISPBC_INDEX, LPMX_F_FIX_INDEX,                               /* int [n*dim] */
+    LPMX_F_COUNT
+};
+
+int lpmx_get(int field, void *host, int64_t count);
+int lpmx_put(int field, const void *host, int64_t count);
+
+/* ---- tangent stiffness ------------------------------------------------------------ */
+/* Analytic elastic tangent (the Jacobian the reference differences), symmetrised as
+ * the reference's 0.5(A + A^T), into the device block matrix. */
+int lpmx_tangent(void);
+/* The device matrix in the reference's 1-based upper CSR (IK [n*dim+1], JK, K [len_K]). */
+int lpmx_export_upper_csr(int *IK, int *JK, double *K);
+/* Replace the device matrix values by a host upper CSR of the same pattern. */
+int lpmx_import_upper_csr(const double *K);
+/* setDispBC_stiffnessUpdate: zero constrained rows/columns, diagonal = |diag|_2. */
+int lpmx_bc_eliminate(void);
+
+/* ---- linear solve ------------------------------------------------------------------- */
+enum { LPMX_FP64 = 0, LPMX_MIXED = 1 };
+/* K disp = residual by block-Jacobi PCG, stopping when |r|^2 <= rtol2 |r0|^2 + atol2
+ * (the reference's dcg test) or after maxit iterations; then xyz += disp.
+ * LPMX_MIXED runs FP32 values, vectors and compensated dots inside FP64 iterative
+ * refinement.  *iters gets the inner CG iterations, *rr the final |r|^2. */
+int lpmx_solve(int precision, double rtol2, double atol2, int maxit, int *iters, double *rr);
+/* The same with the mixed form's inner FP32 target (squared-norm ratio per
+ * refinement pass) and pass cap; *status: 1 converged, 2 iteration cap,
+ * 3 breakdown (p.Kp <= 0).  Returns LPMX_OK whatever the status. */
+int lpmx_solve_ex(int precision, double rtol2, double atol2, int maxit, double inner_rtol2, int ir_maxit,
+                  int *iters, double *rr, int *status);
+
+/* ---- constitutive, residual, boundary conditions, damage ------------------------------ */
+int lpmx_computedL(void);
+/* computeBondForceGeneral(mode): 0 J2, 4 predictor, 6 elastic; then stress and
+ * switchStateV(2).  The _range form runs the per-particle law for [first, first+count)
+ * only (no stress, no state switch). */
+int lpmx_bond_force(int mode);
+int lpmx_bond_force_range(int mode, int first, int count);
+int lpmx_compute_stress(void);
+int lpmx_switch_state(int flag);
+/* updateRR: residual and reaction on the device; norms returned (NULL allowed). */
+int lpmx_update_rr(double *norm_residual, double *norm_reaction, int *nreaction);
+int lpmx_get_reaction_force(double *host, int count);
+
+typedef struct lpmx_dispbc { int type; char flag; double step; } lpmx_dispbc;
+typedef struct lpmx_forcebc { int type; double step[3]; } lpmx_forcebc;
+int lpmx_set_disp_bc(int nbd, const lpmx_dispbc *bc);
+int lpmx_set_force_bc(int nbf, const lpmx_forcebc *bc);
+
+/* updateDamageGeneral for plmode 0 (nonlocal particle-wise ductile) and 6 (brittle);
+ * *broken gets the reference's return value.  Broken pairs of the brittle law can be
+ * read back with lpmx_get_broken (up to cap pairs). */
+int lpmx_damage(int plmode, int *broken);
+int lpmx_get_broken(int *bi, int *bj, int cap, int *count);
+int lpmx_update_crack(void);
+/* writeReaction / writeDisp values for particles of type t1 along axis 'x','y','z'. */
+int lpmx_reaction(char axis, int t1, double *force, double *mean_disp);
+
+/* ---- device-resident load-step loop (reference src/lpmc_project.c:382-546) -------- */
+typedef struct lpmx_solver
+{
+    int precision;          /* LPMX_FP64 or LPMX_MIXED */
+    double rtol2, atol2;    /* CG test on squared norms (reference: 1e-8, 1e-12) */
+    int maxit;              /* CG cap (reference: n*dim) */
+    double newton_tol;      /* reference TOLITER */
+    int newton_maxit;       /* reference MAXITER */
+    double inner_rtol2;     /* mixed form: FP32 inner solve target on squared norms */
+    int ir_maxit;           /* mixed form: FP64 refinement passes per solve */
+} lpmx_solver;
+
+typedef struct lpmx_step_stats
+{
+    double reaction, disp;  /* writeReaction / writeDisp values after the step */
+    int newton_iters, cg_iters, resolves, broken, ir_passes, status;
+    double res_norm, tol_mult;
+    double t_tangent, t_solve, t_force, t_other, t_total; /* seconds */
+} lpmx_step_stats;
+
+int lpmx_run_steps(const lpmx_solver *s, int nsteps, const int *nbd, const lpmx_dispbc *dbc,
+                   const int *nbf, const lpmx_forcebc *fbc, int maxbc, char out_axis, int out_type,
+                   lpmx_step_stats *out);
+
+/* ---- measurement ---------------------------------------------------------------------- */
+/* Time `reps` launches of the block SpMV y = K x (FP64 or FP32) with HIP events on the
+ * library's stream; *avg_ms gets the mean duration, *bytes the algorithmic bytes per
+ * launch (nnzb*(9*s_val+4) + 4*(n+1) + 2*n*dim*s_vec). */
+int lpmx_bench_spmv(int precision, int reps, double *avg_ms, double *bytes);
+/* Counters of the device matrix: block rows, stored blocks (nnzb), slots incl. padding. */
+int lpmx_matrix_info(int64_t *nnzb, int64_t *padded_slots, int *nslices);
+/* y = K x with the device matrix (FP64 or FP32 values), vectors of n*dim doubles. */
+int lpmx_spmv(int precision, const double *x, double *y);
+/* HIP-event timing of the solver's SpMV launches (up to 64 per solve) while on;
+ * switching on resets the totals. */
+void lpmx_profile_spmv(int on);
+void lpmx_profile_spmv_read(double *total_ms, int64_t *launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,396 @@
+"""Python harness over liblpm.so (the drop-in lpm.h API and the lpmx device layer).
+
+Host code of the product is C; this module is the thin test/benchmark harness: it
+loads the in-tree library, reads and writes the reference's globals through ctypes,
+builds the benchmark configurations through the reference API (createCuboid,
+initMatrices, searchNormalNeighbor, ...), and wraps the lpmx entry points.  It never
+computes anything itself; without the built library or a gfx950 device the calls
+raise.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liblpm.so")
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+LPMX_FP64, LPMX_MIXED = 0, 1
+FIELDS = """XYZ XYZ_INITIAL XYZ_TEMP NEIGHBORS NSIGN NB NB_INITIAL NB_CONN TYPE PL_FLAG CONN K_POINTER
+DISTANCE_INITIAL CSX_INITIAL CSY_INITIAL CSZ_INITIAL KN TV DL DL_AVE DDL DDLP F F_TEMP CSX CSY CSZ BOND_STRESS
+DISTANCE DAMAGE_BROKEN DAMAGE_W DLP0 DLP1 DLP2 DAMAGE_D0 DAMAGE_D1 DL_TOTAL TDL_TOTAL DDL_TOTAL TDDL_TOTAL
+J2_BETA0 J2_BETA1 J2_BETA2 STRESS J2_ALPHA0 J2_ALPHA1 J2_ALPHA2 J2_DLAMBDA J2_STRESSEQ J2_STRESSM
+J2_TRIAXIALITY DAMAGE_NONLOCAL0 DAMAGE_NONLOCAL1 DAMAGE_VISUAL SIGMAY PIN PEX PEX_TEMP RESIDUAL DISP
+DISPBC_INDEX FIX_INDEX""".split()
+F = {name: k for k, name in enumerate(FIELDS)}
+INT_FIELDS = {"NEIGHBORS", "NSIGN", "NB", "NB_INITIAL", "NB_CONN", "TYPE", "PL_FLAG", "CONN", "K_POINTER",
+              "DISPBC_INDEX", "FIX_INDEX"}
+
+
+class LpmxError(RuntimeError):
+    pass
+
+
+class ModelDesc(C.Structure):
+    _fields_ = [("dim", C.c_int), ("lattice", C.c_int), ("n", C.c_int), ("nneighbors", C.c_int),
+                ("nneighbors1", C.c_int), ("nneighbors2", C.c_int), ("nneighbors_AFEM", C.c_int),
+                ("radius", C.c_double), ("neighbor1_cutoff", C.c_double), ("neighbor2_cutoff", C.c_double),
+                ("particle_volume", C.c_double)]
+
+
+class Params(C.Structure):
+    _fields_ = [("plmode", C.c_int), ("J2_H", C.c_double), ("J2_xi", C.c_double), ("damage_L", C.c_double),
+                ("damage_threshold", C.c_double), ("damagec_A", C.c_double), ("critical_bstrain", C.c_double),
+                ("nbreak", C.c_int)]
+
+
+class DispBC(C.Structure):
+    _fields_ = [("type", C.c_int), ("flag", C.c_char), ("step", C.c_double)]
+
+
+class ForceBC(C.Structure):
+    _fields_ = [("type", C.c_int), ("step", C.c_double * 3)]
+
+
+class Solver(C.Structure):
+    _fields_ = [("precision", C.c_int), ("rtol2", C.c_double), ("atol2", C.c_double), ("maxit", C.c_int),
+                ("newton_tol", C.c_double), ("newton_maxit", C.c_int), ("inner_rtol2", C.c_double),
+                ("ir_maxit", C.c_int)]
+
+
+class StepStats(C.Structure):
+    _fields_ = [("reaction", C.c_double), ("disp", C.c_double), ("newton_iters", C.c_int), ("cg_iters", C.c_int),
+                ("resolves", C.c_int), ("broken", C.c_int), ("ir_passes", C.c_int), ("status", C.c_int),
+                ("res_norm", C.c_double), ("tol_mult", C.c_double), ("t_tangent", C.c_double),
+                ("t_solve", C.c_double), ("t_force", C.c_double), ("t_other", C.c_double), ("t_total", C.c_double)]
+
+
+# reference API entry points declared in include/*.h (argument types for ctypes)
+_API = {
+    "createCuboid": [], "initMatrices": [], "searchNormalNeighbor": [], "searchAFEMNeighbor": [],
+    "calcKnTv": [], "computedL": [], "computeStress": [], "updateRR": [], "updateCrack": [],
+    "calcStiffness3DFiniteDifference": [C.c_int], "calcStiffness2DFiniteDifference": [C.c_int],
+    "setDispBC_stiffnessUpdate3D": [], "setDispBC_stiffnessUpdate2D": [], "solverCG": [],
+    "switchStateV": [C.c_int], "computeBondForceGeneral": [C.c_int, C.c_int],
+    "setTypeRect": [C.c_double] * 6 + [C.c_int], "setTypeFullNeighbor": [C.c_int],
+    "moveParticle": [_dp], "copyDouble2D": [C.c_void_p, C.c_void_p, C.c_int, C.c_int],
+    "allocInt1D": [C.c_int, C.c_int], "allocDouble1D": [C.c_int, C.c_double],
+    "allocDouble2D": [C.c_int, C.c_int, C.c_double], "cblas_dnrm2": [C.c_int, _dp, C.c_int],
+    "updateDamageGeneral": [C.c_char_p, C.c_int, C.c_int], "slipSysDefine3D": [],
+}
+_RES = {"searchAFEMNeighbor": C.c_int, "allocInt1D": _ip, "allocDouble1D": _dp,
+        "allocDouble2D": C.POINTER(_dp), "cblas_dnrm2": C.c_double, "updateDamageGeneral": C.c_int}
+
+_lib = None
+
+
+def lib():
+    """The loaded liblpm.so with argument types set; raises if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise LpmxError(f"{LIB_PATH} is not built (run __graft_entry__.build())")
+    L = C.CDLL(LIB_PATH)
+    for name, args in _API.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = _RES.get(name)
+    L.lpmx_last_error.restype = C.c_char_p
+    sig = {
+        "lpmx_model_create": [C.POINTER(ModelDesc), _dp], "lpmx_model_destroy": [],
+        "lpmx_set_params": [C.POINTER(Params)], "lpmx_neighbor_build": [],
+        "lpmx_connectivity_build": [C.POINTER(C.c_int64)],
+        "lpmx_set_materials": [C.c_int, _ip, _dp, _dp, _dp], "lpmx_set_bond_critical": [_dp],
+        "lpmx_get": [C.c_int, C.c_void_p, C.c_int64], "lpmx_put": [C.c_int, C.c_void_p, C.c_int64],
+        "lpmx_tangent": [], "lpmx_export_upper_csr": [_ip, _ip, _dp], "lpmx_import_upper_csr": [_dp],
+        "lpmx_bc_eliminate": [], "lpmx_solve": [C.c_int, C.c_double, C.c_double, C.c_int, _ip, _dp],
+        "lpmx_computedL": [], "lpmx_bond_force": [C.c_int], "lpmx_bond_force_range": [C.c_int, C.c_int, C.c_int],
+        "lpmx_compute_stress": [], "lpmx_switch_state": [C.c_int],
+        "lpmx_update_rr": [_dp, _dp, _ip], "lpmx_get_reaction_force": [_dp, C.c_int],
+        "lpmx_set_disp_bc": [C.c_int, C.POINTER(DispBC)], "lpmx_set_force_bc": [C.c_int, C.POINTER(ForceBC)],
+        "lpmx_damage": [C.c_int, _ip], "lpmx_get_broken": [_ip, _ip, C.c_int, _ip], "lpmx_update_crack": [],
+        "lpmx_reaction": [C.c_char, C.c_int, _dp, _dp],
+        "lpmx_run_steps": [C.POINTER(Solver), C.c_int, _ip, C.POINTER(DispBC), _ip, C.POINTER(ForceBC), C.c_int,
+                           C.c_char, C.c_int, C.POINTER(StepStats)],
+        "lpmx_bench_spmv": [C.c_int, C.c_int, _dp, _dp],
+        "lpmx_matrix_info": [C.POINTER(C.c_int64), C.POINTER(C.c_int64), _ip],
+        "lpmx_spmv": [C.c_int, _dp, _dp],
+    }
+    for name, args in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = C.c_int
+    L.lpmx_profile_spmv.argtypes = [C.c_int]
+    L.lpmx_profile_spmv.restype = None
+    L.lpmx_profile_spmv_read.argtypes = [_dp, C.POINTER(C.c_int64)]
+    L.lpmx_profile_spmv_read.restype = None
+    L.lpm_ext_set_modulus.argtypes = [_dp]
+    L.lpm_ext_set_bond_critical.argtypes = [_dp]
+    L.lpm_ext_set_solver.argtypes = [C.POINTER(Solver)]
+    _lib = L
+    return L
+
+
+def check(rc, what=""):
+    if rc != 0:
+        raise LpmxError(f"{what}: status {rc}: {lib().lpmx_last_error().decode()}")
+
+
+def ptr(a):
+    if a is None:
+        return None
+    if a.dtype == np.float64:
+        return a.ctypes.data_as(_dp)
+    if a.dtype == np.int32:
+        return a.ctypes.data_as(_ip)
+    raise TypeError(a.dtype)
+
+
+# ---------------------------------------------------------------- globals --------
+def gint(name):
+    return C.c_int.in_dll(lib(), name)
+
+
+def gdouble(name):
+    return C.c_double.in_dll(lib(), name)
+
+
+def gptr(name, depth, ctype=C.c_double):
+    t = ctype
+    for _ in range(depth):
+        t = C.POINTER(t)
+    return t.in_dll(lib(), name)
+
+
+def read1d(name, count, dtype=np.float64):
+    ct = C.c_double if dtype == np.float64 else C.c_int
+    p = gptr(name, 1, ct)
+    return np.ctypeslib.as_array(p, shape=(count,)).copy()
+
+
+def read2d(name, rows, cols, dtype=np.float64):
+    ct = C.c_double if dtype == np.float64 else C.c_int
+    p = gptr(name, 2, ct)
+    out = np.empty((rows, cols), dtype)
+    for i in range(rows):
+        out[i] = np.ctypeslib.as_array(p[i], shape=(cols,))
+    return out
+
+
+def write1d(name, values):
+    arr = np.asarray(values)
+    ct = C.c_double if arr.dtype == np.float64 else C.c_int
+    p = gptr(name, 1, ct)
+    for k, v in enumerate(arr):
+        p[k] = v
+
+
+def field(name, count=None):
+    """A device field in the reference layout (lpmx_get)."""
+    dt = np.int32 if name in INT_FIELDS else np.float64
+    if count is None:
+        raise ValueError("count required")
+    out = np.zeros(count, dt)
+    check(lib().lpmx_get(F[name], out.ctypes.data_as(C.c_void_p), count), f"lpmx_get {name}")
+    return out
+
+
+# ----------------------------------------------------- model construction --------
+def elastic_constants(E, nu, plane_stress=False):
+    """C11, C12, C44 as the reference driver forms them (src/lpmc_project.c:188-194)."""
+    if plane_stress:
+        return (E / (1.0 - nu) / (1.0 + nu), E * nu / (1.0 - nu) / (1.0 + nu), E / 2.0 / (1.0 + nu))
+    return (E * (1.0 - nu) / (1.0 + nu) / (1.0 - 2.0 * nu), E * nu / (1.0 + nu) / (1.0 - 2.0 * nu),
+            E / 2.0 / (1.0 + nu))
+
+
+def make_lattice(lattice, radius, box, angles=(0.0, 0.0, 0.0), eulerflag=0):
+    """createCuboid + initMatrices + xyz_initial = xyz through the reference API."""
+    L = lib()
+    gint("lattice").value = lattice
+    gint("dim").value = 2 if lattice < 2 else 3
+    gdouble("radius").value = radius
+    gint("eulerflag").value = eulerflag
+    for k, a in enumerate(("angle1", "angle2", "angle3")):
+        gdouble(a).value = angles[k]
+    b = (C.c_double * 6).in_dll(L, "box")
+    for k in range(6):
+        b[k] = box[k]
+    gdouble("box_x").value = box[1] - box[0]
+    gdouble("box_y").value = box[3] - box[2]
+    gdouble("box_z").value = box[5] - box[4]
+    L.createCuboid()
+    mv = np.zeros(3)
+    L.moveParticle(ptr(mv))
+    L.initMatrices()
+    n = gint("nparticle").value
+    L.copyDouble2D(gptr("xyz_initial", 2), gptr("xyz", 2), n, 3)
+    return n
+
+
+def positions():
+    n = gint("nparticle").value
+    return read2d("xyz", n, 3)
+
+
+def set_gptr(name, value, ctype, depth):
+    """Point the global `name` (a depth-level pointer to ctype) at `value`."""
+    t = ctype
+    for _ in range(depth):
+        t = C.POINTER(t)
+    g = t.in_dll(lib(), name)
+    C.cast(C.addressof(g), C.POINTER(t))[0] = value
+
+
+def set_types(types, ntype):
+    """type[] (allocated through allocInt1D, as the reference driver does) and ntype."""
+    n = gint("nparticle").value
+    p = lib().allocInt1D(n, 0)
+    for k, t in enumerate(np.asarray(types, np.int32)):
+        p[k] = int(t)
+    set_gptr("type", p, C.c_int, 1)
+    gint("ntype").value = ntype
+
+
+def set_materials(Ce_rows, sigmay):
+    """Ce[ntype][3] and sigmay[n] through the reference allocators."""
+    L = lib()
+    nt = len(Ce_rows)
+    ce = L.allocDouble2D(nt, 3, 0.0)
+    for t, row in enumerate(Ce_rows):
+        for c in range(3):
+            ce[t][c] = row[c]
+    set_gptr("Ce", ce, C.c_double, 2)
+    n = gint("nparticle").value
+    sy = L.allocDouble1D(n, 0.0)
+    sv = np.ascontiguousarray(np.broadcast_to(np.asarray(sigmay, np.float64), (n,)))
+    C.memmove(sy, sv.ctypes.data, 8 * n)
+    set_gptr("sigmay", sy, C.c_double, 1)
+
+
+def set_params(plmode=0, J2_H=0.0, J2_xi=0.0, damage_L=0.5, damage_threshold=0.9, damagec_A=0.0,
+               critical_bstrain=1e-2, nbreak=20):
+    gint("plmode").value = plmode
+    gdouble("J2_H").value = J2_H
+    gdouble("J2_xi").value = J2_xi
+    gdouble("damage_L").value = damage_L
+    gdouble("damage_threshold").value = damage_threshold
+    gdouble("damagec_A").value = damagec_A
+    gdouble("critical_bstrain").value = critical_bstrain
+    gint("nbreak").value = nbreak
+
+
+def solver(precision=LPMX_FP64, rtol2=1e-8, atol2=1e-12, maxit=None, newton_tol=1e-4, newton_maxit=100,
+           inner_rtol2=1e-8, ir_maxit=12):
+    n = gint("nparticle").value * gint("dim").value
+    return Solver(precision, rtol2, atol2, maxit if maxit is not None else n, newton_tol, newton_maxit,
+                  inner_rtol2, ir_maxit)
+
+
+def bc_tables(steps):
+    """steps: list of (disp_entries, force_entries); disp (type, 'x'|'y'|'z', step),
+    force (type, (fx, fy, fz))."""
+    maxbc = max(1, max(max(len(d), len(f)) for d, f in steps))
+    ns = len(steps)
+    dbc = (DispBC * (ns * maxbc))()
+    fbc = (ForceBC * (ns * maxbc))()
+    nbd = np.zeros(ns, np.int32)
+    nbf = np.zeros(ns, np.int32)
+    for s, (dl, fl) in enumerate(steps):
+        nbd[s], nbf[s] = len(dl), len(fl)
+        for k, (t, flag, st) in enumerate(dl):
+            dbc[s * maxbc + k] = DispBC(t, flag.encode(), st)
+        for k, (t, st) in enumerate(fl):
+            fbc[s * maxbc + k].type = t
+            for c in range(3):
+                fbc[s * maxbc + k].step[c] = st[c]
+    return maxbc, nbd, dbc, nbf, fbc
+
+
+def run_steps(steps, solv, out_axis="z", out_type=1):
+    """lpmx_run_steps over the BC tables; returns the per-step statistics."""
+    maxbc, nbd, dbc, nbf, fbc = bc_tables(steps)
+    out = (StepStats * len(steps))()
+    rc = lib().lpmx_run_steps(C.byref(solv), len(steps), ptr(nbd), dbc, ptr(nbf), fbc, maxbc, out_axis.encode(),
+                              out_type, out)
+    return rc, list(out)
+
+
+# ------------------------------------------------------------ configurations -----
+SC_BOX = {8000: 19.035, 216000: 59.3, 1000000: 99.195}
+
+
+def sc_box_for(k):
+    """Box edge L of an SC lattice (radius 0.5) with k particles per edge.  x starts
+    at 0, y and z at -a + integer (a = sqrt(3) L, the box diagonal), so besides
+    k-1 <= L < k the first y/z value ceil(a) - a must not exceed L - (k-1)."""
+    for m in range(1, 1000):
+        L = k - 1 + m / 1000.0
+        a = np.sqrt(3.0) * L
+        f = np.ceil(a) - a
+        if 1e-6 < f <= L - (k - 1) - 1e-9:
+            return L
+    raise ValueError(k)
+
+
+def sc_types(x, radius=0.5):
+    """Types of the uniaxial SC configurations: 1 top face, 2 bottom face, 3 the
+    bottom corner at minimum x and y (fully fixed), 0 elsewhere."""
+    z = x[:, 2]
+    t = np.zeros(len(x), np.int32)
+    t[z >= z.max() - 0.6 * radius] = 1
+    t[z <= z.min() + 0.6 * radius] = 2
+    bot = np.where(t == 2)[0]
+    corner = bot[np.lexsort((x[bot, 1], x[bot, 0]))[0]]
+    t[corner] = 3
+    return t
+
+
+def sc_uniaxial_steps(height, strains):
+    """Displacement steps: bottom (type 2) fixed in z, corner (type 3) fixed in x, y,
+    z, top (type 1) moved in z by the strain increments times the height."""
+    steps = []
+    prev = 0.0
+    for e in strains:
+        du = (e - prev) * height
+        prev = e
+        steps.append(([(2, "z", 0.0), (3, "x", 0.0), (3, "y", 0.0), (3, "z", 0.0), (1, "z", du)], []))
+    return steps
+
+
+def uniform_modulus(n, seed, amp=0.05):
+    """Per-particle modulus factor 1 + amp*U(-1, 1); U from the splitmix64 stream of
+    `seed` (element i is the (i+1)-th output), 53-bit mantissa."""
+    gamma = np.uint64(0x9E3779B97F4A7C15)
+    with np.errstate(over="ignore"):
+        z = np.uint64(seed) + gamma * np.arange(1, n + 1, dtype=np.uint64)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    u = (z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+    return 1.0 + amp * (2.0 * u - 1.0)
+
+
+def build_sc_model(L, E=200e3, nu=0.3, sigmay=200.0, H=2000.0, emod=None, damage=True):
+    """SC elastoplastic model of the uniaxial configurations through the reference
+    API, on the device: lattice, lists, pattern, types, materials, bond constants."""
+    Lb = lib()
+    n = make_lattice(2, 0.5, [0.0, L, 0.0, L, 0.0, L])
+    Lb.searchNormalNeighbor()
+    Lb.searchAFEMNeighbor()
+    x = positions()
+    t = sc_types(x)
+    set_types(t, 4)
+    c = elastic_constants(E, nu)
+    set_materials([c] * 4, sigmay)
+    set_params(plmode=0, J2_H=H, J2_xi=0.0, damage_L=0.5, damage_threshold=0.9, damagec_A=0.0,
+               critical_bstrain=1e-2, nbreak=20)
+    if emod is not None:
+        Lb.lpm_ext_set_modulus(ptr(emod))
+    Lb.calcKnTv()
+    Lb.computedL()
+    return n, x, t, c

@@ -1,0 +1,153 @@
+// Internal header of the MI355X device layer: the device-resident model, launch
+// helpers and deterministic reductions.  Not part of the C-ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "lpmx.h"
+
+#define LPMX_SLICE 64           // rows per slice of the block matrix = one wavefront
+#define LPMX_BLOCK 256          // default work-group size
+#define LPMX_EPS 1e-6           // the reference's EPS (include/lpm.h:42)
+#define LPMX_MAXNB 32           // largest neighbour capacity handled (SC/FCC: 18)
+#define LPMX_MAXCONN 64         // largest conn row handled (SC/FCC: 61, hex: 31)
+#define LPMX_MAXBC 20           // the reference's MAXSMALL BC entries per step
+
+// All device pointers and sizes of the model, passed by value to every kernel.
+// Bond arrays are slot planes: element (particle i, slot j) at [j * n + i].
+// Per-particle vectors with components are planes too ([c * n + i]), except the
+// AoS position and force vectors xyz/Pin ([i * 3 + c]) and the DOF vectors
+// residual/disp/Pex ([i * dim + c]) that keep the reference's interleaving.
+struct DV
+{
+    int n, dim, nb, nb1, nb2, cm, lattice, d2;
+    double radius, cut1, cut2, volume;
+    // parameters
+    int plmode, nbreak;
+    double H, xi, dmgL, dthr, dA, crit0;
+    // positions
+    double *x, *x0, *xt;
+    // per particle
+    int *type, *nbcur, *nb0, *pl_flag, *nb_conn, *kcnt;
+    double *sigmay, *c44, *kt; // kt: per-particle bond constants [3][n]
+    // bond planes
+    int *nbr, *nsign, *rev, *opp;
+    double *L0, *cs0x, *cs0y, *cs0z, *Kn, *Tv, *db, *dw, *D0, *D1;
+    double *dLp0, *dLp1, *dLp2, *dL, *dLave, *ddLp, *ddL, *F, *Ft, *csx, *csy, *csz, *bstress, *crit, *dist;
+    // per-particle planes
+    double *A, *Bt, *dA_, *dB_;                 // [2][n]: dL_total, TdL_total, ddL_total, TddL_total
+    double *beta0, *beta1, *beta2;              // [6][n]
+    double *alpha0, *alpha1, *alpha2, *beq0, *beq1, *beq2;
+    double *dlam, *stress, *seq, *sm, *triax;   // stress: [6][n]
+    double *dnl0, *dnl1, *dloc0, *dloc1, *dvis;
+    double *Pin, *Pex, *Pext, *res, *disp;
+    int *dispBC, *fix;
+    // block matrix: sliced ELL, 64 rows per slice, slot k of slice s at sptr[s]+k
+    int nslices;
+    int64_t nslots, nnzb;
+    int *sptr, *col, *tpos, *dslot, *conn;      // conn: [cm][n] planes
+    int64_t *rowptr;                            // row-major block offsets of J
+    double *K, *J, *Minv;
+    float *K32, *Minv32;
+    // static nonlocal-damage neighbourhoods (index order) and weights
+    int *dptr, *didx;
+    double *dwt;
+};
+
+extern DV g;
+extern hipStream_t g_stream;
+
+// ------------------------------------------------------------------ errors -------
+int lpmx_fail(int code, const char *fmt, ...);
+int lpmx_hip_fail(hipError_t e, const char *what, const char *file, int line);
+#define HIPCHK(call)                                                        \
+    do                                                                      \
+    {                                                                       \
+        hipError_t e_ = (call);                                             \
+        if (e_ != hipSuccess)                                               \
+            return lpmx_hip_fail(e_, #call, __FILE__, __LINE__);            \
+    } while (0)
+#define LAUNCHCHK() HIPCHK(hipGetLastError())
+#define RETCHK(x)                                                           \
+    do                                                                      \
+    {                                                                       \
+        int r_ = (x);                                                       \
+        if (r_ != LPMX_OK)                                                  \
+            return r_;                                                      \
+    } while (0)
+
+int lpmx_require_model(void);
+int lpmx_require_pattern(void);
+static inline int nblocks(int64_t n, int bs = LPMX_BLOCK) { return (int)((n + bs - 1) / bs); }
+
+// -------------------------------------------------------- deterministic sums -----
+// Wavefront sum in a fixed butterfly order (64 lanes).
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1)
+        v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Work-group sum (blockDim.x multiple of 64, at most 1024): wave sums, then the
+// wave totals in wave order.  All threads get the result.
+template <typename T>
+__device__ __forceinline__ T block_sum(T v, T *sh)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    v = wave_sum(v);
+    __syncthreads();
+    if (lane == 0)
+        sh[w] = v;
+    __syncthreads();
+    T t = 0;
+    for (int k = 0; k < nw; k++)
+        t += sh[k];
+    return t;
+}
+
+// True in the work-group that finishes last; its partials[] reads see every
+// other group's writes.  The counter is reset by the caller (last group).
+__device__ __forceinline__ bool last_group(unsigned *counter)
+{
+    __shared__ bool am_last;
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0)
+        am_last = atomicAdd(counter, 1u) == gridDim.x - 1;
+    __syncthreads();
+    return am_last;
+}
+
+__device__ __forceinline__ double coherent_load(const double *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Sum of parts[0..np) by one work-group in a fixed order (strided per thread,
+// then block_sum), readable coherently after last_group().
+__device__ __forceinline__ double ordered_total(const double *parts, int np, double *sh)
+{
+    double s = 0.0;
+    for (int k = threadIdx.x; k < np; k += blockDim.x)
+        s += coherent_load(parts + k);
+    return block_sum(s, sh);
+}
+
+// ------------------------------------------------------ host-side reductions -----
+// Deterministic device sums of up to 4 per-element values produced by a functor;
+// used for norms and small outputs outside the CG inner loop.
+int lpmx_dev_sum(const double *src, int64_t count, double *out_host);
+int lpmx_dev_sumsq(const double *src, int64_t count, double *out_host);
+int lpmx_dev_scan_int(const int *in, int *out_excl, int count, int *total_host);
+
+// scratch: a device buffer reused by host-side helpers (grown on demand)
+double *lpmx_scratch(int64_t bytes);
+unsigned *lpmx_counters(void); // 64 zero-initialised counters for last_group()
+
+// solver workspace release (called from lpmx_model_destroy)
+void lpmx_solver_free(void);
+void lpmx_damage_free(void);

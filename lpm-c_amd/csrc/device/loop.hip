@@ -1,0 +1,111 @@
+// The implicit load-step loop of the reference driver (src/lpmc_project.c:382-546)
+// with every array resident on the device.  The host only takes the convergence
+// decisions: two FP64 norms per Newton iteration come back, nothing else.
+#include <chrono>
+#include <math.h>
+#include <string.h>
+
+#include "common.h"
+
+int lpmx_last_ir_passes(void);
+
+static double now_s(void)
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+static int sync(void)
+{
+    HIPCHK(hipStreamSynchronize(g_stream));
+    return LPMX_OK;
+}
+
+extern "C" int lpmx_run_steps(const lpmx_solver *s, int nsteps, const int *nbd, const lpmx_dispbc *dbc,
+                              const int *nbf, const lpmx_forcebc *fbc, int maxbc, char out_axis, int out_type,
+                              lpmx_step_stats *out)
+{
+    RETCHK(lpmx_require_pattern());
+    if (!s || nsteps < 0 || (nsteps > 0 && (!nbd || !nbf)) || maxbc <= 0)
+        return lpmx_fail(LPMX_ERR_ARG, "invalid arguments to lpmx_run_steps");
+    const int n = g.n, d = g.dim;
+    const int64_t nbn = (int64_t)n * g.nb;
+    int rc_all = LPMX_OK;
+    for (int st = 0; st < nsteps; st++)
+    {
+        lpmx_step_stats o;
+        memset(&o, 0, sizeof(o));
+        const double t0 = now_s();
+        // xyz_temp, F_temp, Pex_temp (lpmc_project.c:387-389)
+        HIPCHK(hipMemcpyAsync(g.xt, g.x, sizeof(double) * 3 * n, hipMemcpyDeviceToDevice, g_stream));
+        HIPCHK(hipMemcpyAsync(g.Ft, g.F, sizeof(double) * nbn, hipMemcpyDeviceToDevice, g_stream));
+        HIPCHK(hipMemcpyAsync(g.Pext, g.Pex, sizeof(double) * (int64_t)d * n, hipMemcpyDeviceToDevice, g_stream));
+        // elastic tangent of the step (lpmc_project.c:391-396)
+        RETCHK(lpmx_tangent());
+        RETCHK(sync());
+        const double t1 = now_s();
+        o.t_tangent += t1 - t0;
+        RETCHK(lpmx_set_disp_bc(nbd[st], dbc + (int64_t)st * maxbc));
+        RETCHK(lpmx_set_force_bc(nbf[st], fbc + (int64_t)st * maxbc));
+        RETCHK(lpmx_bond_force(4)); // predictor (lpmc_project.c:405)
+        for (;;)
+        {
+            double nres = 0, nrea = 0;
+            RETCHK(lpmx_update_rr(&nres, &nrea, nullptr));
+            const double tolm = nres > nrea ? nres : nrea;
+            int ni = 0;
+            while (nres > s->newton_tol * tolm && ni < s->newton_maxit)
+            {
+                RETCHK(lpmx_switch_state(0));
+                RETCHK(lpmx_bc_eliminate());
+                double ts = now_s();
+                int it = 0, stt = 0;
+                double rr = 0;
+                RETCHK(lpmx_solve_ex(s->precision, s->rtol2, s->atol2, s->maxit, s->inner_rtol2, s->ir_maxit, &it,
+                                     &rr, &stt));
+                RETCHK(sync());
+                double tf = now_s();
+                o.t_solve += tf - ts;
+                o.cg_iters += it;
+                o.ir_passes += lpmx_last_ir_passes();
+                if (stt != 1)
+                    o.status |= stt == 3 ? 4 : 2;
+                RETCHK(lpmx_bond_force(g.plmode));
+                RETCHK(lpmx_update_rr(&nres, nullptr, nullptr));
+                o.t_force += now_s() - tf;
+                ni++;
+            }
+            if (ni >= s->newton_maxit)
+                o.status |= 1;
+            o.newton_iters += ni;
+            o.res_norm = nres;
+            o.tol_mult = tolm;
+            int broken = 0;
+            RETCHK(lpmx_damage(g.plmode, &broken));
+            RETCHK(lpmx_update_crack());
+            RETCHK(lpmx_switch_state(1));
+            RETCHK(lpmx_reaction(out_axis, out_type, &o.reaction, &o.disp));
+            o.broken += broken;
+            if (broken > 0)
+            {
+                if (++o.resolves > 100000)
+                    return lpmx_fail(LPMX_ERR_NOCONV, "bond breakage did not settle in step %d", st);
+                double tb = now_s();
+                RETCHK(lpmx_tangent());
+                RETCHK(sync());
+                o.t_tangent += now_s() - tb;
+                continue;
+            }
+            break;
+        }
+        RETCHK(sync());
+        o.t_total = now_s() - t0;
+        o.t_other = o.t_total - o.t_tangent - o.t_solve - o.t_force;
+        if (o.status)
+            rc_all = LPMX_ERR_NOCONV;
+        if (out)
+            out[st] = o;
+    }
+    if (rc_all != LPMX_OK)
+        return lpmx_fail(rc_all, "a solve or Newton loop hit its cap (see lpmx_step_stats.status)");
+    return LPMX_OK;
+}

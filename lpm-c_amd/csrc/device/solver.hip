@@ -1,0 +1,712 @@
+// Linear solve K disp = residual on the device: block-Jacobi preconditioned CG
+// over the sliced-ELL 3x3 (2x2) block matrix, in FP64 or in the mixed form
+// (FP32 values/vectors, compensated FP32 dot accumulation, FP64 refinement).
+//
+// One CG iteration is two kernels.  k_cg_spmv forms p = z + beta p_old on the fly
+// for every gathered column (p is double-buffered), writes p and q = K p for its
+// own rows and the partial p.q; the last work-group to finish turns the partials
+// into alpha.  k_cg_update does x += alpha p, r -= alpha q, z = M^-1 r and the
+// partials of r.z and r.r; its last work-group forms beta and applies the stopping
+// test.  Both kernels return at once when the device flag `done` is set, so the
+// host launches batches of iterations and reads the flag once per batch.
+#include <math.h>
+#include <string.h>
+
+#include <vector>
+
+#include "common.h"
+
+int lpmx_bj_inverse(void);
+
+struct CgS
+{
+    double rz, rr, rr0, thr, alpha, beta, pq;
+    int it, maxit, done, pad;
+};
+
+// --------------------------------------------------------- compensated sums -----
+// Error-free transformations (Knuth TwoSum, FMA-based TwoProduct) for the FP32
+// dot products of the mixed form.
+struct F2
+{
+    float hi, lo;
+};
+
+__device__ __forceinline__ F2 two_sum(float a, float b)
+{
+    float s = a + b;
+    float bb = s - a;
+    float e = (a - (s - bb)) + (b - bb);
+    return {s, e};
+}
+
+__device__ __forceinline__ F2 f2_add(F2 a, F2 b)
+{
+    F2 s = two_sum(a.hi, b.hi);
+    s.lo += a.lo + b.lo;
+    return two_sum(s.hi, s.lo);
+}
+
+__device__ __forceinline__ F2 f2_dot_acc(F2 acc, float a, float b)
+{
+    float p = a * b;
+    float e = fmaf(a, b, -p);
+    F2 s = two_sum(acc.hi, p);
+    s.lo += acc.lo + e;
+    return s;
+}
+
+__device__ __forceinline__ F2 wave_sum_f2(F2 v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1)
+    {
+        F2 w = {__shfl_xor(v.hi, o, 64), __shfl_xor(v.lo, o, 64)};
+        v = f2_add(v, w);
+    }
+    return v;
+}
+
+// Work-group total as a double: compensated FP32 within wavefronts, the few wave
+// totals combined in FP64.
+__device__ __forceinline__ double block_sum_f2(F2 v, double *sh)
+{
+    v = wave_sum_f2(v);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    __syncthreads();
+    if (lane == 0)
+        sh[w] = (double)v.hi + (double)v.lo;
+    __syncthreads();
+    double t = 0;
+    for (int k = 0; k < nw; k++)
+        t += sh[k];
+    return t;
+}
+
+template <typename T>
+struct Acc;
+template <>
+struct Acc<double>
+{
+    double v = 0.0;
+    __device__ void add(double a, double b) { v += a * b; }
+    __device__ double block(double *sh) { return block_sum(v, sh); }
+};
+template <>
+struct Acc<float>
+{
+    F2 v = {0.0f, 0.0f};
+    __device__ void add(float a, float b) { v = f2_dot_acc(v, a, b); }
+    __device__ double block(double *sh) { return block_sum_f2(v, sh); }
+};
+
+// Work-groups are dealt to the 8 XCDs round robin; remap so that each XCD walks a
+// contiguous range of rows and its L2 keeps the gathered neighbour columns.
+__device__ __forceinline__ int xcd_block(void)
+{
+    const int G = gridDim.x, b = blockIdx.x;
+    const int per = G / 8, rem = G % 8;
+    const int x = b % 8, k = b / 8;
+    // XCDs 0..rem-1 own per+1 groups, the others per
+    const int base = x < rem ? x * (per + 1) : rem * (per + 1) + (x - rem) * per;
+    return base + k;
+}
+
+// ------------------------------------------------------------------ kernels ------
+template <int D, typename TV, typename TX>
+__global__ __launch_bounds__(256) void k_cg_spmv(int n, const int *__restrict__ sptr, const int *__restrict__ col,
+                                                 const TV *__restrict__ K, const TX *__restrict__ z,
+                                                 const TX *__restrict__ pold, TX *__restrict__ pnew,
+                                                 TX *__restrict__ q, double *parts, CgS *sc, unsigned *ctr)
+{
+    __shared__ double sh[4];
+    if (sc->done)
+        return;
+    const int blk = xcd_block();
+    const int i = blk * 256 + threadIdx.x;
+    const TX beta = (TX)sc->beta;
+    Acc<TX> dot;
+    if (i < n)
+    {
+        const int s = i >> 6, l = i & 63;
+        const int k0 = sptr[s], k1 = sptr[s + 1];
+        TX acc[D];
+#pragma unroll
+        for (int r = 0; r < D; r++)
+            acc[r] = 0;
+        for (int k = k0; k < k1; k++)
+        {
+            const int m = col[(int64_t)k * 64 + l];
+            TX pm[D];
+#pragma unroll
+            for (int c = 0; c < D; c++)
+                pm[c] = z[(int64_t)m * D + c] + beta * pold[(int64_t)m * D + c];
+            const TV *kb = K + (int64_t)k * D * D * 64 + l;
+#pragma unroll
+            for (int r = 0; r < D; r++)
+#pragma unroll
+                for (int c = 0; c < D; c++)
+                    acc[r] += (TX)kb[(r * D + c) * 64] * pm[c];
+        }
+#pragma unroll
+        for (int c = 0; c < D; c++)
+        {
+            const TX pi = z[(int64_t)i * D + c] + beta * pold[(int64_t)i * D + c];
+            pnew[(int64_t)i * D + c] = pi;
+            q[(int64_t)i * D + c] = acc[c];
+            dot.add(pi, acc[c]);
+        }
+    }
+    const double part = dot.block(sh);
+    if (threadIdx.x == 0)
+        parts[blk] = part;
+    if (last_group(ctr))
+    {
+        const double pq = ordered_total(parts, gridDim.x, sh);
+        if (threadIdx.x == 0)
+        {
+            sc->pq = pq;
+            if (!(pq > 0.0) || !isfinite(pq))
+                sc->done = 3; // breakdown: K not positive definite on p
+            else
+                sc->alpha = sc->rz / pq;
+            *ctr = 0;
+        }
+    }
+}
+
+template <int D, typename TX, typename TM>
+__global__ __launch_bounds__(256) void k_cg_update(int n, TX *__restrict__ x, const TX *__restrict__ p,
+                                                   TX *__restrict__ r, const TX *__restrict__ q,
+                                                   TX *__restrict__ z, const TM *__restrict__ Minv, double *parts,
+                                                   CgS *sc, unsigned *ctr)
+{
+    __shared__ double sh[4];
+    if (sc->done)
+        return;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const TX alpha = (TX)sc->alpha;
+    Acc<TX> rz, rr;
+    if (i < n)
+    {
+        TX rv[D];
+#pragma unroll
+        for (int c = 0; c < D; c++)
+        {
+            const int64_t o = (int64_t)i * D + c;
+            x[o] += alpha * p[o];
+            rv[c] = r[o] - alpha * q[o];
+            r[o] = rv[c];
+        }
+#pragma unroll
+        for (int a = 0; a < D; a++)
+        {
+            TX zv = 0;
+#pragma unroll
+            for (int c = 0; c < D; c++)
+                zv += (TX)Minv[(int64_t)(a * D + c) * n + i] * rv[c];
+            z[(int64_t)i * D + a] = zv;
+            rz.add(rv[a], zv);
+            rr.add(rv[a], rv[a]);
+        }
+    }
+    const double prz = rz.block(sh), prr = rr.block(sh);
+    if (threadIdx.x == 0)
+        parts[2 * blockIdx.x] = prz, parts[2 * blockIdx.x + 1] = prr;
+    if (last_group(ctr))
+    {
+        double a = 0.0, b = 0.0;
+        for (int k = threadIdx.x; k < (int)gridDim.x; k += blockDim.x)
+            a += coherent_load(parts + 2 * k), b += coherent_load(parts + 2 * k + 1);
+        a = block_sum(a, sh);
+        b = block_sum(b, sh);
+        if (threadIdx.x == 0)
+        {
+            sc->it += 1;
+            sc->rr = b;
+            if (b <= sc->thr)
+                sc->done = 1;
+            else if (sc->it >= sc->maxit)
+                sc->done = 2;
+            else
+            {
+                sc->beta = a / sc->rz;
+                sc->rz = a;
+            }
+            *ctr = 0;
+        }
+    }
+}
+
+// r = b (converted), x = 0, p = 0, z = M^-1 r; rz, rr, threshold.
+template <int D, typename TX, typename TM>
+__global__ __launch_bounds__(256) void k_cg_init(int n, const double *__restrict__ b, TX *__restrict__ r,
+                                                 TX *__restrict__ x, TX *__restrict__ p0, TX *__restrict__ z,
+                                                 const TM *__restrict__ Minv, double *parts, CgS *sc, unsigned *ctr,
+                                                 double rtol2, double atol2, int maxit)
+{
+    __shared__ double sh[4];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    Acc<TX> rz, rr;
+    if (i < n)
+    {
+        TX rv[D];
+#pragma unroll
+        for (int c = 0; c < D; c++)
+        {
+            const int64_t o = (int64_t)i * D + c;
+            rv[c] = (TX)b[o];
+            r[o] = rv[c];
+            x[o] = 0;
+            p0[o] = 0;
+        }
+#pragma unroll
+        for (int a = 0; a < D; a++)
+        {
+            TX zv = 0;
+#pragma unroll
+            for (int c = 0; c < D; c++)
+                zv += (TX)Minv[(int64_t)(a * D + c) * n + i] * rv[c];
+            z[(int64_t)i * D + a] = zv;
+            rz.add(rv[a], zv);
+            rr.add(rv[a], rv[a]);
+        }
+    }
+    const double prz = rz.block(sh), prr = rr.block(sh);
+    if (threadIdx.x == 0)
+        parts[2 * blockIdx.x] = prz, parts[2 * blockIdx.x + 1] = prr;
+    if (last_group(ctr))
+    {
+        double a = 0.0, bb = 0.0;
+        for (int k = threadIdx.x; k < (int)gridDim.x; k += blockDim.x)
+            a += coherent_load(parts + 2 * k), bb += coherent_load(parts + 2 * k + 1);
+        a = block_sum(a, sh);
+        bb = block_sum(bb, sh);
+        if (threadIdx.x == 0)
+        {
+            sc->rz = a;
+            sc->rr = bb;
+            sc->rr0 = bb;
+            sc->thr = rtol2 * bb + atol2;
+            sc->beta = 0.0;
+            sc->alpha = 0.0;
+            sc->it = 0;
+            sc->maxit = maxit;
+            sc->done = (bb <= sc->thr) ? 1 : (maxit <= 0 ? 2 : 0);
+            *ctr = 0;
+        }
+    }
+}
+
+// r = b - K x in FP64 with |r|^2 (the refinement residual of the mixed form).
+template <int D>
+__global__ __launch_bounds__(256) void k_residual64(int n, const int *__restrict__ sptr, const int *__restrict__ col,
+                                                    const double *__restrict__ K, const double *__restrict__ x,
+                                                    const double *__restrict__ b, double *__restrict__ r,
+                                                    double *parts, double *out, unsigned *ctr)
+{
+    __shared__ double sh[4];
+    const int blk = xcd_block();
+    const int i = blk * 256 + threadIdx.x;
+    double rr = 0.0;
+    if (i < n)
+    {
+        const int s = i >> 6, l = i & 63;
+        double acc[D];
+        for (int c = 0; c < D; c++)
+            acc[c] = 0;
+        for (int k = sptr[s]; k < sptr[s + 1]; k++)
+        {
+            const int m = col[(int64_t)k * 64 + l];
+            const double *kb = K + (int64_t)k * D * D * 64 + l;
+            for (int rr_ = 0; rr_ < D; rr_++)
+                for (int c = 0; c < D; c++)
+                    acc[rr_] += kb[(rr_ * D + c) * 64] * x[(int64_t)m * D + c];
+        }
+        for (int c = 0; c < D; c++)
+        {
+            const double v = b[(int64_t)i * D + c] - acc[c];
+            r[(int64_t)i * D + c] = v;
+            rr += v * v;
+        }
+    }
+    const double part = block_sum(rr, sh);
+    if (threadIdx.x == 0)
+        parts[blk] = part;
+    if (last_group(ctr))
+    {
+        const double t = ordered_total(parts, gridDim.x, sh);
+        if (threadIdx.x == 0)
+        {
+            *out = t;
+            *ctr = 0;
+        }
+    }
+}
+
+template <int D>
+__global__ void k_add_f32(int n, double *x, const float *e)
+{
+    int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t < (int64_t)n * D)
+        x[t] += (double)e[t];
+}
+
+// xyz += disp (solver.c:403-408), disp is dim-strided.
+__global__ void k_apply_disp(int n, int dim, double *xyz, const double *disp)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    for (int c = 0; c < dim; c++)
+        xyz[3 * i + c] += disp[(int64_t)i * dim + c];
+}
+
+// Plain y = K x (FP64 or FP32 values), for tests and the standalone bench.
+template <int D, typename TV>
+__global__ __launch_bounds__(256) void k_spmv_plain(int n, const int *__restrict__ sptr, const int *__restrict__ col,
+                                                    const TV *__restrict__ K, const double *__restrict__ x,
+                                                    double *__restrict__ y)
+{
+    const int i = xcd_block() * 256 + threadIdx.x;
+    if (i >= n)
+        return;
+    const int s = i >> 6, l = i & 63;
+    double acc[D];
+    for (int c = 0; c < D; c++)
+        acc[c] = 0;
+    for (int k = sptr[s]; k < sptr[s + 1]; k++)
+    {
+        const int m = col[(int64_t)k * 64 + l];
+        const TV *kb = K + (int64_t)k * D * D * 64 + l;
+        for (int r = 0; r < D; r++)
+            for (int c = 0; c < D; c++)
+                acc[r] += (double)kb[(r * D + c) * 64] * x[(int64_t)m * D + c];
+    }
+    for (int c = 0; c < D; c++)
+        y[(int64_t)i * D + c] = acc[c];
+}
+
+// ------------------------------------------------------------------ workspace ----
+struct Work
+{
+    int n = 0, dim = 0;
+    double *x64 = nullptr, *r64 = nullptr, *z64 = nullptr, *q64 = nullptr, *p64[2] = {nullptr, nullptr};
+    float *x32 = nullptr, *r32 = nullptr, *z32 = nullptr, *q32 = nullptr, *p32[2] = {nullptr, nullptr};
+    double *b = nullptr, *parts = nullptr, *scal = nullptr;
+    CgS *sc = nullptr;
+    CgS *hsc = nullptr; // pinned host copy
+    hipEvent_t ev[2 * 64];
+    bool have_ev = false;
+} W;
+
+static int g_prof = 0;
+static double g_prof_ms = 0.0;
+static int64_t g_prof_n = 0;
+
+void lpmx_solver_free(void)
+{
+    void *ps[] = {W.x64, W.r64, W.z64, W.q64, W.p64[0], W.p64[1], W.x32, W.r32, W.z32, W.q32, W.p32[0], W.p32[1],
+                  W.b, W.parts, W.scal, W.sc};
+    for (void *p : ps)
+        if (p)
+            hipFree(p);
+    if (W.hsc)
+        hipHostFree(W.hsc);
+    if (W.have_ev)
+        for (auto &e : W.ev)
+            hipEventDestroy(e);
+    W = Work();
+}
+
+static int ensure_work(void)
+{
+    if (W.n == g.n && W.dim == g.dim && W.sc)
+        return LPMX_OK;
+    lpmx_solver_free();
+    const int64_t m = (int64_t)g.n * g.dim;
+    double **d64[] = {&W.x64, &W.r64, &W.z64, &W.q64, &W.p64[0], &W.p64[1], &W.b};
+    for (double **p : d64)
+        HIPCHK(hipMalloc(p, sizeof(double) * m));
+    float **d32[] = {&W.x32, &W.r32, &W.z32, &W.q32, &W.p32[0], &W.p32[1]};
+    for (float **p : d32)
+        HIPCHK(hipMalloc(p, sizeof(float) * m));
+    const int nbk = nblocks(g.n, 256) + 8;
+    HIPCHK(hipMalloc(&W.parts, sizeof(double) * 2 * nbk));
+    HIPCHK(hipMalloc(&W.scal, sizeof(double) * 8));
+    HIPCHK(hipMalloc(&W.sc, sizeof(CgS)));
+    HIPCHK(hipHostMalloc(&W.hsc, sizeof(CgS)));
+    for (auto &e : W.ev)
+        HIPCHK(hipEventCreate(&e));
+    W.have_ev = true;
+    W.n = g.n, W.dim = g.dim;
+    return LPMX_OK;
+}
+
+// Run PCG on the prepared r/x/p/z buffers until the device flag is set.
+template <int D, typename TV, typename TX, typename TM>
+static int pcg_loop(const TV *Kv, const TM *Minv, TX *x, TX *r, TX *z, TX *q, TX *p[2], const double *b, double rtol2,
+                    double atol2, int maxit, int *iters, double *rr_out, int *status)
+{
+    const int n = g.n;
+    const int G = nblocks(n, 256);
+    unsigned *ctr = lpmx_counters();
+    hipLaunchKernelGGL((k_cg_init<D, TX, TM>), dim3(G), dim3(256), 0, g_stream, n, b, r, x, p[0], z, Minv, W.parts,
+                       W.sc, ctr, rtol2, atol2, maxit);
+    LAUNCHCHK();
+    const int batch = n > 200000 ? 16 : 48;
+    int t = 0, ev_used = 0;
+    for (;;)
+    {
+        for (int k = 0; k < batch; k++, t++)
+        {
+            const bool timed = g_prof && ev_used < 64;
+            if (timed)
+                HIPCHK(hipEventRecord(W.ev[2 * ev_used], g_stream));
+            hipLaunchKernelGGL((k_cg_spmv<D, TV, TX>), dim3(G), dim3(256), 0, g_stream, n, g.sptr, g.col, Kv, z,
+                               p[t & 1], p[(t + 1) & 1], q, W.parts, W.sc, ctr + 1);
+            if (timed)
+            {
+                HIPCHK(hipEventRecord(W.ev[2 * ev_used + 1], g_stream));
+                ev_used++;
+            }
+            hipLaunchKernelGGL((k_cg_update<D, TX, TM>), dim3(G), dim3(256), 0, g_stream, n, x, p[(t + 1) & 1], r, q,
+                               z, Minv, W.parts, W.sc, ctr + 2);
+        }
+        LAUNCHCHK();
+        HIPCHK(hipMemcpyAsync(W.hsc, W.sc, sizeof(CgS), hipMemcpyDeviceToHost, g_stream));
+        HIPCHK(hipStreamSynchronize(g_stream));
+        if (W.hsc->done)
+            break;
+        if (t > (int64_t)maxit + 2 * batch) // the device flag must have been set by now
+            return lpmx_fail(LPMX_ERR_STATE, "CG loop ran past its cap without a stop flag");
+    }
+    if (g_prof)
+        for (int e = 0; e < ev_used; e++)
+        {
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, W.ev[2 * e], W.ev[2 * e + 1]));
+            // a launch skipped because the solve had converged takes ~no time; skip it
+            if (e < W.hsc->it)
+                g_prof_ms += ms, g_prof_n++;
+        }
+    *iters = W.hsc->it;
+    *rr_out = W.hsc->rr;
+    *status = W.hsc->done;
+    return LPMX_OK;
+}
+
+template <int D>
+static int solve_fp64(double rtol2, double atol2, int maxit, int *iters, double *rr, int *status)
+{
+    return pcg_loop<D, double, double, double>(g.K, g.Minv, g.disp, W.r64, W.z64, W.q64, W.p64, g.res, rtol2, atol2,
+                                               maxit, iters, rr, status);
+}
+
+template <int D>
+static int solve_mixed(double rtol2, double atol2, double inner_rtol2, int ir_maxit, int maxit, int *iters,
+                       double *rr, int *status, int *passes)
+{
+    const int n = g.n;
+    const int64_t m = (int64_t)n * D;
+    const int G = nblocks(n, 256);
+    unsigned *ctr = lpmx_counters();
+    double bb = 0.0;
+    RETCHK(lpmx_dev_sumsq(g.res, m, &bb));
+    const double thr = rtol2 * bb + atol2;
+    HIPCHK(hipMemsetAsync(g.disp, 0, sizeof(double) * m, g_stream));
+    HIPCHK(hipMemcpyAsync(W.b, g.res, sizeof(double) * m, hipMemcpyDeviceToDevice, g_stream));
+    int tot = 0, st = 0, pass = 0;
+    double cur = bb;
+    *status = 2;
+    if (bb <= thr)
+    {
+        *iters = 0, *rr = bb, *status = 1, *passes = 0;
+        return LPMX_OK;
+    }
+    for (pass = 0; pass < ir_maxit && tot < maxit; pass++)
+    {
+        int it = 0;
+        double irr = 0;
+        // inner FP32 solve on the current FP64 residual W.b
+        RETCHK((pcg_loop<D, float, float, float>(g.K32, g.Minv32, W.x32, W.r32, W.z32, W.q32, W.p32, W.b,
+                                                 inner_rtol2, 0.0, maxit - tot, &it, &irr, &st)));
+        tot += it;
+        hipLaunchKernelGGL((k_add_f32<D>), dim3(nblocks(m)), dim3(LPMX_BLOCK), 0, g_stream, n, g.disp, W.x32);
+        LAUNCHCHK();
+        hipLaunchKernelGGL((k_residual64<D>), dim3(G), dim3(256), 0, g_stream, n, g.sptr, g.col, g.K, g.disp, g.res,
+                           W.b, W.parts, W.scal, ctr + 3);
+        LAUNCHCHK();
+        HIPCHK(hipMemcpyAsync(&cur, W.scal, sizeof(double), hipMemcpyDeviceToHost, g_stream));
+        HIPCHK(hipStreamSynchronize(g_stream));
+        if (cur <= thr)
+        {
+            *status = 1;
+            pass++;
+            break;
+        }
+        if (st == 3)
+        {
+            *status = 3;
+            pass++;
+            break;
+        }
+    }
+    if (*status != 1 && *status != 3)
+        *status = 2;
+    *iters = tot;
+    *rr = cur;
+    *passes = pass;
+    return LPMX_OK;
+}
+
+static int g_last_passes = 0;
+int lpmx_last_ir_passes(void) { return g_last_passes; }
+
+// Full solve of the Newton correction: block-Jacobi inverse of the current matrix,
+// PCG (FP64 or mixed), then xyz += disp.  status: 1 converged, 2 cap, 3 breakdown.
+extern "C" int lpmx_solve_ex(int precision, double rtol2, double atol2, int maxit, double inner_rtol2,
+                             int ir_maxit, int *iters, double *rr, int *status)
+{
+    RETCHK(lpmx_require_pattern());
+    RETCHK(ensure_work());
+    RETCHK(lpmx_bj_inverse());
+    int it = 0, st = 0, passes = 1;
+    double r2 = 0;
+    if (precision == LPMX_MIXED)
+    {
+        if (g.dim == 3)
+            RETCHK(solve_mixed<3>(rtol2, atol2, inner_rtol2, ir_maxit, maxit, &it, &r2, &st, &passes));
+        else
+            RETCHK(solve_mixed<2>(rtol2, atol2, inner_rtol2, ir_maxit, maxit, &it, &r2, &st, &passes));
+    }
+    else
+    {
+        if (g.dim == 3)
+            RETCHK(solve_fp64<3>(rtol2, atol2, maxit, &it, &r2, &st));
+        else
+            RETCHK(solve_fp64<2>(rtol2, atol2, maxit, &it, &r2, &st));
+    }
+    g_last_passes = passes;
+    hipLaunchKernelGGL(k_apply_disp, dim3(nblocks(g.n)), dim3(LPMX_BLOCK), 0, g_stream, g.n, g.dim, g.x, g.disp);
+    LAUNCHCHK();
+    if (iters)
+        *iters = it;
+    if (rr)
+        *rr = r2;
+    if (status)
+        *status = st;
+    return LPMX_OK;
+}
+
+extern "C" int lpmx_solve(int precision, double rtol2, double atol2, int maxit, int *iters, double *rr)
+{
+    int st = 0;
+    RETCHK(lpmx_solve_ex(precision, rtol2, atol2, maxit, 1e-8, 12, iters, rr, &st));
+    HIPCHK(hipStreamSynchronize(g_stream));
+    if (st == 3)
+        return lpmx_fail(LPMX_ERR_NOCONV, "CG breakdown (p.Kp <= 0)");
+    if (st != 1)
+        return lpmx_fail(LPMX_ERR_NOCONV, "CG reached its iteration cap");
+    return LPMX_OK;
+}
+
+// ------------------------------------------------------------------ measuring ----
+extern "C" void lpmx_profile_spmv(int on)
+{
+    g_prof = on;
+    g_prof_ms = 0.0;
+    g_prof_n = 0;
+}
+
+extern "C" void lpmx_profile_spmv_read(double *total_ms, int64_t *launches)
+{
+    *total_ms = g_prof_ms;
+    *launches = g_prof_n;
+}
+
+extern "C" int lpmx_spmv(int precision, const double *x, double *y)
+{
+    RETCHK(lpmx_require_pattern());
+    const int64_t m = (int64_t)g.n * g.dim;
+    double *dx = nullptr, *dy = nullptr;
+    HIPCHK(hipMalloc(&dx, sizeof(double) * m));
+    HIPCHK(hipMalloc(&dy, sizeof(double) * m));
+    HIPCHK(hipMemcpyAsync(dx, x, sizeof(double) * m, hipMemcpyHostToDevice, g_stream));
+    const int G = nblocks(g.n, 256);
+    if (g.dim == 3)
+    {
+        if (precision == LPMX_MIXED)
+            hipLaunchKernelGGL((k_spmv_plain<3, float>), dim3(G), dim3(256), 0, g_stream, g.n, g.sptr, g.col, g.K32, dx, dy);
+        else
+            hipLaunchKernelGGL((k_spmv_plain<3, double>), dim3(G), dim3(256), 0, g_stream, g.n, g.sptr, g.col, g.K, dx, dy);
+    }
+    else
+    {
+        if (precision == LPMX_MIXED)
+            hipLaunchKernelGGL((k_spmv_plain<2, float>), dim3(G), dim3(256), 0, g_stream, g.n, g.sptr, g.col, g.K32, dx, dy);
+        else
+            hipLaunchKernelGGL((k_spmv_plain<2, double>), dim3(G), dim3(256), 0, g_stream, g.n, g.sptr, g.col, g.K, dx, dy);
+    }
+    LAUNCHCHK();
+    HIPCHK(hipMemcpyAsync(y, dy, sizeof(double) * m, hipMemcpyDeviceToHost, g_stream));
+    HIPCHK(hipStreamSynchronize(g_stream));
+    hipFree(dx), hipFree(dy);
+    return LPMX_OK;
+}
+
+// Times the CG SpMV kernel itself (the solve's dominant kernel) on the current
+// matrix: reps launches on the library stream, bracketed by HIP events.
+extern "C" int lpmx_bench_spmv(int precision, int reps, double *avg_ms, double *bytes)
+{
+    RETCHK(lpmx_require_pattern());
+    RETCHK(ensure_work());
+    const int n = g.n, D = g.dim, G = nblocks(n, 256);
+    unsigned *ctr = lpmx_counters();
+    CgS h;
+    memset(&h, 0, sizeof(h));
+    h.rz = 1.0, h.beta = 0.5, h.maxit = 1 << 30;
+    HIPCHK(hipMemcpyAsync(W.sc, &h, sizeof(CgS), hipMemcpyHostToDevice, g_stream));
+    const int64_t m = (int64_t)n * D;
+    const bool f32 = precision == LPMX_MIXED;
+    if (f32)
+    {
+        HIPCHK(hipMemsetAsync(W.z32, 0, sizeof(float) * m, g_stream));
+        HIPCHK(hipMemsetAsync(W.p32[0], 0, sizeof(float) * m, g_stream));
+    }
+    else
+    {
+        HIPCHK(hipMemsetAsync(W.z64, 0, sizeof(double) * m, g_stream));
+        HIPCHK(hipMemsetAsync(W.p64[0], 0, sizeof(double) * m, g_stream));
+    }
+    hipEvent_t e0 = W.ev[0], e1 = W.ev[1];
+    for (int pass = 0; pass < 2; pass++) // warm-up pass, then the timed pass
+    {
+        if (pass == 1)
+            HIPCHK(hipEventRecord(e0, g_stream));
+        for (int r = 0; r < (pass ? reps : 3); r++)
+        {
+            if (D == 3 && f32)
+                hipLaunchKernelGGL((k_cg_spmv<3, float, float>), dim3(G), dim3(256), 0, g_stream, n, g.sptr, g.col,
+                                   g.K32, W.z32, W.p32[0], W.p32[1], W.q32, W.parts, W.sc, ctr + 1);
+            else if (D == 3)
+                hipLaunchKernelGGL((k_cg_spmv<3, double, double>), dim3(G), dim3(256), 0, g_stream, n, g.sptr, g.col,
+                                   g.K, W.z64, W.p64[0], W.p64[1], W.q64, W.parts, W.sc, ctr + 1);
+            else if (f32)
+                hipLaunchKernelGGL((k_cg_spmv<2, float, float>), dim3(G), dim3(256), 0, g_stream, n, g.sptr, g.col,
+                                   g.K32, W.z32, W.p32[0], W.p32[1], W.q32, W.parts, W.sc, ctr + 1);
+            else
+                hipLaunchKernelGGL((k_cg_spmv<2, double, double>), dim3(G), dim3(256), 0, g_stream, n, g.sptr, g.col,
+                                   g.K, W.z64, W.p64[0], W.p64[1], W.q64, W.parts, W.sc, ctr + 1);
+        }
+        LAUNCHCHK();
+    }
+    HIPCHK(hipEventRecord(e1, g_stream));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    *avg_ms = ms / reps;
+    const double sv = f32 ? 4.0 : 8.0;
+    *bytes = (double)g.nnzb * (D * D * sv + 4) + 4.0 * (n + 1) + 2.0 * n * D * sv;
+    return LPMX_OK;
+}

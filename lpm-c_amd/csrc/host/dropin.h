@@ -1,0 +1,14 @@
+/* Internal interface between the drop-in host layer and its helpers. */
+#ifndef LPM_DROPIN_H
+#define LPM_DROPIN_H
+
+#include "lpmx.h"
+
+/* Abort like the reference's fatal paths: message, then exit(1). */
+void lpm_dropin_fatal(const char *where, int rc);
+
+/* Transfers between the reference's host globals and the device fields. */
+void lpm_dropin_push(int field);
+void lpm_dropin_pull(int field);
+
+#endif

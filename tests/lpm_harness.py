@@ -1,0 +1,43 @@
+"""Shared setup of the parity tests: the product harness (lpm-c_amd/__init__.py,
+loaded by path because of the package directory's name) and model builders that
+give the oracle exactly the product's inputs."""
+import importlib.util
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def product():
+    spec = importlib.util.spec_from_file_location("lpm_c_amd", os.path.join(ROOT, "lpm-c_amd", "__init__.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+P = product()
+
+
+def oracle_model(O, x, lattice, types, Ce, **kw):
+    m = O.Model(x, lattice, 0.5, types, Ce, **kw)
+    m.setup()
+    return m
+
+
+def sc_pair(O, k, E=200e3, nu=0.3, sigmay=200.0, H=2000.0, emod=None):
+    """The same SC model in the product (device, through the drop-in API) and in
+    the oracle: k^3 particles, uniaxial types, J2 mixed-linear material."""
+    L = P.sc_box_for(k)
+    n, x, t, c = P.build_sc_model(L, E=E, nu=nu, sigmay=sigmay, H=H, emod=emod)
+    m = oracle_model(O, x, 2, t, [c] * 4, plmode=0, sigmay=sigmay, J2_H=H, emod=emod)
+    return L, n, x, t, m
+
+
+def push(name, arr):
+    a = np.ascontiguousarray(arr)
+    P.check(P.lib().lpmx_put(P.F[name], a.ctypes.data, a.size), name)
+
+
+def get(name, shape):
+    return P.field(name, int(np.prod(shape))).reshape(shape)

@@ -1,0 +1,278 @@
+"""Parity of the HIP path (through the drop-in API and lpmx) with the CPU oracle on
+the same inputs.  Integer and index results are compared bit for bit; the
+floating-point laws are the reference's evaluation order compiled without
+contraction on both sides and are compared bit for bit too, except where a
+device library function (exp) or a different summation order enters, which is
+stated per check."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from lpm_harness import P, get, oracle_model, push, sc_pair
+
+pytestmark = pytest.mark.gpu
+
+
+def host2d(name, n, w, dtype=np.float64):
+    return P.read2d(name, n, w, dtype)
+
+
+@pytest.mark.parametrize("kind", ["sc", "hex", "fcc"])
+def test_neighbor_lists_and_pattern_bit_exact(oracle, kind):
+    L = P.lib()
+    if kind == "sc":
+        lat, box = 2, [0, 7.321, 0, 7.321, 0, 7.321]
+    elif kind == "hex":
+        lat, box = 1, [0, 12.0, 0, 10.0, 0, 1.0]
+    else:
+        lat, box = 3, [0, 6.5, 0, 6.5, 0, 6.5]
+    n = P.make_lattice(lat, 0.5, box)
+    L.searchNormalNeighbor()
+    lenK = L.searchAFEMNeighbor()
+    x = P.positions()
+    m = oracle_model(oracle, x, lat, np.zeros(n, np.int32), [(1.0, 0.5, 0.25)])
+    nb = P.gint("nneighbors").value
+    np.testing.assert_array_equal(host2d("neighbors", n, nb, np.int32), m.nbr)
+    np.testing.assert_array_equal(host2d("nsign", n, nb, np.int32), m.nsign)
+    np.testing.assert_array_equal(P.read1d("nb_initial", n, np.int32), m.nb_initial)
+    np.testing.assert_array_equal(host2d("distance_initial", n, nb), m.L0)
+    for c, name in enumerate(("csx_initial", "csy_initial", "csz_initial")):
+        np.testing.assert_array_equal(host2d(name, n, nb), m.cs0[:, :, c])
+    cm = P.gint("nneighbors_AFEM").value + 1
+    np.testing.assert_array_equal(host2d("conn", n, cm, np.int32), m.conn)
+    np.testing.assert_array_equal(P.read1d("nb_conn", n, np.int32), m.nb_conn)
+    np.testing.assert_array_equal(host2d("K_pointer", n + 1, 2, np.int32), m.kptr)
+    assert lenK == m.s.nK
+    nb1 = P.gint("nneighbors1").value
+    np.testing.assert_array_equal(host2d("neighbors1", n, nb1, np.int32), m.nbr1)
+
+
+def test_bond_constants_and_tangent(oracle):
+    L = P.lib()
+    _, n, x, t, m = sc_pair(oracle, 6)
+    np.testing.assert_array_equal(host2d("Kn", n, 18), m.Kn)
+    np.testing.assert_array_equal(host2d("Tv", n, 18), m.Tv)
+    rng = np.random.default_rng(4)
+    xd = x + 2e-3 * rng.standard_normal(x.shape)
+    dlp = 1e-4 * rng.standard_normal((n, 18)) * (m.nbr >= 0)
+    push("XYZ", xd)
+    push("DLP0", dlp)
+    P.check(L.lpmx_tangent(), "tangent")
+    nK = m.s.nK
+    IK = np.zeros(3 * n + 1, np.int32)
+    JK = np.zeros(nK, np.int32)
+    K = np.zeros(nK)
+    P.check(L.lpmx_export_upper_csr(P.ptr(IK), P.ptr(JK), P.ptr(K)), "export")
+    m.x[:] = xd
+    m.dLp[:, :, 0] = dlp
+    oracle.lib().orc_tangent_analytic(m.p)
+    np.testing.assert_array_equal(IK, m.IK)
+    np.testing.assert_array_equal(JK, m.JK)
+    # same Jacobian, different summation order of the contributions
+    assert np.abs(K - m.K).max() <= 1e-12 * np.abs(m.K).max()
+    oracle.lib().orc_tangent_fd(m.p)
+    np.testing.assert_array_equal(IK, m.IK)
+    np.testing.assert_array_equal(JK, m.JK)
+    assert np.abs(K - m.K).max() <= 1e-5 * np.abs(m.K).max()
+    # the device SpMV equals the upper-CSR product of the same values
+    oracle.lib().orc_tangent_analytic(m.p)
+    v = rng.standard_normal(3 * n)
+    y = np.zeros(3 * n)
+    P.check(L.lpmx_spmv(P.LPMX_FP64, P.ptr(v), P.ptr(y)), "spmv")
+    m.A["K"][:] = K
+    yo = np.zeros(3 * n)
+    oracle.lib().orc_spmv_upper(m.p, oracle.ptr(v), oracle.ptr(yo))
+    assert np.abs(y - yo).max() <= 1e-13 * np.abs(yo).max()
+    P.check(L.lpmx_spmv(P.LPMX_MIXED, P.ptr(v), P.ptr(y)), "spmv32")
+    assert np.abs(y - yo).max() <= 1e-5 * np.abs(yo).max()
+
+
+@pytest.mark.parametrize("mode", [6, 0, 4])
+def test_constitutive_laws_bit_exact(oracle, mode):
+    L = P.lib()
+    _, n, x, t, m = sc_pair(oracle, 6)
+    rng = np.random.default_rng(9 + mode)
+    eps = 2.5e-3 * np.array([[1.0, 0.1, 0.0], [0.1, -0.4, 0.05], [0.0, 0.05, -0.3]])
+    xd = x + x @ eps.T + 2e-4 * rng.standard_normal(x.shape)
+    xt = x + 0.5 * (xd - x)
+    ft = 10.0 * rng.standard_normal((n, 18)) * (m.nbr >= 0)
+    dlp = 2e-5 * rng.standard_normal((n, 18)) * (m.nbr >= 0)
+    beta = 3.0 * rng.standard_normal((n, 6))
+    alpha = 1e-3 * rng.uniform(size=n)
+    for name, v in (("XYZ", xd), ("XYZ_TEMP", xt), ("F_TEMP", ft), ("DLP0", dlp), ("J2_BETA0", beta),
+                    ("J2_ALPHA0", alpha)):
+        push(name, v)
+    # predictor uses the direction cosines of the last evaluation: give both the same
+    P.check(L.lpmx_computedL(), "computedL")
+    P.check(L.lpmx_bond_force(mode), "bond force")
+    m.x[:] = xd
+    m.xt[:] = xt
+    m.Ftemp[:] = ft
+    m.dLp[:, :, 0] = dlp
+    m.beta[:, :, 0] = beta
+    m.alpha[:, 0] = alpha
+    oracle.lib().orc_computedL(m.p)
+    oracle.lib().orc_bond_force_general(m.p, mode)
+    nb = (n, 18)
+    np.testing.assert_array_equal(get("F", nb), m.F)
+    np.testing.assert_array_equal(get("PIN", (3 * n,)), m.Pin)
+    np.testing.assert_array_equal(get("STRESS", (n, 6)), m.stress)
+    np.testing.assert_array_equal(get("BOND_STRESS", nb), m.bond_stress)
+    if mode != 4:
+        np.testing.assert_array_equal(get("DL", nb), m.dL)
+        for c, f in enumerate(("CSX", "CSY", "CSZ")):
+            np.testing.assert_array_equal(get(f, nb), m.cs[:, :, c])
+    if mode == 0:
+        dl = get("J2_DLAMBDA", (n,))
+        assert (dl > 0).sum() > n // 4, "the test deformation yields a good share of particles"
+        np.testing.assert_array_equal(dl, m.dlambda)
+        np.testing.assert_array_equal(get("DLP0", nb), m.dLp[:, :, 0])
+        np.testing.assert_array_equal(get("J2_BETA0", (n, 6)), m.beta[:, :, 0])
+        np.testing.assert_array_equal(get("J2_ALPHA0", (n,)), m.alpha[:, 0])
+        np.testing.assert_array_equal(get("DL_AVE", nb), m.dL_ave)
+        np.testing.assert_array_equal(get("PL_FLAG", (n,)), m.pl_flag)
+
+
+def test_bc_elimination_and_solve(oracle):
+    L = P.lib()
+    _, n, x, t, m = sc_pair(oracle, 6)
+    L.calcStiffness3DFiniteDifference(6)
+    dbc = np.ones(3 * n, np.int32)
+    dbc.reshape(n, 3)[t == 2, 2] = 0
+    dbc.reshape(n, 3)[t == 3, :] = 0
+    dbc.reshape(n, 3)[t == 1, 2] = 0
+    P.write1d("dispBC_index", dbc)
+    rng = np.random.default_rng(2)
+    r = rng.standard_normal(3 * n) * dbc
+    P.write1d("residual", r)
+    L.setDispBC_stiffnessUpdate3D()
+    nK = m.s.nK
+    K = P.read1d("K_global", nK)
+    oracle.lib().orc_tangent_analytic(m.p)
+    m.dispBC[:] = dbc
+    m.residual[:] = r
+    oracle.lib().orc_bc_stiffness_update(m.p)
+    np.testing.assert_array_equal(P.read1d("residual", 3 * n), m.residual)
+    assert np.abs(K - m.K).max() <= 1e-12 * np.abs(m.K).max()
+    dense = oracle.upper_csr_dense(m)
+    want = np.linalg.solve(dense, m.residual)
+    for prec, tol in ((P.LPMX_FP64, 1e-10), (P.LPMX_MIXED, 1e-10)):
+        s = P.solver(precision=prec, rtol2=1e-26, atol2=0.0, maxit=50 * 3 * n, inner_rtol2=1e-8)
+        L.lpm_ext_set_solver(C.byref(s))
+        P.write1d("residual", m.residual)
+        x0 = P.positions()
+        L.setDispBC_stiffnessUpdate3D()
+        L.solverCG()
+        d = P.read1d("disp", 3 * n)
+        assert np.abs(d - want).max() <= tol * np.abs(want).max(), prec
+        np.testing.assert_allclose(P.positions() - x0, d.reshape(n, 3), rtol=0, atol=1e-15)
+    L.lpm_ext_set_solver(None)
+
+
+def test_nonlocal_damage_and_crack(oracle):
+    L = P.lib()
+    _, n, x, t, m = sc_pair(oracle, 6)
+    rng = np.random.default_rng(5)
+    dlam = rng.uniform(0, 1e-2, n)
+    dlam[rng.uniform(size=n) < 0.05] = 2.0  # a few particles far past the threshold
+    tri = rng.uniform(-0.5, 0.5, n)
+    for name, v in (("J2_DLAMBDA", dlam), ("J2_TRIAXIALITY", tri)):
+        push(name, v)
+    P.set_params(plmode=0, damage_L=0.5, damage_threshold=0.9, damagec_A=0.3)
+    P.check(L.lpmx_set_params(C.byref(P.Params(0, 2000.0, 0.0, 0.5, 0.9, 0.3, 1e-2, 20))), "params")
+    broken = C.c_int()
+    P.check(L.lpmx_damage(0, C.byref(broken)), "damage")
+    m.dlambda[:] = dlam
+    m.triax[:] = tri
+    m.s.damagec_A = 0.3
+    k = oracle.lib().orc_damage_pwise_nonlocal(m.p)
+    assert broken.value == k and k > 0
+    # Gaussian weights use the device exp: agreement to rounding
+    np.testing.assert_allclose(get("DAMAGE_NONLOCAL0", (n,)), m.dnl[:, 0], rtol=1e-14, atol=0)
+    np.testing.assert_array_equal(get("DAMAGE_BROKEN", (n, 18)), m.dmg_broken)
+    np.testing.assert_allclose(get("DAMAGE_W", (n, 18)), m.dmg_w, rtol=1e-13, atol=1e-15)
+    # crack bookkeeping on forces of a deformed state
+    ft = rng.standard_normal((n, 18)) * (m.nbr >= 0)
+    push("F", ft)
+    push("DAMAGE_W", m.dmg_w)
+    P.check(L.lpmx_computedL(), "computedL")
+    P.check(L.lpmx_update_crack(), "crack")
+    m.F[:] = ft
+    oracle.lib().orc_computedL(m.p)
+    oracle.lib().orc_update_crack(m.p)
+    np.testing.assert_array_equal(get("NB", (n,)), m.nb)
+    np.testing.assert_array_equal(get("F", (n, 18)), m.F)
+    np.testing.assert_array_equal(get("PIN", (3 * n,)), m.Pin)
+    np.testing.assert_array_equal(get("FIX_INDEX", (3 * n,)), m.fixidx)
+
+
+def test_brittle_selection_bit_exact(oracle):
+    L = P.lib()
+    n = P.make_lattice(1, 0.5, [0, 14.0, 0, 12.0, 0, 1.0])
+    L.searchNormalNeighbor()
+    L.searchAFEMNeighbor()
+    x = P.positions()
+    P.set_types(np.zeros(n, np.int32), 1)
+    c = P.elastic_constants(70e3, 0.25)
+    P.set_materials([c], 200.0)
+    P.set_params(plmode=6, critical_bstrain=1e-3, nbreak=7)
+    L.calcKnTv()
+    m = oracle_model(oracle, x, 1, np.zeros(n, np.int32), [c], plmode=6, critical_bstrain=1e-3, nbreak=7)
+    rng = np.random.default_rng(8)
+    live = m.nbr >= 0
+    # symmetric stretches with many ties, as (i, j) and (j, i) carry the same dL
+    s = np.round(rng.uniform(0, 2e-3, (n, 12)), 5)
+    for i in range(n):
+        for a in range(m.nb_initial[i]):
+            j = m.nbr[i, a]
+            if j < i:
+                b = list(m.nbr[j]).index(i)
+                s[i, a] = s[j, b]
+    dl = np.where(live, s * m.L0, 0.0)
+    push("DL", dl)
+    P.check(L.lpmx_set_params(C.byref(P.Params(6, 0.0, 0.0, 0.5, 0.9, 0.0, 1e-3, 7))), "params")
+    k = C.c_int()
+    P.check(L.lpmx_damage(6, C.byref(k)), "brittle")
+    m.dL[:] = dl
+    bi = np.zeros(64, np.int32)
+    bj = np.zeros(64, np.int32)
+    ko = oracle.lib().orc_damage_brittle(m.p, oracle.ptr(bi), oracle.ptr(bj), 64)
+    assert k.value == ko and ko > 7
+    np.testing.assert_array_equal(get("DAMAGE_BROKEN", (n, 12)), m.dmg_broken)
+    gi = np.zeros(64, np.int32)
+    gj = np.zeros(64, np.int32)
+    cnt = C.c_int()
+    P.check(L.lpmx_get_broken(P.ptr(gi), P.ptr(gj), 64, C.byref(cnt)), "broken")
+    assert cnt.value == 7
+    np.testing.assert_array_equal(gi[:7], bi[:7])
+    np.testing.assert_array_equal(gj[:7], bj[:7])
+
+
+def uniaxial(k, nsteps, strain):
+    L = P.sc_box_for(k)
+    return P.sc_uniaxial_steps(k - 1.0, np.linspace(strain / nsteps, strain, nsteps))
+
+
+@pytest.mark.parametrize("prec", [P.LPMX_FP64, P.LPMX_MIXED])
+def test_load_steps_match_oracle(oracle, prec):
+    k, nsteps = 6, 6
+    _, n, x, t, m = sc_pair(oracle, k)
+    steps = uniaxial(k, nsteps, 6e-3)
+    s = P.solver(precision=prec, rtol2=1e-24, atol2=0.0, maxit=100 * 3 * n, newton_tol=1e-11, newton_maxit=400,
+                 inner_rtol2=1e-8)
+    rc, out = P.run_steps(steps, s, out_axis="z", out_type=1)
+    assert rc == 0, P.lib().lpmx_last_error()
+    so = oracle.solver(rtol2=1e-24, atol2=0.0, maxit=100 * 3 * n, newton_tol=1e-11, newton_maxit=400, tangent=1)
+    rco, outo = oracle.run_steps(m, steps, so, out_axis="z", out_type=1)
+    assert rco == 0
+    xg = get("XYZ", (n, 3))
+    u, uo = xg - x, m.x - x
+    assert np.abs(u - uo).max() <= 1e-9 * np.abs(uo).max()
+    r = np.array([o.reaction for o in out])
+    ro = np.array([o.reaction for o in outo])
+    np.testing.assert_allclose(r, ro, rtol=1e-8)
+    dl = get("J2_ALPHA1", (n,))
+    assert dl.max() > 0, "the load path reaches the plastic regime"
+    np.testing.assert_allclose(dl, m.alpha[:, 1], rtol=0, atol=1e-10 * dl.max())
+    np.testing.assert_allclose(get("DAMAGE_NONLOCAL1", (n,)), m.dnl[:, 1], rtol=0, atol=1e-9 * m.dnl[:, 1].max())

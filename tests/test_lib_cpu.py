@@ -1,0 +1,97 @@
+"""CPU checks of liblpm.so: it loads, exports every entry point its headers declare,
+its host setup reproduces the reference lattice rules bit for bit, and its device
+entry points fail loudly when no GPU is present."""
+import ctypes as C
+import os
+import re
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from lpm_harness import P, ROOT
+
+HEADERS = ["lpm.h", "boundary.h", "constitutive.h", "data_handler.h", "initialization.h", "lpm_basic.h",
+           "neighbor.h", "solver.h", "stiffness.h", "lpmx.h", "lpm_ext.h"]
+
+
+def declared_functions():
+    names = set()
+    for h in HEADERS:
+        text = open(os.path.join(ROOT, "include", h)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"^[A-Za-z_][\w \*]*?\b([A-Za-z_]\w*)\s*\([^;{]*\)\s*;", text, flags=re.M):
+            name = m.group(1)
+            if name not in ("if", "while", "for", "return", "sizeof"):
+                names.add(name)
+    return names
+
+
+def test_library_exports_every_declared_function():
+    L = P.lib()
+    names = declared_functions()
+    assert len(names) > 100
+    missing = [n for n in sorted(names) if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_globals_are_exported():
+    for g in ("nparticle", "xyz", "K_global", "dLp", "J2_beta", "damage_D", "box", "R_matrix", "pbc"):
+        C.c_int.in_dll(P.lib(), g)
+
+
+@pytest.mark.parametrize("lat,box", [(2, [0, 9.151, 0, 9.151, 0, 9.151]), (1, [0, 20.0, 0, 17.0, 0, 1.0]),
+                                     (3, [0, 8.0, 0, 8.0, 0, 8.0]), (0, [0, 10.0, 0, 8.0, 0, 1.0]),
+                                     (4, [0, 6.0, 0, 6.0, 0, 6.0])])
+def test_create_cuboid_matches_oracle(oracle, lat, box):
+    n = P.make_lattice(lat, 0.5, box)
+    x = P.positions()
+    xo = oracle.create_lattice(lat, 0.5, box)
+    assert n == xo.shape[0] > 0
+    np.testing.assert_array_equal(x, xo)
+    lp = oracle.lattice_params(lat, 0.5, box[5] - box[4])
+    assert P.gint("nneighbors").value == lp.nbmax
+    assert P.gdouble("particle_volume").value == lp.volume
+    assert P.gdouble("neighbor2_cutoff").value == lp.cut2
+
+
+def test_sc_box_sizes_give_the_configured_counts(oracle):
+    for k, n in ((20, 8000), (60, 216000), (100, 1000000)):
+        L = P.SC_BOX[n]
+        assert oracle.create_lattice(2, 0.5, [0, L, 0, L, 0, L]).shape[0] == n
+    for k in (6, 8, 10):
+        L = P.sc_box_for(k)
+        assert oracle.create_lattice(2, 0.5, [0, L, 0, L, 0, L]).shape[0] == k ** 3
+
+
+def test_cblas_dnrm2_and_helpers():
+    rng = np.random.default_rng(0)
+    v = rng.standard_normal(1000) * 1e150
+    got = P.lib().cblas_dnrm2(1000, P.ptr(v), 1)
+    assert abs(got - np.linalg.norm(v)) <= 1e-14 * np.linalg.norm(v)
+    assert P.lib().cblas_dnrm2(3, P.ptr(np.array([3.0, 0.0, 4.0])), 1) == 5.0
+
+
+def test_modulus_field_statistics():
+    f = P.uniform_modulus(200000, 0)
+    assert f.min() >= 0.95 and f.max() <= 1.05
+    assert abs(f.mean() - 1.0) < 1e-3
+    np.testing.assert_array_equal(f, P.uniform_modulus(200000, 0))
+    assert not np.array_equal(f, P.uniform_modulus(200000, 2))
+
+
+def test_device_calls_fail_loudly_without_gpu():
+    if os.path.exists("/dev/kfd"):
+        pytest.skip("a GPU is present")
+    desc = P.ModelDesc(3, 2, 8, 18, 6, 12, 60, 0.5, 1.0, 1.41, 1.0)
+    x = np.zeros((8, 3))
+    assert P.lib().lpmx_model_create(C.byref(desc), P.ptr(x)) == -5
+    assert b"device" in P.lib().lpmx_last_error()
+    # the drop-in entry point exits with status 1, as the reference's fatal paths do
+    code = ("import sys; sys.path.insert(0, %r); from lpm_harness import P; "
+            "P.make_lattice(2, 0.5, [0, 5.125, 0, 5.125, 0, 5.125]); P.lib().searchNormalNeighbor()"
+            % os.path.join(ROOT, "tests"))
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 1
+    assert "device" in r.stderr
