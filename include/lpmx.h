@@ -105,8 +105,14 @@ int lpmx_tangent(void);
 int lpmx_export_upper_csr(int *IK, int *JK, double *K);
 /* Replace the device matrix values by a host upper CSR of the same pattern. */
 int lpmx_import_upper_csr(const double *K);
-/* setDispBC_stiffnessUpdate: zero constrained rows/columns, diagonal = |diag|_2. */
+/* setDispBC_stiffnessUpdate: zero constrained rows/columns, diagonal = |diag|_2 of
+ * the current matrix (the reference's rule, src/boundary.c:159-281). */
 int lpmx_bc_eliminate(void);
+/* fixed_diag = 1 keeps the diagonal value of the first elimination after each
+ * tangent assembly instead of re-deriving it from the already modified matrix
+ * (the reference's value grows by ~sqrt(#constrained DOFs) per call and overflows
+ * after ~150 Newton iterations of one step; it never changes the solution). */
+int lpmx_bc_eliminate_ex(int fixed_diag);
 
 /* ---- linear solve ------------------------------------------------------------------- */
 enum { LPMX_FP64 = 0, LPMX_MIXED = 1 };

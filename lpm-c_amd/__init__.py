@@ -339,26 +339,32 @@ def sc_box_for(k):
 
 def sc_types(x, radius=0.5):
     """Types of the uniaxial SC configurations: 1 top face, 2 bottom face, 3 the
-    bottom corner at minimum x and y (fully fixed), 0 elsewhere."""
+    bottom corner at minimum x and y (fully fixed), 4 the bottom corner at maximum x
+    and minimum y (held in y, which removes the rigid rotation about z that a face
+    fixed in z and one fixed corner leave free), 0 elsewhere."""
     z = x[:, 2]
     t = np.zeros(len(x), np.int32)
     t[z >= z.max() - 0.6 * radius] = 1
     t[z <= z.min() + 0.6 * radius] = 2
     bot = np.where(t == 2)[0]
-    corner = bot[np.lexsort((x[bot, 1], x[bot, 0]))[0]]
-    t[corner] = 3
+    a = bot[np.lexsort((x[bot, 0], x[bot, 1]))[0]]
+    b = bot[np.lexsort((-x[bot, 0], x[bot, 1]))[0]]
+    t[a] = 3
+    t[b] = 4
     return t
 
 
 def sc_uniaxial_steps(height, strains):
-    """Displacement steps: bottom (type 2) fixed in z, corner (type 3) fixed in x, y,
-    z, top (type 1) moved in z by the strain increments times the height."""
+    """Displacement steps: bottom (type 2) fixed in z, corner 3 fixed in x, y, z,
+    corner 4 fixed in y and z, top (type 1) moved in z by the strain increments
+    times the height."""
     steps = []
     prev = 0.0
     for e in strains:
         du = (e - prev) * height
         prev = e
-        steps.append(([(2, "z", 0.0), (3, "x", 0.0), (3, "y", 0.0), (3, "z", 0.0), (1, "z", du)], []))
+        steps.append(([(2, "z", 0.0), (3, "x", 0.0), (3, "y", 0.0), (3, "z", 0.0), (4, "y", 0.0), (4, "z", 0.0),
+                       (1, "z", du)], []))
     return steps
 
 
@@ -384,9 +390,9 @@ def build_sc_model(L, E=200e3, nu=0.3, sigmay=200.0, H=2000.0, emod=None, damage
     Lb.searchAFEMNeighbor()
     x = positions()
     t = sc_types(x)
-    set_types(t, 4)
+    set_types(t, 5)
     c = elastic_constants(E, nu)
-    set_materials([c] * 4, sigmay)
+    set_materials([c] * 5, sigmay)
     set_params(plmode=0, J2_H=H, J2_xi=0.0, damage_L=0.5, damage_threshold=0.9, damagec_A=0.0,
                critical_bstrain=1e-2, nbreak=20)
     if emod is not None:

@@ -56,7 +56,7 @@ extern "C" int lpmx_run_steps(const lpmx_solver *s, int nsteps, const int *nbd, 
             while (nres > s->newton_tol * tolm && ni < s->newton_maxit)
             {
                 RETCHK(lpmx_switch_state(0));
-                RETCHK(lpmx_bc_eliminate());
+                RETCHK(lpmx_bc_eliminate_ex(1));
                 double ts = now_s();
                 int it = 0, stt = 0;
                 double rr = 0;

@@ -7,6 +7,8 @@
 
 static double *t_A = nullptr, *t_B = nullptr; // shell sums of the elastic law
 static int *t_kstart = nullptr;               // K_pointer[i][1] (0-based starts), n+1
+static double t_nd = 0.0;                     // diagonal value of the constrained DOFs
+static bool t_nd_valid = false;               // computed since the last tangent assembly
 
 // Shell sums A_s = sum dL, B_s = sum Tv dL of every particle under the elastic law
 // (constitutive.c:241-261): dL = (|x_i - x_j| - L0 - dLp[0]) * damage_broken.
@@ -242,6 +244,7 @@ extern "C" int lpmx_tangent(void)
 {
     RETCHK(lpmx_require_pattern());
     RETCHK(ensure_tangent_buffers());
+    t_nd_valid = false;
     hipLaunchKernelGGL(k_elastic_sums, dim3(nblocks(g.n)), dim3(LPMX_BLOCK), 0, g_stream, g, t_A, t_B);
     LAUNCHCHK();
     if (g.dim == 3)
@@ -336,28 +339,33 @@ __global__ void k_bc_diag(DV v, double nd)
         }
 }
 
-extern "C" int lpmx_bc_eliminate(void)
+// fixed_diag = 0: the reference's rule, |diag(K)|_2 of the current matrix (which
+// already holds the previous call's value on constrained DOFs, so it grows by about
+// sqrt(#constrained DOFs) per call).  fixed_diag = 1: the value of the first call
+// after the tangent assembly, kept until the next assembly.  Constrained rows and
+// columns are decoupled, so the value never changes the solution.
+extern "C" int lpmx_bc_eliminate_ex(int fixed_diag)
 {
     RETCHK(lpmx_require_pattern());
     const int n = g.n, D = g.dim;
     double *sq = lpmx_scratch(sizeof(double) * (int64_t)D * n + 64);
     if (!sq)
         return lpmx_fail(LPMX_ERR_HIP, "scratch allocation failed");
-    // norm of the current diagonal first (the reference computes it before zeroing)
+    // the current diagonal first (the reference takes its norm before zeroing)
     if (D == 3)
         hipLaunchKernelGGL(k_diag_sq<3>, dim3(nblocks(n)), dim3(LPMX_BLOCK), 0, g_stream, g, sq);
     else
         hipLaunchKernelGGL(k_diag_sq<2>, dim3(nblocks(n)), dim3(LPMX_BLOCK), 0, g_stream, g, sq);
     LAUNCHCHK();
-    double ss = 0;
-    // sq is the scratch buffer; the reduction uses its own scratch region, so copy
-    double *tmp = nullptr;
-    HIPCHK(hipMalloc(&tmp, sizeof(double) * (int64_t)D * n));
-    HIPCHK(hipMemcpyAsync(tmp, sq, sizeof(double) * (int64_t)D * n, hipMemcpyDeviceToDevice, g_stream));
-    int rc = lpmx_dev_sumsq(tmp, (int64_t)D * n, &ss);
-    hipFree(tmp);
-    RETCHK(rc);
-    const double nd = sqrt(ss);
+    double nd = t_nd;
+    if (!(fixed_diag && t_nd_valid))
+    {
+        double ss = 0;
+        RETCHK(lpmx_dev_sumsq(sq, (int64_t)D * n, &ss));
+        nd = sqrt(ss);
+        t_nd = nd;
+        t_nd_valid = true;
+    }
     if (D == 3)
     {
         hipLaunchKernelGGL(k_bc_zero<3>, dim3(g.nslices), dim3(64), 0, g_stream, g);
@@ -374,6 +382,8 @@ extern "C" int lpmx_bc_eliminate(void)
     return LPMX_OK;
 }
 
+extern "C" int lpmx_bc_eliminate(void) { return lpmx_bc_eliminate_ex(0); }
+
 // ------------------------------------------------------ block-Jacobi inverse -----
 template <int D>
 __global__ void k_bj_inverse(DV v)
@@ -386,7 +396,7 @@ __global__ void k_bj_inverse(DV v)
     double a[D * D], inv[D * D];
     for (int e = 0; e < D * D; e++)
         a[e] = v.K[(slot * D * D + e) * LPMX_SLICE + l];
-    if (D == 3)
+    if constexpr (D == 3)
     {
         inv[0] = a[4] * a[8] - a[5] * a[7];
         inv[1] = a[2] * a[7] - a[1] * a[8];

@@ -124,6 +124,7 @@ typedef struct orc_solver
     double newton_tol;
     int newton_maxit;
     int tangent;         /* 0: forward FD (reference), 1: analytic elastic Jacobian */
+    int fixed_diag;      /* 1: constrained-DOF diagonal fixed per tangent assembly */
     int nthreads;        /* >1 only for timing (CPU baseline); 1 is the specification */
 } orc_solver;
 
@@ -168,6 +169,7 @@ void orc_update_rr(orc_model *m);
 void orc_set_disp_bc(orc_model *m, int nbd, const orc_dispbc *bc);
 void orc_set_force_bc(orc_model *m, int nbf, const orc_forcebc *bc);
 void orc_bc_stiffness_update(orc_model *m);
+void orc_bc_stiffness_update_nd(orc_model *m, double nd);
 double orc_nrm2(int n, const double *x);
 
 /* ---- linear solve ----------------------------------------------------------- */

@@ -874,6 +874,13 @@ void orc_bc_stiffness_update(orc_model *m)
     }
     double nd = orc_nrm2(d * m->n, diag);
     free(diag);
+    orc_bc_stiffness_update_nd(m, nd);
+}
+
+/* The same elimination with a given diagonal value for the constrained DOFs. */
+void orc_bc_stiffness_update_nd(orc_model *m, double nd)
+{
+    const int d = m->dim;
     for (int i = 0; i < m->n; i++)
     {
         int64_t p = row_start(m, i);

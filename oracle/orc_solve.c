@@ -353,6 +353,22 @@ double orc_mean_disp(const orc_model *m, char axis, int t1)
     return (p - p0) / num;
 }
 
+/* The diagonal value the last elimination put on a constrained DOF (-1 if none). */
+static double first_bc_diag(const orc_model *m)
+{
+    const int d = m->dim;
+    for (int i = 0; i < m->n; i++)
+        for (int c = 0; c < d; c++)
+            if (m->dispBC[d * i + c] == 0 || m->fixidx[d * i + c] == 0)
+            {
+                int64_t p = m->kptr[2 * i + 1];
+                int k = m->kptr[2 * i];
+                int64_t pos = c == 0 ? p : (c == 1 ? p + d * k : p + 2 * d * k - 1);
+                return m->K[pos];
+            }
+    return -1.0;
+}
+
 static int damage_general(orc_model *m)
 {
     if (m->plmode == 0)
@@ -392,10 +408,17 @@ int orc_run_steps(orc_model *m, const orc_solver *s, int nsteps,
             double nrea = orc_nrm2(m->nreaction, m->reaction);
             double tolm = nres > nrea ? nres : nrea;
             int ni = 0;
+            double nd = -1.0;
             while (nres > s->newton_tol * tolm && ni < s->newton_maxit)
             {
                 orc_switch_state(m, 0);
-                orc_bc_stiffness_update(m);
+                if (s->fixed_diag && nd >= 0.0)
+                    orc_bc_stiffness_update_nd(m, nd);
+                else
+                {
+                    orc_bc_stiffness_update(m);
+                    nd = first_bc_diag(m);
+                }
                 int it = orc_solve_cg(m, s, NULL);
                 o.cg_iters += it < 0 ? -it : it;
                 orc_bond_force_general(m, m->plmode);

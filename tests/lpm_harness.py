@@ -30,7 +30,7 @@ def sc_pair(O, k, E=200e3, nu=0.3, sigmay=200.0, H=2000.0, emod=None):
     the oracle: k^3 particles, uniaxial types, J2 mixed-linear material."""
     L = P.sc_box_for(k)
     n, x, t, c = P.build_sc_model(L, E=E, nu=nu, sigmay=sigmay, H=H, emod=emod)
-    m = oracle_model(O, x, 2, t, [c] * 4, plmode=0, sigmay=sigmay, J2_H=H, emod=emod)
+    m = oracle_model(O, x, 2, t, [c] * 5, plmode=0, sigmay=sigmay, J2_H=H, emod=emod)
     return L, n, x, t, m
 
 
