@@ -1,0 +1,257 @@
+"""Throughput of the implicit load-step loop on one MI355X.
+
+Workload: simple cubic lattice, 100 x 100 x 100 = 1,000,000 particles (3M DOF),
+J2 mixed-linear elastoplastic (E = 200 GPa, nu = 0.3, yield 200 MPa, H = 2 GPa),
+per-particle modulus 1 + 0.05 U(-1, 1) (splitmix64, seed 0), uniaxial tension to 2%
+in 50 load steps (bottom face held in z, 3-2-1 corner constraints, top face moved).
+
+One bench step is one load step of that schedule: `--warmup W` load steps run
+untimed, the next `--steps K` are timed.  Both precision forms run the same steps
+from a fresh model: the FP64 form (block-Jacobi PCG in FP64) and the mixed form
+(FP32 matrix values, vectors and SpMV, compensated FP32 dots, FP64 iterative
+refinement); Newton converges both to 1e-8 relative residual, the linear solves stop
+by the reference's rule |r|^2 <= 1e-8 |r0|^2 + 1e-12.  `value` is the mixed form's load steps/s.
+
+roofline: the CG SpMV kernel (k_cg_spmv), timed with HIP events on the library's
+stream during the timed steps; achieved = algorithmic bytes per launch / mean
+launch time.  cpu_baseline: the oracle's threaded CG (reference solverCG algorithm)
+on the same 1M matrix, a bounded number of iterations on the host cores, in
+DOF-iters/s.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+METRIC = "load steps/s & DOF-iters/s, 1M-particle SC lattice; mixed vs FP64 at 1e-8 err"
+HBM_PEAK_GBS = 8000.0
+NSTEPS_SCHEDULE = 50
+STRAIN_END = 0.02
+EDGE = 100
+BOX = 99.195  # SC box edge giving 100 particles per edge (radius 0.5)
+# linear solves: the reference's CG stopping rule |r|^2 <= 1e-8 |r0|^2 + 1e-12
+RTOL2, ATOL2 = 1e-8, 1e-12
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def product():
+    from lpm_harness import P  # loads lpm-c_amd/liblpm.so; fails loudly without it
+
+    return P
+
+
+def schedule(P):
+    strains = np.arange(1, NSTEPS_SCHEDULE + 1) * (STRAIN_END / NSTEPS_SCHEDULE)
+    return P.sc_uniaxial_steps(EDGE - 1.0, strains)
+
+
+def run_form(P, prec, warmup, steps, profile):
+    L = P.lib()
+    t0 = time.time()
+    emod = P.uniform_modulus(EDGE ** 3, 0)
+    n, x, _, _ = P.build_sc_model(BOX, emod=emod)
+    log(f"form {prec}: model of {n} particles built in {time.time() - t0:.1f} s")
+    sched = schedule(P)
+    s = P.solver(precision=prec, rtol2=RTOL2, atol2=ATOL2, maxit=20000, newton_tol=1e-8, newton_maxit=100,
+                 inner_rtol2=1e-12, ir_maxit=12)
+    if warmup:
+        rc, wout = P.run_steps(sched[:warmup], s)
+        P.check(rc, "warmup steps")
+        log(f"form {prec}: warmup {warmup} steps, Newton {[o.newton_iters for o in wout]}")
+    sync()
+    if profile:
+        L.lpmx_profile_spmv(1)
+    t1 = time.perf_counter()
+    out, rc = [], 0
+    for k in range(warmup, warmup + steps):  # one call per load step, for progress lines
+        rck, o = P.run_steps(sched[k:k + 1], s)
+        out += o
+        rc = rc or rck
+        log(f"form {prec}: step {k + 1}: Newton {o[0].newton_iters}, CG {o[0].cg_iters}, "
+            f"{o[0].t_total:.2f} s, status {o[0].status}")
+    sync()
+    wall = time.perf_counter() - t1
+    prof = None
+    if profile:
+        ms, cnt = C.c_double(), C.c_int64()
+        L.lpmx_profile_spmv_read(C.byref(ms), C.byref(cnt))
+        L.lpmx_profile_spmv(0)
+        by = C.c_double()
+        P.check(L.lpmx_spmv_bytes(prec, C.byref(by)), "spmv bytes")
+        prof = (ms.value, cnt.value, by.value)
+    if rc != 0:
+        log(f"form {prec}: {L.lpmx_last_error().decode()} "
+            f"{[(o.status, o.newton_iters, o.cg_iters) for o in out]}")
+    u = P.field("XYZ", 3 * n).reshape(n, 3) - x
+    return dict(n=n, wall=wall, out=out, rc=rc, prof=prof, u=u)
+
+
+def sync():
+    try:
+        import torch
+
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
+    except ImportError:
+        pass
+
+
+def summarize(r, n):
+    out = r["out"]
+    t_solve = sum(o.t_solve for o in out)
+    cg = sum(o.cg_iters for o in out)
+    k = len(out)
+    return {
+        "load_steps_per_s": k / r["wall"],
+        "ms_per_step": 1e3 * r["wall"] / k,
+        "dof_iters_per_s": 3.0 * n * cg / t_solve if t_solve > 0 else None,
+        "newton_iters": [o.newton_iters for o in out],
+        "cg_iters": [o.cg_iters for o in out],
+        "ir_passes": [o.ir_passes for o in out],
+        "status": [o.status for o in out],
+        "reaction": [o.reaction for o in out],
+        "t_solve_s": t_solve,
+        "t_tangent_s": sum(o.t_tangent for o in out),
+        "t_force_s": sum(o.t_force for o in out),
+    }
+
+
+def roofline(prof):
+    ms, cnt, by = prof
+    if cnt <= 0:
+        return None
+    avg = ms / cnt
+    ach = by / (avg * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+            "traffic": None, "kernel": "k_cg_spmv", "algorithmic_bytes": by, "avg_launch_ms": avg,
+            "launches_timed": cnt}
+
+
+def cpu_baseline(P, budget_s):
+    """The oracle's threaded CG on the device's current (boundary-eliminated) matrix."""
+    import oracle_py as O
+
+    L = P.lib()
+    n = P.gint("nparticle").value
+    d = P.gint("dim").value
+    nK = P.gptr("K_pointer", 2, C.c_int)[n][1]
+    IK = np.zeros(d * n + 1, np.int32)
+    JK = np.zeros(nK, np.int32)
+    K = np.zeros(nK)
+    P.check(L.lpmx_export_upper_csr(P.ptr(IK), P.ptr(JK), P.ptr(K)), "export")
+    rhs = np.random.default_rng(7).standard_normal(d * n)
+    m = O.OrcModel()
+    m.dim, m.n = d, n
+    m.IK, m.JK, m.K, m.residual = O.ptr(IK), O.ptr(JK), O.ptr(K), O.ptr(rhs)
+    m.nK = nK
+    lo = O.lib()
+    lo.orc_cg_baseline.argtypes = [C.POINTER(O.OrcModel), C.c_int, C.c_int, C.POINTER(C.c_double)]
+    lo.orc_cg_baseline.restype = C.c_double
+    cores = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+    cores = max(1, min(cores, 16))
+    rr = C.c_double()
+    t5 = lo.orc_cg_baseline(C.byref(m), cores, 5, C.byref(rr))
+    iters = int(max(10, min(2000, budget_s / max(t5 / 5, 1e-6))))
+    t = lo.orc_cg_baseline(C.byref(m), cores, iters, C.byref(rr))
+    if t <= 0:
+        return None
+    return {"value": d * n * iters / t, "unit": "DOF-iters/s", "cores": cores, "kind": "port",
+            "sample": f"{iters} iterations of the oracle's unpreconditioned CG (reference solverCG algorithm, "
+                      f"full CSR expanded from the upper CSR, OpenMP) on the 1M-particle tangent "
+                      f"({d * n} DOF, {nK} upper entries), {t:.1f} s"}
+
+
+def copy_probe():
+    """A plain copy of known size, the scale of the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+
+    a = torch.ones(256 * 1024 * 1024, dtype=torch.float64, device="cuda")  # 2 GiB
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_each_way": a.numel() * 8, "launches": 5}))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--forms", default="fp64,mixed")
+    ap.add_argument("--copy-probe", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {a.gpus}): the load-step loop here is a single-GPU "
+              f"path; multi-GPU sharding is not part of this build.", file=sys.stderr)
+        sys.exit(2)
+    if a.copy_probe:
+        copy_probe()
+        return
+    if a.warmup + a.steps > NSTEPS_SCHEDULE or a.steps < 1 or a.warmup < 0:
+        sys.exit(f"--warmup + --steps must lie in 1..{NSTEPS_SCHEDULE}")
+    P = product()
+    forms = [f.strip() for f in a.forms.split(",") if f.strip()]
+    res = {}
+    for f in forms:
+        prec = P.LPMX_FP64 if f == "fp64" else P.LPMX_MIXED
+        res[f] = run_form(P, prec, a.warmup, a.steps, profile=True)
+        log(f"{f}: {a.steps} steps in {res[f]['wall']:.2f} s")
+    main_form = "mixed" if "mixed" in res else forms[0]
+    r = res[main_form]
+    n = r["n"]
+    sm = {f: summarize(res[f], n) for f in res}
+    line = {
+        "metric": METRIC,
+        "value": sm[main_form]["load_steps_per_s"],
+        "unit": "load steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": sm[main_form]["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64" if main_form == "mixed" else "f64",
+        "data": "synthetic",
+        "config": {"workload": f"SC lattice {EDGE}^3 = {n} particles ({3 * n} DOF), J2 elastoplastic, "
+                               f"uniaxial tension to 2% in {NSTEPS_SCHEDULE} load steps, E +/-5% (seed 0); "
+                               f"timed load steps {a.warmup + 1}..{a.warmup + a.steps}",
+                   "form": main_form, "newton_tol": 1e-8, "newton_maxit": 100, "linear_rtol2": RTOL2,
+                   "preconditioner": "block-Jacobi 3x3"},
+        "dof_iters_per_s": sm[main_form]["dof_iters_per_s"],
+        "forms": sm,
+    }
+    if "fp64" in res and "mixed" in res:
+        u64, um = res["fp64"]["u"], res["mixed"]["u"]
+        line["mixed_speedup_vs_fp64"] = sm["mixed"]["load_steps_per_s"] / sm["fp64"]["load_steps_per_s"]
+        line["mixed_disp_rel_err"] = float(np.abs(um - u64).max() / np.abs(u64).max())
+        r64 = np.array(sm["fp64"]["reaction"])
+        rm = np.array(sm["mixed"]["reaction"])
+        line["mixed_reaction_rel_err"] = float(np.abs(rm - r64).max() / np.abs(r64).max())
+    line["roofline"] = roofline(r["prof"]) if r["prof"] else None
+    if "fp64" in res and main_form != "fp64" and res["fp64"]["prof"]:
+        line["roofline_fp64"] = roofline(res["fp64"]["prof"])
+    if not a.no_cpu:
+        t0 = time.time()
+        line["cpu_baseline"] = cpu_baseline(P, a.cpu_seconds)
+        log(f"cpu baseline in {time.time() - t0:.1f} s")
+    else:
+        line["cpu_baseline"] = None
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -121,9 +121,11 @@ enum { LPMX_FP64 = 0, LPMX_MIXED = 1 };
  * LPMX_MIXED runs FP32 values, vectors and compensated dots inside FP64 iterative
  * refinement.  *iters gets the inner CG iterations, *rr the final |r|^2. */
 int lpmx_solve(int precision, double rtol2, double atol2, int maxit, int *iters, double *rr);
-/* The same with the mixed form's inner FP32 target (squared-norm ratio per
- * refinement pass) and pass cap; *status: 1 converged, 2 iteration cap,
- * 3 breakdown (p.Kp <= 0).  Returns LPMX_OK whatever the status. */
+/* The same with the mixed form's refinement settings: each pass runs FP32 PCG on the
+ * current FP64 residual r_k to the squared-norm ratio max(inner_rtol2,
+ * 0.01 (rtol2 |r0|^2 + atol2) / |r_k|^2), at most ir_maxit passes; *status:
+ * 1 converged, 2 iteration cap, 3 breakdown (p.Kp <= 0).  Returns LPMX_OK whatever
+ * the status. */
 int lpmx_solve_ex(int precision, double rtol2, double atol2, int maxit, double inner_rtol2, int ir_maxit,
                   int *iters, double *rr, int *status);
 
@@ -162,7 +164,7 @@ typedef struct lpmx_solver
     int maxit;              /* CG cap (reference: n*dim) */
     double newton_tol;      /* reference TOLITER */
     int newton_maxit;       /* reference MAXITER */
-    double inner_rtol2;     /* mixed form: FP32 inner solve target on squared norms */
+    double inner_rtol2;     /* mixed form: floor of the FP32 inner target (squared norms) */
     int ir_maxit;           /* mixed form: FP64 refinement passes per solve */
 } lpmx_solver;
 
@@ -179,10 +181,15 @@ int lpmx_run_steps(const lpmx_solver *s, int nsteps, const int *nbd, const lpmx_
                    lpmx_step_stats *out);
 
 /* ---- measurement ---------------------------------------------------------------------- */
-/* Time `reps` launches of the block SpMV y = K x (FP64 or FP32) with HIP events on the
- * library's stream; *avg_ms gets the mean duration, *bytes the algorithmic bytes per
- * launch (nnzb*(9*s_val+4) + 4*(n+1) + 2*n*dim*s_vec). */
+/* Time `reps` launches of the CG's block SpMV kernel (p = z + beta p_old gathered on the
+ * fly, q = K p, partial p.q; FP64 or FP32) with HIP events on the library's stream;
+ * *avg_ms gets the mean duration, *bytes the algorithmic bytes per launch as returned
+ * by lpmx_spmv_bytes. */
 int lpmx_bench_spmv(int precision, int reps, double *avg_ms, double *bytes);
+/* Algorithmic HBM bytes of one CG SpMV launch on the current pattern:
+ * nnzb*(dim^2*s_val + 4) + 4*(nslices+1) + 4*n*dim*s_vec (z and p_old read, p and q
+ * written once; s = 8 for FP64, 4 for the mixed form). */
+int lpmx_spmv_bytes(int precision, double *bytes);
 /* Counters of the device matrix: block rows, stored blocks (nnzb), slots incl. padding. */
 int lpmx_matrix_info(int64_t *nnzb, int64_t *padded_slots, int *nslices);
 /* y = K x with the device matrix (FP64 or FP32 values), vectors of n*dim doubles. */

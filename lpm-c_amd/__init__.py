@@ -117,6 +117,7 @@ def lib():
         "lpmx_run_steps": [C.POINTER(Solver), C.c_int, _ip, C.POINTER(DispBC), _ip, C.POINTER(ForceBC), C.c_int,
                            C.c_char, C.c_int, C.POINTER(StepStats)],
         "lpmx_bench_spmv": [C.c_int, C.c_int, _dp, _dp],
+        "lpmx_spmv_bytes": [C.c_int, _dp],
         "lpmx_matrix_info": [C.POINTER(C.c_int64), C.POINTER(C.c_int64), _ip],
         "lpmx_spmv": [C.c_int, _dp, _dp],
     }
@@ -285,7 +286,7 @@ def set_params(plmode=0, J2_H=0.0, J2_xi=0.0, damage_L=0.5, damage_threshold=0.9
 
 
 def solver(precision=LPMX_FP64, rtol2=1e-8, atol2=1e-12, maxit=None, newton_tol=1e-4, newton_maxit=100,
-           inner_rtol2=1e-8, ir_maxit=12):
+           inner_rtol2=1e-12, ir_maxit=12):
     n = gint("nparticle").value * gint("dim").value
     return Solver(precision, rtol2, atol2, maxit if maxit is not None else n, newton_tol, newton_maxit,
                   inner_rtol2, ir_maxit)

@@ -175,6 +175,8 @@ double orc_nrm2(int n, const double *x);
 /* ---- linear solve ----------------------------------------------------------- */
 void orc_spmv_upper(const orc_model *m, const double *x, double *y);
 int orc_solve_cg(orc_model *m, const orc_solver *s, double *final_rr);
+/* Timing only (CPU baseline): seconds of `iters` threaded CG iterations. */
+double orc_cg_baseline(const orc_model *m, int nthreads, int iters, double *rr_out);
 
 /* ---- damage and cracks ------------------------------------------------------ */
 int orc_damage_pwise_nonlocal(orc_model *m);

@@ -134,6 +134,97 @@ int orc_solve_cg(orc_model *m, const orc_solver *s, double *final_rr)
     return conv ? it : -it;
 }
 
+/* CPU timing baseline of the reference's linear solve (solverCG, src/solver.c:329-411:
+ * unpreconditioned CG, the rate-setting part of a load step) on host cores.  The
+ * symmetric upper CSR is expanded once to full CSR (untimed), as a threaded
+ * symmetric SpMV would otherwise need per-thread scatter buffers; then `iters` CG
+ * iterations from disp = 0 on m->residual run with the SpMV, dots and updates
+ * spread over `nthreads` OpenMP threads.  Returns the wall seconds of those
+ * iterations (negative on allocation failure); *rr_out gets the last |r|^2.
+ * Timing only: the summation order is not the specification's. */
+double orc_cg_baseline(const orc_model *m, int nthreads, int iters, double *rr_out)
+{
+    const int nr = m->dim * m->n;
+    const int nt = nthreads > 0 ? nthreads : 1;
+    int64_t *rp = calloc((size_t)nr + 1, sizeof(int64_t));
+    if (!rp)
+        return -1.0;
+    for (int r = 0; r < nr; r++)
+        for (int p = m->IK[r] - 1; p < m->IK[r + 1] - 1; p++)
+        {
+            int c = m->JK[p] - 1;
+            rp[r + 1]++;
+            if (c != r)
+                rp[c + 1]++;
+        }
+    for (int r = 0; r < nr; r++)
+        rp[r + 1] += rp[r];
+    int *ci = malloc(sizeof(int) * (size_t)rp[nr]);
+    double *va = malloc(sizeof(double) * (size_t)rp[nr]);
+    int64_t *fill = malloc(sizeof(int64_t) * (size_t)nr);
+    double *x = malloc(sizeof(double) * nr), *r = malloc(sizeof(double) * nr), *p = malloc(sizeof(double) * nr),
+           *q = malloc(sizeof(double) * nr);
+    if (!ci || !va || !fill || !x || !r || !p || !q)
+    {
+        free(rp), free(ci), free(va), free(fill), free(x), free(r), free(p), free(q);
+        return -1.0;
+    }
+    memcpy(fill, rp, sizeof(int64_t) * nr);
+    for (int row = 0; row < nr; row++)
+        for (int k = m->IK[row] - 1; k < m->IK[row + 1] - 1; k++)
+        {
+            int c = m->JK[k] - 1;
+            ci[fill[row]] = c, va[fill[row]++] = m->K[k];
+            if (c != row)
+                ci[fill[c]] = row, va[fill[c]++] = m->K[k];
+        }
+    double rr = 0.0;
+#pragma omp parallel for num_threads(nt) reduction(+ : rr) schedule(static)
+    for (int k = 0; k < nr; k++)
+    {
+        x[k] = 0.0, r[k] = m->residual[k], p[k] = r[k];
+        rr += r[k] * r[k];
+    }
+#ifdef _OPENMP
+    const double t0 = omp_get_wtime();
+#endif
+    for (int it = 0; it < iters; it++)
+    {
+        double pq = 0.0;
+#pragma omp parallel for num_threads(nt) reduction(+ : pq) schedule(static)
+        for (int row = 0; row < nr; row++)
+        {
+            double acc = 0.0;
+            for (int64_t k = rp[row]; k < rp[row + 1]; k++)
+                acc += va[k] * p[ci[k]];
+            q[row] = acc;
+            pq += p[row] * acc;
+        }
+        const double alpha = rr / pq;
+        double rrn = 0.0;
+#pragma omp parallel for num_threads(nt) reduction(+ : rrn) schedule(static)
+        for (int k = 0; k < nr; k++)
+        {
+            x[k] += alpha * p[k];
+            r[k] -= alpha * q[k];
+            rrn += r[k] * r[k];
+        }
+        const double beta = rrn / rr;
+        rr = rrn;
+#pragma omp parallel for num_threads(nt) schedule(static)
+        for (int k = 0; k < nr; k++)
+            p[k] = r[k] + beta * p[k];
+    }
+    double secs = -1.0;
+#ifdef _OPENMP
+    secs = omp_get_wtime() - t0;
+#endif
+    if (rr_out)
+        *rr_out = rr;
+    free(rp), free(ci), free(va), free(fill), free(x), free(r), free(p), free(q);
+    return secs;
+}
+
 /* updateDuctileDamagePwiseNonlocal (reference src/constitutive.c:1757-1862):
  * Gaussian-weighted average of dlambda*(1 + A*triaxiality) over all particles
  * within 3L in the initial configuration (index order), bond breakage when either

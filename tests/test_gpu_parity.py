@@ -159,7 +159,7 @@ def test_bc_elimination_and_solve(oracle):
     dense = oracle.upper_csr_dense(m)
     want = np.linalg.solve(dense, m.residual)
     for prec, tol in ((P.LPMX_FP64, 1e-10), (P.LPMX_MIXED, 1e-10)):
-        s = P.solver(precision=prec, rtol2=1e-26, atol2=0.0, maxit=50 * 3 * n, inner_rtol2=1e-8)
+        s = P.solver(precision=prec, rtol2=1e-26, atol2=0.0, maxit=50 * 3 * n, inner_rtol2=1e-12)
         L.lpm_ext_set_solver(C.byref(s))
         P.write1d("residual", m.residual)
         x0 = P.positions()
@@ -252,15 +252,17 @@ def test_brittle_selection_bit_exact(oracle):
 
 @pytest.mark.parametrize("prec", [P.LPMX_FP64, P.LPMX_MIXED])
 def test_load_steps_match_oracle(oracle, prec):
-    """An elastic step and three plastic steps of uniaxial tension (0.05% .. 0.2%),
-    Newton and CG run to round-off on both sides."""
+    """An elastic step and three plastic steps of uniaxial tension (0.05% .. 0.2%).
+    Newton runs to 1e-11 on both sides; the FP64 CG to 1e-12 relative residual, the
+    mixed form's refinement to 1e-8 (its FP64 residual cannot be driven further
+    through FP32 corrections), which leaves the converged Newton solution unchanged."""
     k = 6
     _, n, x, t, m = sc_pair(oracle, k)
     steps = P.sc_uniaxial_steps(k - 1.0, np.array([0.5e-3, 1e-3, 1.5e-3, 2e-3]))
-    s = P.solver(precision=prec, rtol2=1e-24, atol2=0.0, maxit=100 * 3 * n, newton_tol=1e-11, newton_maxit=600,
-                 inner_rtol2=1e-8)
+    s = P.solver(precision=prec, rtol2=1e-24 if prec == P.LPMX_FP64 else 1e-16, atol2=0.0, maxit=100 * 3 * n,
+                 newton_tol=1e-11, newton_maxit=600, inner_rtol2=1e-12)
     rc, out = P.run_steps(steps, s, out_axis="z", out_type=1)
-    assert rc == 0, P.lib().lpmx_last_error()
+    assert rc == 0, (P.lib().lpmx_last_error(), [(o.status, o.newton_iters, o.cg_iters, o.ir_passes) for o in out])
     so = oracle.solver(rtol2=1e-24, atol2=0.0, maxit=100 * 3 * n, newton_tol=1e-11, newton_maxit=600, tangent=1,
                        fixed_diag=1)
     rco, outo = oracle.run_steps(m, steps, so, out_axis="z", out_type=1)

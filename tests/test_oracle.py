@@ -7,6 +7,8 @@ transposed operand fails a check.
 """
 import itertools
 
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -285,6 +287,17 @@ def test_bc_elimination_and_cg_vs_dense(oracle):
     assert it > 0
     np.testing.assert_allclose(m.disp, want, rtol=0, atol=1e-10 * np.abs(want).max())
     np.testing.assert_allclose(m.x - x0, m.disp.reshape(n, 3), rtol=0, atol=1e-15)
+    # the threaded timing baseline runs the same CG recurrence on the full CSR
+    for k in (1, 7, 30):
+        s1 = oracle.solver(rtol2=0.0, atol2=0.0, maxit=k)
+        m.residual[:] = rng.standard_normal(3 * n) * m.dispBC
+        rr_ser = C.c_double()
+        oracle.lib().orc_solve_cg(m.p, s1, C.byref(rr_ser))
+        rr_par = C.c_double()
+        oracle.lib().orc_cg_baseline.restype = C.c_double
+        t = oracle.lib().orc_cg_baseline(m.p, 4, k, C.byref(rr_par))
+        assert t >= 0.0
+        assert abs(rr_par.value - rr_ser.value) <= 1e-9 * rr_ser.value
 
 
 # ---------------------------------------------------------------- damage ---------
