@@ -55,15 +55,15 @@ def schedule(P):
     return P.sc_uniaxial_steps(EDGE - 1.0, strains)
 
 
-def run_form(P, prec, warmup, steps, profile):
+def run_form(P, prec, warmup, steps, profile, accel_depth, newton_tol, newton_maxit):
     L = P.lib()
     t0 = time.time()
     emod = P.uniform_modulus(EDGE ** 3, 0)
     n, x, _, _ = P.build_sc_model(BOX, emod=emod)
     log(f"form {prec}: model of {n} particles built in {time.time() - t0:.1f} s")
     sched = schedule(P)
-    s = P.solver(precision=prec, rtol2=RTOL2, atol2=ATOL2, maxit=20000, newton_tol=1e-8, newton_maxit=100,
-                 inner_rtol2=1e-12, ir_maxit=12)
+    s = P.solver(precision=prec, rtol2=RTOL2, atol2=ATOL2, maxit=20000, newton_tol=newton_tol, newton_maxit=newton_maxit,
+                 inner_rtol2=1e-2, ir_maxit=100, accel_depth=accel_depth)
     if warmup:
         rc, wout = P.run_steps(sched[:warmup], s)
         P.check(rc, "warmup steps")
@@ -78,7 +78,8 @@ def run_form(P, prec, warmup, steps, profile):
         out += o
         rc = rc or rck
         log(f"form {prec}: step {k + 1}: Newton {o[0].newton_iters}, CG {o[0].cg_iters}, "
-            f"{o[0].t_total:.2f} s, status {o[0].status}")
+            f"{o[0].t_total:.2f} s, status {o[0].status}, accelerated {o[0].accel_mixed}, "
+            f"reverted {o[0].accel_reverts}, residual {o[0].res_norm / max(o[0].tol_mult, 1e-300):.2e}")
     sync()
     wall = time.perf_counter() - t1
     prof = None
@@ -126,15 +127,32 @@ def summarize(r, n):
     }
 
 
-def roofline(prof):
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_cg_spmv_sc1m_hbm_traffic.md")
+
+
+def measured_traffic(form):
+    """HBM bytes per k_cg_spmv launch from the committed PMC summary (FETCH_SIZE and
+    WRITE_SIZE, scaled by a copy of known size), or None."""
+    try:
+        for line in open(TRAFFIC_FILE):
+            f = line.split()
+            if len(f) == 3 and f[0] == "traffic_bytes" and f[1] == form:
+                return float(f[2])
+    except OSError:
+        pass
+    return None
+
+
+def roofline(prof, form):
     ms, cnt, by = prof
     if cnt <= 0:
         return None
     avg = ms / cnt
     ach = by / (avg * 1e-3) / 1e9
     return {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
-            "traffic": None, "kernel": "k_cg_spmv", "algorithmic_bytes": by, "avg_launch_ms": avg,
-            "launches_timed": cnt}
+            "traffic": measured_traffic(form), "kernel": "k_cg_spmv", "algorithmic_bytes": by,
+            "avg_launch_ms": avg, "launches_timed": cnt,
+            "traffic_source": "profiles/k_cg_spmv_sc1m_hbm_traffic.md (bytes per launch)"}
 
 
 def cpu_baseline(P, budget_s):
@@ -187,11 +205,16 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--forms", default="fp64,mixed")
     ap.add_argument("--copy-probe", action="store_true")
+    ap.add_argument("--newton-tol", type=float, default=1e-8,
+                    help="Newton test |R| <= tol * max(|R0|, |reaction|) (reference TOLITER semantics)")
+    ap.add_argument("--newton-maxit", type=int, default=100, help="Newton cap (reference MAXITER = 100)")
+    ap.add_argument("--accel-depth", type=int, default=5,
+                    help="Anderson acceleration depth of the Newton loop (0: modified Newton)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {a.gpus}): the load-step loop here is a single-GPU "
@@ -207,7 +230,8 @@ def main():
     res = {}
     for f in forms:
         prec = P.LPMX_FP64 if f == "fp64" else P.LPMX_MIXED
-        res[f] = run_form(P, prec, a.warmup, a.steps, profile=True)
+        res[f] = run_form(P, prec, a.warmup, a.steps, profile=True, accel_depth=a.accel_depth,
+                          newton_tol=a.newton_tol, newton_maxit=a.newton_maxit)
         log(f"{f}: {a.steps} steps in {res[f]['wall']:.2f} s")
     main_form = "mixed" if "mixed" in res else forms[0]
     r = res[main_form]
@@ -229,8 +253,8 @@ def main():
         "config": {"workload": f"SC lattice {EDGE}^3 = {n} particles ({3 * n} DOF), J2 elastoplastic, "
                                f"uniaxial tension to 2% in {NSTEPS_SCHEDULE} load steps, E +/-5% (seed 0); "
                                f"timed load steps {a.warmup + 1}..{a.warmup + a.steps}",
-                   "form": main_form, "newton_tol": 1e-8, "newton_maxit": 100, "linear_rtol2": RTOL2,
-                   "preconditioner": "block-Jacobi 3x3"},
+                   "form": main_form, "newton_tol": a.newton_tol, "newton_maxit": a.newton_maxit, "linear_rtol2": RTOL2,
+                   "preconditioner": "block-Jacobi 3x3", "newton_accel_depth": a.accel_depth},
         "dof_iters_per_s": sm[main_form]["dof_iters_per_s"],
         "forms": sm,
     }
@@ -241,9 +265,9 @@ def main():
         r64 = np.array(sm["fp64"]["reaction"])
         rm = np.array(sm["mixed"]["reaction"])
         line["mixed_reaction_rel_err"] = float(np.abs(rm - r64).max() / np.abs(r64).max())
-    line["roofline"] = roofline(r["prof"]) if r["prof"] else None
+    line["roofline"] = roofline(r["prof"], main_form) if r["prof"] else None
     if "fp64" in res and main_form != "fp64" and res["fp64"]["prof"]:
-        line["roofline_fp64"] = roofline(res["fp64"]["prof"])
+        line["roofline_fp64"] = roofline(res["fp64"]["prof"], "fp64")
     if not a.no_cpu:
         t0 = time.time()
         line["cpu_baseline"] = cpu_baseline(P, a.cpu_seconds)

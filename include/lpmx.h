@@ -121,11 +121,12 @@ enum { LPMX_FP64 = 0, LPMX_MIXED = 1 };
  * LPMX_MIXED runs FP32 values, vectors and compensated dots inside FP64 iterative
  * refinement.  *iters gets the inner CG iterations, *rr the final |r|^2. */
 int lpmx_solve(int precision, double rtol2, double atol2, int maxit, int *iters, double *rr);
-/* The same with the mixed form's refinement settings: each pass runs FP32 PCG on the
- * current FP64 residual r_k to the squared-norm ratio max(inner_rtol2,
- * 0.01 (rtol2 |r0|^2 + atol2) / |r_k|^2), at most ir_maxit passes; *status:
- * 1 converged, 2 iteration cap, 3 breakdown (p.Kp <= 0).  Returns LPMX_OK whatever
- * the status. */
+/* The same with the mixed form's refinement settings: one FP32 PCG whose residual is
+ * replaced by the FP64 residual b - K x (x accumulated in FP64) each time the FP32
+ * recurrence has dropped by the squared factor inner_rtol2 (or below a quarter of the
+ * outer threshold), the search direction kept; at most ir_maxit replacements.
+ * *status: 1 converged, 2 iteration cap, 3 breakdown (p.Kp <= 0).  Returns LPMX_OK
+ * whatever the status. */
 int lpmx_solve_ex(int precision, double rtol2, double atol2, int maxit, double inner_rtol2, int ir_maxit,
                   int *iters, double *rr, int *status);
 
@@ -164,8 +165,12 @@ typedef struct lpmx_solver
     int maxit;              /* CG cap (reference: n*dim) */
     double newton_tol;      /* reference TOLITER */
     int newton_maxit;       /* reference MAXITER */
-    double inner_rtol2;     /* mixed form: floor of the FP32 inner target (squared norms) */
-    int ir_maxit;           /* mixed form: FP64 refinement passes per solve */
+    double inner_rtol2;     /* mixed form: squared reduction of an FP32 segment between
+                             * FP64 residual replacements (1e-2) */
+    int ir_maxit;           /* mixed form: cap on residual replacements per solve */
+    int accel_depth;        /* 0: the reference's modified Newton; 1..8: Anderson
+                             * acceleration of it with that many differences (same
+                             * fixed points, far fewer iterations in plastic flow) */
 } lpmx_solver;
 
 typedef struct lpmx_step_stats
@@ -174,6 +179,8 @@ typedef struct lpmx_step_stats
     int newton_iters, cg_iters, resolves, broken, ir_passes, status;
     double res_norm, tol_mult;
     double t_tangent, t_solve, t_force, t_other, t_total; /* seconds */
+    int accel_mixed;        /* Newton iterations that took the accelerated iterate */
+    int accel_reverts;      /* accelerated iterates rejected for the plain one */
 } lpmx_step_stats;
 
 int lpmx_run_steps(const lpmx_solver *s, int nsteps, const int *nbd, const lpmx_dispbc *dbc,
@@ -181,8 +188,9 @@ int lpmx_run_steps(const lpmx_solver *s, int nsteps, const int *nbd, const lpmx_
                    lpmx_step_stats *out);
 
 /* ---- measurement ---------------------------------------------------------------------- */
-/* Time `reps` launches of the CG's block SpMV kernel (p = z + beta p_old gathered on the
- * fly, q = K p, partial p.q; FP64 or FP32) with HIP events on the library's stream;
+/* Time `reps` launches of the CG's block SpMV kernel (w = K z over the gathered columns,
+ * p = z + beta p, q = w + beta q, partial p.q; FP64 or FP32) with HIP events on the
+ * library's stream;
  * *avg_ms gets the mean duration, *bytes the algorithmic bytes per launch as returned
  * by lpmx_spmv_bytes. */
 int lpmx_bench_spmv(int precision, int reps, double *avg_ms, double *bytes);

@@ -151,3 +151,10 @@ unsigned *lpmx_counters(void); // 64 zero-initialised counters for last_group()
 // solver workspace release (called from lpmx_model_destroy)
 void lpmx_solver_free(void);
 void lpmx_damage_free(void);
+void lpmx_accel_free(void);
+
+// Anderson acceleration of the Newton loop (accel.hip)
+int lpmx_accel_begin(int depth);
+int lpmx_accel_step(int *mixed);
+int lpmx_accel_revert(void);
+void lpmx_accel_forget(void);

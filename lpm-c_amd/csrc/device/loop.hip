@@ -1,6 +1,9 @@
 // The implicit load-step loop of the reference driver (src/lpmc_project.c:382-546)
 // with every array resident on the device.  The host only takes the convergence
 // decisions: two FP64 norms per Newton iteration come back, nothing else.
+// With accel_depth > 0 the modified-Newton iterates are Anderson-accelerated
+// (accel.hip); an accelerated iterate whose residual norm is not finite or grew
+// more than 100-fold is replaced by the plain iterate and the history restarts.
 #include <chrono>
 #include <math.h>
 #include <string.h>
@@ -53,6 +56,8 @@ extern "C" int lpmx_run_steps(const lpmx_solver *s, int nsteps, const int *nbd, 
             RETCHK(lpmx_update_rr(&nres, &nrea, nullptr));
             const double tolm = nres > nrea ? nres : nrea;
             int ni = 0;
+            RETCHK(lpmx_accel_begin(s->accel_depth));
+            double nprev = nres;
             while (nres > s->newton_tol * tolm && ni < s->newton_maxit)
             {
                 RETCHK(lpmx_switch_state(0));
@@ -62,6 +67,8 @@ extern "C" int lpmx_run_steps(const lpmx_solver *s, int nsteps, const int *nbd, 
                 double rr = 0;
                 RETCHK(lpmx_solve_ex(s->precision, s->rtol2, s->atol2, s->maxit, s->inner_rtol2, s->ir_maxit, &it,
                                      &rr, &stt));
+                int mixed = 0;
+                RETCHK(lpmx_accel_step(&mixed));
                 RETCHK(sync());
                 double tf = now_s();
                 o.t_solve += tf - ts;
@@ -71,7 +78,20 @@ extern "C" int lpmx_run_steps(const lpmx_solver *s, int nsteps, const int *nbd, 
                     o.status |= stt == 3 ? 4 : 2;
                 RETCHK(lpmx_bond_force(g.plmode));
                 RETCHK(lpmx_update_rr(&nres, nullptr, nullptr));
+                if (mixed)
+                {
+                    o.accel_mixed++;
+                    if (!(nres <= 1e2 * nprev))
+                    {
+                        o.accel_reverts++;
+                        RETCHK(lpmx_accel_revert());
+                        RETCHK(lpmx_switch_state(0));
+                        RETCHK(lpmx_bond_force(g.plmode));
+                        RETCHK(lpmx_update_rr(&nres, nullptr, nullptr));
+                    }
+                }
                 o.t_force += now_s() - tf;
+                nprev = nres;
                 ni++;
             }
             if (ni >= s->newton_maxit)

@@ -147,6 +147,7 @@ static void free_all(void)
     lpmx_tangent_free();
     lpmx_solver_free();
     lpmx_damage_free();
+    lpmx_accel_free();
     memset(&g, 0, sizeof(g));
     g_have_model = g_have_pattern = g_have_nbr = false;
 }

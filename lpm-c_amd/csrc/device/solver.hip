@@ -283,8 +283,10 @@ __global__ __launch_bounds__(256) void k_cg_beta(const double *parts, int np, Cg
     }
 }
 
-// r = b (converted), x = p = q = 0, z = M^-1 r; partials of rz and rr.
-template <int D, typename TX, typename TM>
+// r = b (converted), x = 0, z = M^-1 r; partials of rz and rr.  RESTART also sets
+// p = q = 0 (a new solve); without it the search direction is kept (residual
+// replacement of the mixed form).
+template <int D, typename TX, typename TM, bool RESTART>
 __global__ __launch_bounds__(256) void k_cg_init(int n, const double *__restrict__ b, TX *__restrict__ r,
                                                  TX *__restrict__ x, TX *__restrict__ p0, TX *__restrict__ q0,
                                                  TX *__restrict__ z, const TM *__restrict__ Minv, double *parts)
@@ -302,8 +304,11 @@ __global__ __launch_bounds__(256) void k_cg_init(int n, const double *__restrict
             rv[c] = (TX)b[o];
             r[o] = rv[c];
             x[o] = 0;
-            p0[o] = 0; // with beta = 0 the first SpMV gives p = z, q = K z
-            q0[o] = 0;
+            if (RESTART)
+            {
+                p0[o] = 0; // with beta = 0 the first SpMV gives p = z, q = K z
+                q0[o] = 0;
+            }
         }
 #pragma unroll
         for (int a = 0; a < D; a++)
@@ -341,6 +346,23 @@ __global__ __launch_bounds__(256) void k_cg_start(const double *parts, int np, C
         sc->it = 0;
         sc->maxit = maxit;
         sc->done = (bb <= sc->thr) ? 1 : (maxit <= 0 ? 2 : 0);
+    }
+}
+
+// Residual replacement: the CG continues with the replaced residual's r.z and the
+// kept search direction, beta = (r.z)_new / (r.z)_old (one work-group).
+__global__ __launch_bounds__(256) void k_cg_resume(const double *parts, int np, CgS *sc, double rtol2, double atol2)
+{
+    __shared__ double sh[4];
+    const double a = group_total(parts, np, 2, 0, sh);
+    const double bb = group_total(parts, np, 2, 1, sh);
+    if (threadIdx.x == 0)
+    {
+        sc->beta = a / sc->rz;
+        sc->rz = a;
+        sc->rr = bb;
+        sc->thr = rtol2 * bb + atol2;
+        sc->done = (bb <= sc->thr) ? 1 : (sc->it >= sc->maxit ? 2 : 0);
     }
 }
 
@@ -489,17 +511,37 @@ static int ensure_work(void)
     return LPMX_OK;
 }
 
-// Run PCG on the prepared r/x/p/z buffers until the device flag is set.
-template <int D, typename TV, typename TX, typename TM>
-static int pcg_loop(const TV *Kv, const TM *Minv, TX *x, TX *r, TX *z, TX *q, TX *p, const double *b, double rtol2,
-                    double atol2, int maxit, int *iters, double *rr_out, int *status)
+// Start a CG on right-hand side b: a new solve (RESTART) or, for the mixed form, a
+// residual replacement that keeps p and q; the device scalars get the stopping
+// threshold rtol2 |r|^2 + atol2 of the new residual.
+template <int D, typename TX, typename TM, bool RESTART>
+static int cg_begin(const TM *Minv, TX *x, TX *r, TX *z, TX *q, TX *p, const double *b, double rtol2, double atol2,
+                    int maxit)
 {
     const int n = g.n;
     const int G = nblocks(n, 256);
-    hipLaunchKernelGGL((k_cg_init<D, TX, TM>), dim3(G), dim3(256), 0, g_stream, n, b, r, x, p, q, z, Minv,
+    hipLaunchKernelGGL((k_cg_init<D, TX, TM, RESTART>), dim3(G), dim3(256), 0, g_stream, n, b, r, x, p, q, z, Minv,
                        W.parts);
-    hipLaunchKernelGGL(k_cg_start, dim3(1), dim3(256), 0, g_stream, W.parts, G, W.sc, rtol2, atol2, maxit);
+    if (RESTART)
+        hipLaunchKernelGGL(k_cg_start, dim3(1), dim3(256), 0, g_stream, W.parts, G, W.sc, rtol2, atol2, maxit);
+    else
+        hipLaunchKernelGGL(k_cg_resume, dim3(1), dim3(256), 0, g_stream, W.parts, G, W.sc, rtol2, atol2);
     LAUNCHCHK();
+    return LPMX_OK;
+}
+
+// Iterate until the device flag is set; W.hsc gets the final scalars.  it0 is the
+// iteration count at entry (for the profiling filter and the runaway guard).
+template <int D, typename TV, typename TX, typename TM>
+static int cg_iterate(const TV *Kv, const TM *Minv, TX *x, TX *r, TX *z, TX *q, TX *p, int maxit)
+{
+    const int n = g.n;
+    const int G = nblocks(n, 256);
+    HIPCHK(hipMemcpyAsync(W.hsc, W.sc, sizeof(CgS), hipMemcpyDeviceToHost, g_stream));
+    HIPCHK(hipStreamSynchronize(g_stream));
+    const int it0 = W.hsc->it;
+    if (W.hsc->done)
+        return LPMX_OK;
     const int batch = n > 200000 ? 16 : 48;
     int t = 0, ev_used = 0;
     for (;;)
@@ -534,10 +576,20 @@ static int pcg_loop(const TV *Kv, const TM *Minv, TX *x, TX *r, TX *z, TX *q, TX
         {
             float ms = 0;
             HIPCHK(hipEventElapsedTime(&ms, W.ev[2 * e], W.ev[2 * e + 1]));
-            // a launch skipped because the solve had converged takes ~no time; skip it
-            if (e < W.hsc->it)
+            // a launch skipped because the solve had stopped takes ~no time; skip it
+            if (e < W.hsc->it - it0)
                 g_prof_ms += ms, g_prof_n++;
         }
+    return LPMX_OK;
+}
+
+// A complete PCG solve on the prepared buffers.
+template <int D, typename TV, typename TX, typename TM>
+static int pcg_loop(const TV *Kv, const TM *Minv, TX *x, TX *r, TX *z, TX *q, TX *p, const double *b, double rtol2,
+                    double atol2, int maxit, int *iters, double *rr_out, int *status)
+{
+    RETCHK((cg_begin<D, TX, TM, true>(Minv, x, r, z, q, p, b, rtol2, atol2, maxit)));
+    RETCHK((cg_iterate<D, TV, TX, TM>(Kv, Minv, x, r, z, q, p, maxit)));
     *iters = W.hsc->it;
     *rr_out = W.hsc->rr;
     *status = W.hsc->done;
@@ -551,8 +603,13 @@ static int solve_fp64(double rtol2, double atol2, int maxit, int *iters, double 
                                                maxit, iters, rr, status);
 }
 
+// Mixed form: one FP32 block-Jacobi CG whose residual is replaced by the FP64 one
+// every time the FP32 recurrence has dropped by the factor sqrt(seg_r2) (the
+// "reliable update" form of iterative refinement): x32 is folded into the FP64
+// solution, r = b - K x is formed with the FP64 matrix, and the CG continues from
+// it with its search direction kept.  The outer test is FP64 throughout.
 template <int D>
-static int solve_mixed(double rtol2, double atol2, double inner_rtol2, int ir_maxit, int maxit, int *iters,
+static int solve_mixed(double rtol2, double atol2, double seg_r2, int max_replace, int maxit, int *iters,
                        double *rr, int *status, int *passes)
 {
     const int n = g.n;
@@ -563,29 +620,21 @@ static int solve_mixed(double rtol2, double atol2, double inner_rtol2, int ir_ma
     const double thr = rtol2 * bb + atol2;
     HIPCHK(hipMemsetAsync(g.disp, 0, sizeof(double) * m, g_stream));
     HIPCHK(hipMemcpyAsync(W.b, g.res, sizeof(double) * m, hipMemcpyDeviceToDevice, g_stream));
-    int tot = 0, st = 0, pass = 0;
-    double cur = bb;
-    *status = 2;
+    *iters = 0, *rr = bb, *status = 1, *passes = 0;
     if (bb <= thr)
-    {
-        *iters = 0, *rr = bb, *status = 1, *passes = 0;
         return LPMX_OK;
-    }
-    for (pass = 0; pass < ir_maxit && tot < maxit; pass++)
+    // segment target: seg_r2 of the current residual, never below a quarter of the
+    // outer threshold (the FP64 residual then lands under it with margin)
+    RETCHK((cg_begin<D, float, float, true>(g.Minv32, W.x32, W.r32, W.z32, W.q32, W.p32, W.b, seg_r2, 0.25 * thr,
+                                            maxit)));
+    double cur = bb;
+    int pass = 0, st = 2;
+    for (;;)
     {
-        int it = 0;
-        double irr = 0;
-        // inner FP32 solve on the current FP64 residual W.b, asked for 10x (in norm)
-        // more than the outer test still needs, so that the FP32 recurrence's drift
-        // does not leave the FP64 residual just above the threshold and force a
-        // second, nearly full pass; inner_rtol2 is the floor of that request
-        const double want = 0.01 * thr / cur;
-        const double rt = want > inner_rtol2 ? want : inner_rtol2;
-        RETCHK((pcg_loop<D, float, float, float>(g.K32, g.Minv32, W.x32, W.r32, W.z32, W.q32, W.p32, W.b, rt, 0.0,
-                                                 maxit - tot, &it, &irr, &st)));
-        tot += it;
+        RETCHK((cg_iterate<D, float, float, float>(g.K32, g.Minv32, W.x32, W.r32, W.z32, W.q32, W.p32, maxit)));
+        st = W.hsc->done;
+        pass++;
         hipLaunchKernelGGL((k_add_f32<D>), dim3(nblocks(m)), dim3(LPMX_BLOCK), 0, g_stream, n, g.disp, W.x32);
-        LAUNCHCHK();
         hipLaunchKernelGGL((k_residual64<D>), dim3(G), dim3(256), 0, g_stream, n, g.sptr, g.col, g.K, g.disp, g.res,
                            W.b, W.parts);
         hipLaunchKernelGGL(k_parts_total, dim3(1), dim3(256), 0, g_stream, W.parts, G, W.scal);
@@ -594,21 +643,20 @@ static int solve_mixed(double rtol2, double atol2, double inner_rtol2, int ir_ma
         HIPCHK(hipStreamSynchronize(g_stream));
         if (cur <= thr)
         {
-            *status = 1;
-            pass++;
+            st = 1;
             break;
         }
-        if (st == 3)
+        if (st == 3 || W.hsc->it >= maxit || pass > max_replace)
         {
-            *status = 3;
-            pass++;
+            st = st == 3 ? 3 : 2;
             break;
         }
+        RETCHK((cg_begin<D, float, float, false>(g.Minv32, W.x32, W.r32, W.z32, W.q32, W.p32, W.b, seg_r2, 0.25 * thr,
+                                                 maxit)));
     }
-    if (*status != 1 && *status != 3)
-        *status = 2;
-    *iters = tot;
+    *iters = W.hsc->it;
     *rr = cur;
+    *status = st;
     *passes = pass;
     return LPMX_OK;
 }
@@ -655,7 +703,7 @@ extern "C" int lpmx_solve_ex(int precision, double rtol2, double atol2, int maxi
 extern "C" int lpmx_solve(int precision, double rtol2, double atol2, int maxit, int *iters, double *rr)
 {
     int st = 0;
-    RETCHK(lpmx_solve_ex(precision, rtol2, atol2, maxit, 1e-12, 12, iters, rr, &st));
+    RETCHK(lpmx_solve_ex(precision, rtol2, atol2, maxit, 1e-2, 100, iters, rr, &st));
     HIPCHK(hipStreamSynchronize(g_stream));
     if (st == 3)
         return lpmx_fail(LPMX_ERR_NOCONV, "CG breakdown (p.Kp <= 0)");

@@ -444,8 +444,8 @@ static void solve(double rtol2, double atol2, int announce)
     CHK("K import", lpmx_import_upper_csr(K_global));
     PUSH(LPMX_F_RESIDUAL, LPMX_F_XYZ);
     int prec = ext_solver_set ? ext_solver.precision : LPMX_FP64;
-    double inner = ext_solver_set ? ext_solver.inner_rtol2 : 1e-12;
-    int irmax = ext_solver_set ? ext_solver.ir_maxit : 12;
+    double inner = ext_solver_set ? ext_solver.inner_rtol2 : 1e-2;
+    int irmax = ext_solver_set ? ext_solver.ir_maxit : 100;
     int it = 0, st = 0;
     double rr = 0;
     CHK("solverCG", lpmx_solve_ex(prec, rtol2, atol2, dim * nparticle, inner, irmax, &it, &rr, &st));
