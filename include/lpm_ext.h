@@ -19,7 +19,8 @@ extern "C" {
 int lpm_ext_set_modulus(const double *factor);
 
 /* Per-bond critical stretch [nparticle][nneighbors] of the brittle law (plmode 6),
- * symmetric in (i, j); NULL restores the global critical_bstrain. */
+ * symmetric in (i, j); NULL restores the global critical_bstrain.  The array is read
+ * again by every updateBrittleDamage() call and must stay valid until it is replaced. */
 int lpm_ext_set_bond_critical(const double *crit);
 
 /* Precision and tolerances of solverCG(); NULL restores the reference settings

@@ -58,14 +58,15 @@ class ForceBC(C.Structure):
 class Solver(C.Structure):
     _fields_ = [("precision", C.c_int), ("rtol2", C.c_double), ("atol2", C.c_double), ("maxit", C.c_int),
                 ("newton_tol", C.c_double), ("newton_maxit", C.c_int), ("inner_rtol2", C.c_double),
-                ("ir_maxit", C.c_int)]
+                ("ir_maxit", C.c_int), ("accel_depth", C.c_int)]
 
 
 class StepStats(C.Structure):
     _fields_ = [("reaction", C.c_double), ("disp", C.c_double), ("newton_iters", C.c_int), ("cg_iters", C.c_int),
                 ("resolves", C.c_int), ("broken", C.c_int), ("ir_passes", C.c_int), ("status", C.c_int),
                 ("res_norm", C.c_double), ("tol_mult", C.c_double), ("t_tangent", C.c_double),
-                ("t_solve", C.c_double), ("t_force", C.c_double), ("t_other", C.c_double), ("t_total", C.c_double)]
+                ("t_solve", C.c_double), ("t_force", C.c_double), ("t_other", C.c_double), ("t_total", C.c_double),
+                ("accel_mixed", C.c_int), ("accel_reverts", C.c_int)]
 
 
 # reference API entry points declared in include/*.h (argument types for ctypes)
@@ -286,10 +287,10 @@ def set_params(plmode=0, J2_H=0.0, J2_xi=0.0, damage_L=0.5, damage_threshold=0.9
 
 
 def solver(precision=LPMX_FP64, rtol2=1e-8, atol2=1e-12, maxit=None, newton_tol=1e-4, newton_maxit=100,
-           inner_rtol2=1e-12, ir_maxit=12):
+           inner_rtol2=1e-2, ir_maxit=100, accel_depth=0):
     n = gint("nparticle").value * gint("dim").value
     return Solver(precision, rtol2, atol2, maxit if maxit is not None else n, newton_tol, newton_maxit,
-                  inner_rtol2, ir_maxit)
+                  inner_rtol2, ir_maxit, accel_depth)
 
 
 def bc_tables(steps):
@@ -398,6 +399,81 @@ def build_sc_model(L, E=200e3, nu=0.3, sigmay=200.0, H=2000.0, emod=None, damage
                critical_bstrain=1e-2, nbreak=20)
     if emod is not None:
         Lb.lpm_ext_set_modulus(ptr(emod))
+    Lb.calcKnTv()
+    Lb.computedL()
+    return n, x, t, c
+
+
+_ext_keep = {}
+
+
+def hex_types(x, radius=0.5):
+    """Types of the 2D shear configurations: 1 top edge, 2 bottom edge, 0 elsewhere."""
+    y = x[:, 1]
+    t = np.zeros(len(x), np.int32)
+    t[y >= y.max() - 0.6 * radius] = 1
+    t[y <= y.min() + 0.6 * radius] = 2
+    return t
+
+
+def hex_shear_steps(height, strains):
+    """Displacement steps: bottom (type 2) held in x and y, top (type 1) held in y and
+    moved in x by the shear-strain increments times the height."""
+    steps = []
+    prev = 0.0
+    for g in strains:
+        du = (g - prev) * height
+        prev = g
+        steps.append(([(2, "x", 0.0), (2, "y", 0.0), (1, "y", 0.0), (1, "x", du)], []))
+    return steps
+
+
+def _splitmix64(z):
+    with np.errstate(over="ignore"):
+        z = z + np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return z ^ (z >> np.uint64(31))
+
+
+def weibull_bond_critical(nbr, seed, shape=10.0, scale=1e-3):
+    """Per-bond critical stretch [n][nb] from Weibull(shape, scale), one draw per
+    undirected bond: the uniform of bond {i, j} is the splitmix64 hash of
+    (seed, min(i, j) * n + max(i, j)), so (i, j) and (j, i) share it.  Empty slots
+    get the scale."""
+    nbr = np.asarray(nbr)
+    n = nbr.shape[0]
+    i = np.repeat(np.arange(n, dtype=np.int64)[:, None], nbr.shape[1], axis=1)
+    j = nbr.astype(np.int64)
+    lo, hi = np.minimum(i, j), np.maximum(i, j)
+    key = (lo * n + hi).astype(np.uint64)
+    with np.errstate(over="ignore"):
+        h = _splitmix64(_splitmix64(np.uint64(seed)) ^ key)
+    u = (h >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+    crit = scale * (-np.log1p(-u)) ** (1.0 / shape)
+    return np.where(nbr >= 0, crit, scale)
+
+
+def build_hex_model(box, E=70e3, nu=0.25, critical=1e-3, nbreak=20, crit=None):
+    """2D hexagonal elastic-brittle model (plmode 6) of the shear configurations, plane
+    strain constants, through the reference API on the device."""
+    Lb = lib()
+    n = make_lattice(1, 0.5, box)
+    Lb.searchNormalNeighbor()
+    Lb.searchAFEMNeighbor()
+    x = positions()
+    t = hex_types(x)
+    set_types(t, 3)
+    c = elastic_constants(E, nu)
+    set_materials([c] * 3, 1e30)
+    set_params(plmode=6, critical_bstrain=critical, nbreak=nbreak)
+    if crit is not None:
+        # the library keeps the pointer (updateBrittleDamage reads it each call)
+        _ext_keep["crit"] = np.ascontiguousarray(crit, np.float64)
+        Lb.lpm_ext_set_bond_critical(ptr(_ext_keep["crit"]))
+    else:
+        _ext_keep.pop("crit", None)
+        Lb.lpm_ext_set_bond_critical(None)
     Lb.calcKnTv()
     Lb.computedL()
     return n, x, t, c

@@ -5,11 +5,13 @@ contraction on both sides and are compared bit for bit too, except where a
 device library function (exp) or a different summation order enters, which is
 stated per check."""
 import ctypes as C
+import os
+import subprocess
 
 import numpy as np
 import pytest
 
-from lpm_harness import P, get, oracle_model, push, sc_pair
+from lpm_harness import ROOT, P, get, oracle_model, push, sc_pair
 
 pytestmark = pytest.mark.gpu
 
@@ -159,7 +161,7 @@ def test_bc_elimination_and_solve(oracle):
     dense = oracle.upper_csr_dense(m)
     want = np.linalg.solve(dense, m.residual)
     for prec, tol in ((P.LPMX_FP64, 1e-10), (P.LPMX_MIXED, 1e-10)):
-        s = P.solver(precision=prec, rtol2=1e-26, atol2=0.0, maxit=50 * 3 * n, inner_rtol2=1e-12)
+        s = P.solver(precision=prec, rtol2=1e-26, atol2=0.0, maxit=50 * 3 * n)
         L.lpm_ext_set_solver(C.byref(s))
         P.write1d("residual", m.residual)
         x0 = P.positions()
@@ -260,7 +262,7 @@ def test_load_steps_match_oracle(oracle, prec):
     _, n, x, t, m = sc_pair(oracle, k)
     steps = P.sc_uniaxial_steps(k - 1.0, np.array([0.5e-3, 1e-3, 1.5e-3, 2e-3]))
     s = P.solver(precision=prec, rtol2=1e-24 if prec == P.LPMX_FP64 else 1e-16, atol2=0.0, maxit=100 * 3 * n,
-                 newton_tol=1e-11, newton_maxit=600, inner_rtol2=1e-12)
+                 newton_tol=1e-11, newton_maxit=600)
     rc, out = P.run_steps(steps, s, out_axis="z", out_type=1)
     assert rc == 0, (P.lib().lpmx_last_error(), [(o.status, o.newton_iters, o.cg_iters, o.ir_passes) for o in out])
     so = oracle.solver(rtol2=1e-24, atol2=0.0, maxit=100 * 3 * n, newton_tol=1e-11, newton_maxit=600, tangent=1,
@@ -277,3 +279,91 @@ def test_load_steps_match_oracle(oracle, prec):
     assert dl.max() > 0, "the load path reaches the plastic regime"
     np.testing.assert_allclose(dl, m.alpha[:, 1], rtol=0, atol=1e-10 * dl.max())
     np.testing.assert_allclose(get("DAMAGE_NONLOCAL1", (n,)), m.dnl[:, 1], rtol=0, atol=1e-9 * m.dnl[:, 1].max())
+
+
+@pytest.mark.parametrize("prec", [P.LPMX_FP64, P.LPMX_MIXED])
+def test_accelerated_newton_same_solution(oracle, prec):
+    """Anderson-accelerated Newton reaches the oracle's (plain modified-Newton)
+    converged solution on the elastic + plastic path, in fewer iterations."""
+    k = 6
+    _, n, x, t, m = sc_pair(oracle, k)
+    steps = P.sc_uniaxial_steps(k - 1.0, np.array([0.5e-3, 1e-3, 1.5e-3, 2e-3]))
+    s = P.solver(precision=prec, rtol2=1e-24 if prec == P.LPMX_FP64 else 1e-16, atol2=0.0, maxit=100 * 3 * n,
+                 newton_tol=1e-11, newton_maxit=600, accel_depth=5)
+    rc, out = P.run_steps(steps, s, out_axis="z", out_type=1)
+    assert rc == 0, (P.lib().lpmx_last_error(), [(o.status, o.newton_iters, o.cg_iters) for o in out])
+    so = oracle.solver(rtol2=1e-24, atol2=0.0, maxit=100 * 3 * n, newton_tol=1e-11, newton_maxit=600, tangent=1,
+                       fixed_diag=1)
+    rco, outo = oracle.run_steps(m, steps, so, out_axis="z", out_type=1)
+    assert rco == 0
+    u, uo = get("XYZ", (n, 3)) - x, m.x - x
+    assert np.abs(u - uo).max() <= 1e-9 * np.abs(uo).max()
+    np.testing.assert_allclose([o.reaction for o in out], [o.reaction for o in outo], rtol=1e-8)
+    ni = [o.newton_iters for o in out]
+    nio = [o.newton_iters for o in outo]
+    assert sum(ni[2:]) < 0.5 * sum(nio[2:]), (ni, nio)
+    assert all(o.accel_mixed > 0 for o in out[1:])
+
+
+DRIVER = os.path.join(ROOT, "tests", "drivers", "sc_uniaxial")
+
+
+def test_reference_api_driver_matches_oracle(oracle, tmp_path):
+    """A C driver that uses only the reference API (lpm.h) and its own Newton loop,
+    linked to liblpm.so, reproduces the oracle's load steps (drop-in at run time)."""
+    assert os.path.exists(DRIVER), "tests/drivers/sc_uniaxial is built by __graft_entry__.build()"
+    k = 6
+    L, height = P.sc_box_for(k), k - 1.0
+    strains = [0.5e-3, 1e-3]
+    out = tmp_path / "driver.txt"
+    print(f"reference-API driver: k={k}, strains {strains}", flush=True)
+    r = subprocess.run([DRIVER, str(out), repr(L), repr(height), "1e-11", "600"] + [repr(e) for e in strains],
+                       capture_output=True, text=True, timeout=300, cwd=str(tmp_path))
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    rows = open(out).read().split("\n")
+    steps = [row.split() for row in rows[:len(strains)]]
+    xg = np.array([[float(v) for v in row.split()] for row in rows[len(strains):] if row.strip()])
+    x0 = oracle.create_lattice(2, 0.5, [0.0, L, 0.0, L, 0.0, L])
+    n = len(x0)
+    assert xg.shape == (n, 3)
+    c = P.elastic_constants(200e3, 0.3)
+    m = oracle_model(oracle, x0, 2, P.sc_types(x0), [c] * 5, plmode=0, sigmay=200.0, J2_H=2000.0)
+    so = oracle.solver(rtol2=1e-24, atol2=0.0, maxit=100 * 3 * n, newton_tol=1e-11, newton_maxit=600, tangent=1,
+                       fixed_diag=0)
+    rco, outo = oracle.run_steps(m, P.sc_uniaxial_steps(height, np.array(strains)), so, out_axis="z", out_type=1)
+    assert rco == 0
+    np.testing.assert_allclose([float(s[1]) for s in steps], [o.reaction for o in outo], rtol=1e-8)
+    u, uo = xg - x0, m.x - x0
+    assert np.abs(u - uo).max() <= 1e-9 * np.abs(uo).max()
+    assert int(steps[-1][3]) > 1
+
+
+@pytest.mark.parametrize("prec", [P.LPMX_FP64, P.LPMX_MIXED])
+def test_brittle_shear_steps_same_broken_bonds(oracle, prec):
+    """Elastic-brittle hexagonal lattice in shear with a Weibull per-bond critical
+    stretch: bonds break, the tangent is rebuilt and the step re-solved; the broken
+    bond set, the per-step break counts, reactions and displacements match the
+    oracle (broken sets bit-exact)."""
+    box = [0.0, 14.0, 0.0, 12.0, 0.0, 1.0]
+    n, x, t, c = P.build_hex_model(box, critical=1e-3, nbreak=20)
+    nb = P.gint("nneighbors").value
+    crit = P.weibull_bond_critical(get("NEIGHBORS", (n, nb)), 1, shape=10.0, scale=1e-3)
+    P.build_hex_model(box, critical=1e-3, nbreak=20, crit=crit)
+    m = oracle_model(oracle, x, 1, t, [c] * 3, plmode=6, sigmay=1e30, critical_bstrain=1e-3, nbreak=20, crit=crit)
+    height = float(x[:, 1].max() - x[:, 1].min())
+    steps = P.hex_shear_steps(height, np.linspace(4e-4, 4e-3, 10))
+    s = P.solver(precision=prec, rtol2=1e-24 if prec == P.LPMX_FP64 else 1e-16, atol2=0.0, maxit=100 * 2 * n,
+                 newton_tol=1e-11, newton_maxit=600)
+    rc, out = P.run_steps(steps, s, out_axis="x", out_type=1)
+    assert rc == 0, (P.lib().lpmx_last_error(), [(o.status, o.newton_iters, o.broken) for o in out])
+    so = oracle.solver(rtol2=1e-24, atol2=0.0, maxit=100 * 2 * n, newton_tol=1e-11, newton_maxit=600, tangent=1,
+                       fixed_diag=1)
+    rco, outo = oracle.run_steps(m, steps, so, out_axis="x", out_type=1)
+    assert rco == 0
+    assert [o.broken for o in out] == [o.broken for o in outo]
+    assert sum(o.broken for o in outo) > 0, "the load path breaks bonds"
+    np.testing.assert_array_equal(get("DAMAGE_BROKEN", (n, nb)), m.dmg_broken)
+    r, ro = np.array([o.reaction for o in out]), np.array([o.reaction for o in outo])
+    assert np.abs(r - ro).max() <= 1e-8 * np.abs(ro).max()
+    u, uo = get("XYZ", (n, 3)) - x, m.x - x
+    assert np.abs(u - uo).max() <= 1e-9 * np.abs(uo).max()

@@ -95,3 +95,21 @@ def test_device_calls_fail_loudly_without_gpu():
     r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120)
     assert r.returncode == 1
     assert "device" in r.stderr
+
+
+REF_SRC = "/root/reference"
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_SRC), reason="the reference sources are not mounted here")
+@pytest.mark.parametrize("driver", ["src/lpmc_project.c", "examples/3_point_bending_sq_brittle.c",
+                                    "examples/shear_hex_brittle.c", "examples/CT_sc_ductile_nonlocal.c"])
+def test_reference_drivers_compile_and_link_unchanged(tmp_path, driver):
+    """The reference's own drivers build against include/ and liblpm.so with no edit
+    (the drop-in claim at link level; nothing is run)."""
+    lib_dir = os.path.join(ROOT, "lpm-c_amd")
+    out = tmp_path / "drv"
+    r = subprocess.run(["gcc", "-std=gnu11", "-O2", "-fopenmp", "-I" + os.path.join(ROOT, "include"),
+                        os.path.join(REF_SRC, driver), "-L" + lib_dir, "-llpm", "-Wl,-rpath," + lib_dir, "-lm",
+                        "-o", str(out)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert out.exists()

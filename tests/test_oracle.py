@@ -367,3 +367,28 @@ def test_brittle_breaks_largest(oracle):
     assert len(broken) == 4
     top = np.sort(strain[live])[-4:]
     assert np.allclose(np.sort(strain[tuple(broken.T)]), top)
+
+
+def test_weibull_bond_critical_is_symmetric_with_the_weibull_mean(oracle):
+    from lpm_harness import P
+
+    x = oracle.create_lattice(1, 0.5, [0.0, 40.0, 0.0, 40.0, 0.0, 1.0])
+    t = np.zeros(len(x), np.int32)
+    m = oracle.Model(x, 1, 0.5, t, [oracle.elastic_constants(70e3, 0.25)], plmode=6)
+    m.setup()
+    crit = P.weibull_bond_critical(m.nbr, 1, shape=10.0, scale=1e-3)
+    vals = []
+    for i in range(m.n):
+        for a in range(m.nb_initial[i]):
+            j = m.nbr[i, a]
+            b = list(m.nbr[j]).index(i)
+            assert crit[i, a] == crit[j, b]
+            vals.append(crit[i, a])
+    vals = np.array(vals)
+    import math
+
+    mean = 1e-3 * math.gamma(1.0 + 1.0 / 10.0)
+    sd = 1e-3 * math.sqrt(math.gamma(1.0 + 2.0 / 10.0) - math.gamma(1.0 + 1.0 / 10.0) ** 2)
+    # each undirected bond appears twice
+    assert abs(vals.mean() - mean) <= 4 * sd / math.sqrt(len(vals) / 2)
+    assert not np.array_equal(crit, P.weibull_bond_critical(m.nbr, 2))
