@@ -91,6 +91,12 @@ enum lpmx_field
     LPMX_F_PIN,                                                          /* double [n*3] */
     LPMX_F_PEX, LPMX_F_PEX_TEMP, LPMX_F_RESIDUAL, LPMX_F_DISP,           /* double [n*dim] */
     LPMX_F_DISPBC_INDEX, LPMX_F_FIX_INDEX,                               /* int [n*dim] */
+    LPMX_F_CP_GY0, LPMX_F_CP_GY1, LPMX_F_CP_GY2,                         /* double [n][nslip] */
+    LPMX_F_CP_A_SINGLE0, LPMX_F_CP_A_SINGLE1, LPMX_F_CP_A_SINGLE2,       /* double [n][nslip] */
+    LPMX_F_CP_A0, LPMX_F_CP_A1, LPMX_F_CP_A2, LPMX_F_CP_DA,              /* double [n] */
+    LPMX_F_CP_DGY, LPMX_F_CP_DA_SINGLE, LPMX_F_CP_RSS,                   /* double [n][nslip] */
+    LPMX_F_CP_JACT,                                                      /* int [n][nslip] */
+    LPMX_F_CP_CAB,                                                       /* double [n][nslip*nslip] */
     LPMX_F_COUNT
 };
 
@@ -130,9 +136,26 @@ int lpmx_solve(int precision, double rtol2, double atol2, int maxit, int *iters,
 int lpmx_solve_ex(int precision, double rtol2, double atol2, int maxit, double inner_rtol2, int ir_maxit,
                   int *iters, double *rr, int *status);
 
+/* ---- crystal plasticity (plmode 1) ---------------------------------------------------- */
+/* Parameters of the reference's rate-dependent slip law (cp_tau0[0], cp_taus[0], cp_h0,
+ * cp_q, cp_eta, cp_p, dtime, cp_maxloop in include/lpm.h). */
+typedef struct lpmx_cp_params
+{
+    double tau0, taus, h0, q, eta, cp_p, dtime;
+    int maxloop;
+} lpmx_cp_params;
+/* slipSysDefine3D's output for the device: nslip Schmid tensors [nslip][6]
+ * (s11 s22 s33 s23 s13 s12, the reference's schmid_tensor), parameters; the slip
+ * state is set to the initial yield stress tau0 and zero slip. */
+int lpmx_set_slip_systems(int nslip, const double *schmid, const lpmx_cp_params *p);
+/* computeCab (reference constitutive.c:1864-1917) at the current geometry. */
+int lpmx_compute_cab(void);
+/* Number of return maps that met a singular slip Jacobian (the reference exits). */
+int lpmx_cp_failures(int *count);
+
 /* ---- constitutive, residual, boundary conditions, damage ------------------------------ */
 int lpmx_computedL(void);
-/* computeBondForceGeneral(mode): 0 J2, 4 predictor, 6 elastic; then stress and
+/* computeBondForceGeneral(mode): 0 J2, 1 crystal plasticity, 4 predictor, 6 elastic; then stress and
  * switchStateV(2).  The _range form runs the per-particle law for [first, first+count)
  * only (no stress, no state switch). */
 int lpmx_bond_force(int mode);

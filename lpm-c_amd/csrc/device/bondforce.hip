@@ -371,6 +371,24 @@ __global__ void k_switch(DV v, int flag)
         }
         else
             v.alpha0[t] = v.alpha2[t], v.beq0[t] = v.beq2[t];
+        if (v.nslip > 0)
+        {
+            if (flag == 0)
+                v.cp_A0[t] = v.cp_A1[t];
+            else if (flag == 1)
+                v.cp_A1[t] = v.cp_A0[t];
+            else
+                v.cp_A0[t] = v.cp_A2[t];
+        }
+    }
+    if (t < (int64_t)v.nslip * v.n)
+    {
+        if (flag == 0)
+            v.cp_gy0[t] = v.cp_gy1[t], v.cp_As0[t] = v.cp_As1[t];
+        else if (flag == 1)
+            v.cp_gy1[t] = v.cp_gy0[t], v.cp_As1[t] = v.cp_As0[t];
+        else
+            v.cp_gy0[t] = v.cp_gy2[t], v.cp_As0[t] = v.cp_As2[t];
     }
 }
 
@@ -379,7 +397,10 @@ extern "C" int lpmx_switch_state(int flag)
     RETCHK(lpmx_require_model());
     if (flag < 0 || flag > 2)
         return lpmx_fail(LPMX_ERR_ARG, "switchStateV flag %d", flag);
-    const int64_t m = (int64_t)g.n * (g.nb > 6 ? g.nb : 6);
+    int w = g.nb > 6 ? g.nb : 6;
+    if (g.nslip > w)
+        w = g.nslip;
+    const int64_t m = (int64_t)g.n * w;
     hipLaunchKernelGGL(k_switch, dim3(nblocks(m)), dim3(LPMX_BLOCK), 0, g_stream, g, flag);
     LAUNCHCHK();
     return LPMX_OK;
@@ -416,6 +437,11 @@ extern "C" int lpmx_bond_force_range(int mode, int first, int count)
         LAUNCHCHK();
         hipLaunchKernelGGL(k_j2_force, G, B, 0, g_stream, g, first, count);
     }
+    else if (mode == 1)
+    {
+        RETCHK(lpmx_cp_particles());
+        hipLaunchKernelGGL(k_j2_force, G, B, 0, g_stream, g, first, count);
+    }
     else if (mode == 4)
     {
         hipLaunchKernelGGL(k_pred_pass, dim3(nblocks(g.n)), B, 0, g_stream, g, 0, g.n);
@@ -429,7 +455,7 @@ extern "C" int lpmx_bond_force_range(int mode, int first, int count)
         hipLaunchKernelGGL(k_elastic_force, G, B, 0, g_stream, g, first, count);
     }
     else
-        return lpmx_fail(LPMX_ERR_ARG, "constitutive mode %d is not on this path (0, 4 and 6 are)", mode);
+        return lpmx_fail(LPMX_ERR_ARG, "constitutive mode %d is not on this path (0, 1, 4 and 6 are)", mode);
     LAUNCHCHK();
     return LPMX_OK;
 }

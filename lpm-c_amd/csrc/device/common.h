@@ -13,6 +13,7 @@
 #define LPMX_MAXNB 32           // largest neighbour capacity handled (SC/FCC: 18)
 #define LPMX_MAXCONN 64         // largest conn row handled (SC/FCC: 61, hex: 31)
 #define LPMX_MAXBC 20           // the reference's MAXSMALL BC entries per step
+#define LPMX_MAXSLIP 24         // slip systems handled (FCC: 24 unidirectional)
 
 // All device pointers and sizes of the model, passed by value to every kernel.
 // Bond arrays are slot planes: element (particle i, slot j) at [j * n + i].
@@ -53,6 +54,16 @@ struct DV
     // static nonlocal-damage neighbourhoods (index order) and weights
     int *dptr, *didx;
     double *dwt;
+    // crystal plasticity (plmode 1): slip systems, parameters, state planes
+    int nslip, cp_maxloop;
+    double cp_tau0, cp_taus, cp_h0, cp_q, cp_eta, cp_p, dtime;
+    double *schmid;                                          // [nslip][6]
+    double *cp_gy0, *cp_gy1, *cp_gy2, *cp_As0, *cp_As1, *cp_As2; // [nslip][n]
+    double *cp_A0, *cp_A1, *cp_A2, *cp_dA;                    // [n]
+    double *cp_dgy, *cp_dAs, *cp_RSS;                         // [nslip][n]
+    int *cp_Jact;                                            // [nslip][n]
+    double *cp_Cab, *cp_D;                                   // [nslip*nslip][n]
+    int *cp_fail;                                            // singular slip Jacobians met
 };
 
 extern DV g;
@@ -152,6 +163,8 @@ unsigned *lpmx_counters(void); // 64 zero-initialised counters for last_group()
 void lpmx_solver_free(void);
 void lpmx_damage_free(void);
 void lpmx_accel_free(void);
+void lpmx_cp_free(void);
+int lpmx_cp_particles(void);
 
 // Anderson acceleration of the Newton loop (accel.hip)
 int lpmx_accel_begin(int depth);

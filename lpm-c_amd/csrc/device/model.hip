@@ -148,6 +148,7 @@ static void free_all(void)
     lpmx_solver_free();
     lpmx_damage_free();
     lpmx_accel_free();
+    lpmx_cp_free();
     memset(&g, 0, sizeof(g));
     g_have_model = g_have_pattern = g_have_nbr = false;
 }
@@ -463,6 +464,21 @@ static int field_desc(int f, FieldDesc *fd)
     case LPMX_F_DISP: p = g.disp, w = dim; break;
     case LPMX_F_DISPBC_INDEX: p = g.dispBC, w = dim, ii = 1; break;
     case LPMX_F_FIX_INDEX: p = g.fix, w = dim, ii = 1; break;
+    case LPMX_F_CP_GY0: p = g.cp_gy0, w = g.nslip, pl = 1; break;
+    case LPMX_F_CP_GY1: p = g.cp_gy1, w = g.nslip, pl = 1; break;
+    case LPMX_F_CP_GY2: p = g.cp_gy2, w = g.nslip, pl = 1; break;
+    case LPMX_F_CP_A_SINGLE0: p = g.cp_As0, w = g.nslip, pl = 1; break;
+    case LPMX_F_CP_A_SINGLE1: p = g.cp_As1, w = g.nslip, pl = 1; break;
+    case LPMX_F_CP_A_SINGLE2: p = g.cp_As2, w = g.nslip, pl = 1; break;
+    case LPMX_F_CP_A0: p = g.cp_A0; break;
+    case LPMX_F_CP_A1: p = g.cp_A1; break;
+    case LPMX_F_CP_A2: p = g.cp_A2; break;
+    case LPMX_F_CP_DA: p = g.cp_dA; break;
+    case LPMX_F_CP_DGY: p = g.cp_dgy, w = g.nslip, pl = 1; break;
+    case LPMX_F_CP_DA_SINGLE: p = g.cp_dAs, w = g.nslip, pl = 1; break;
+    case LPMX_F_CP_RSS: p = g.cp_RSS, w = g.nslip, pl = 1; break;
+    case LPMX_F_CP_JACT: p = g.cp_Jact, w = g.nslip, pl = 1, ii = 1; break;
+    case LPMX_F_CP_CAB: p = g.cp_Cab, w = g.nslip * g.nslip, pl = 1; break;
     default: return lpmx_fail(LPMX_ERR_ARG, "unknown field %d", f);
     }
     fd->ptr = p, fd->width = w, fd->planes = pl, fd->isint = ii;

@@ -121,8 +121,22 @@ static int bind(int f, struct hb *b)
     case LPMX_F_DISP: b->kind = K_D1, b->p = disp, b->width = dim; break;
     case LPMX_F_DISPBC_INDEX: b->kind = K_I1, b->p = dispBC_index, b->width = dim; break;
     case LPMX_F_FIX_INDEX: b->kind = K_I1, b->p = fix_index, b->width = dim; break;
+    case LPMX_F_CP_GY0: case LPMX_F_CP_GY1: case LPMX_F_CP_GY2:
+        b->kind = K_D3S, b->p = cp_gy, b->width = nslipSys, b->slot = f - LPMX_F_CP_GY0; break;
+    case LPMX_F_CP_A_SINGLE0: case LPMX_F_CP_A_SINGLE1: case LPMX_F_CP_A_SINGLE2:
+        b->kind = K_D3S, b->p = cp_A_single, b->width = nslipSys, b->slot = f - LPMX_F_CP_A_SINGLE0; break;
+    case LPMX_F_CP_A0: case LPMX_F_CP_A1: case LPMX_F_CP_A2:
+        b->kind = K_D2S, b->p = cp_A, b->slot = f - LPMX_F_CP_A0; break;
+    case LPMX_F_CP_DA: b->kind = K_D1, b->p = cp_dA; break;
+    case LPMX_F_CP_DGY: b->kind = K_D2, b->p = cp_dgy, b->width = nslipSys; break;
+    case LPMX_F_CP_DA_SINGLE: b->kind = K_D2, b->p = cp_dA_single, b->width = nslipSys; break;
+    case LPMX_F_CP_RSS: b->kind = K_D2, b->p = cp_RSS, b->width = nslipSys; break;
+    case LPMX_F_CP_JACT: b->kind = K_I2, b->p = cp_Jact, b->width = nslipSys; break;
+    case LPMX_F_CP_CAB: b->kind = K_D2, b->p = cp_Cab, b->width = nslipSys * nslipSys; break;
     default: return 0;
     }
+    if (f >= LPMX_F_CP_GY0 && f <= LPMX_F_CP_CAB && nslipSys <= 0)
+        return 0;
     return b->p != NULL;
 }
 
@@ -498,21 +512,24 @@ void switchStateV(int conv_flag)
     need_model("switchStateV");
     PUSH(LPMX_F_DLP0, LPMX_F_DLP1, LPMX_F_DLP2, LPMX_F_DAMAGE_D0, LPMX_F_DAMAGE_D1, LPMX_F_J2_BETA0, LPMX_F_J2_BETA1,
          LPMX_F_J2_BETA2, LPMX_F_J2_ALPHA0, LPMX_F_J2_ALPHA1, LPMX_F_J2_ALPHA2, LPMX_F_DAMAGE_NONLOCAL0,
-         LPMX_F_DAMAGE_NONLOCAL1);
+         LPMX_F_DAMAGE_NONLOCAL1, LPMX_F_CP_GY0, LPMX_F_CP_GY1, LPMX_F_CP_GY2, LPMX_F_CP_A_SINGLE0,
+         LPMX_F_CP_A_SINGLE1, LPMX_F_CP_A_SINGLE2, LPMX_F_CP_A0, LPMX_F_CP_A1, LPMX_F_CP_A2);
     CHK("switchStateV", lpmx_switch_state(conv_flag));
     if (conv_flag == 0)
     {
-        PULL(LPMX_F_DLP0, LPMX_F_DAMAGE_D0, LPMX_F_J2_BETA0, LPMX_F_J2_ALPHA0, LPMX_F_DAMAGE_NONLOCAL0);
+        PULL(LPMX_F_DLP0, LPMX_F_DAMAGE_D0, LPMX_F_J2_BETA0, LPMX_F_J2_ALPHA0, LPMX_F_DAMAGE_NONLOCAL0, LPMX_F_CP_GY0,
+             LPMX_F_CP_A_SINGLE0, LPMX_F_CP_A0);
         host_slot_copy(0, 1);
     }
     else if (conv_flag == 1)
     {
-        PULL(LPMX_F_DLP1, LPMX_F_DAMAGE_D1, LPMX_F_J2_BETA1, LPMX_F_J2_ALPHA1, LPMX_F_DAMAGE_NONLOCAL1);
+        PULL(LPMX_F_DLP1, LPMX_F_DAMAGE_D1, LPMX_F_J2_BETA1, LPMX_F_J2_ALPHA1, LPMX_F_DAMAGE_NONLOCAL1, LPMX_F_CP_GY1,
+             LPMX_F_CP_A_SINGLE1, LPMX_F_CP_A1);
         host_slot_copy(1, 0);
     }
     else
     {
-        PULL(LPMX_F_DLP0, LPMX_F_J2_BETA0, LPMX_F_J2_ALPHA0);
+        PULL(LPMX_F_DLP0, LPMX_F_J2_BETA0, LPMX_F_J2_ALPHA0, LPMX_F_CP_GY0, LPMX_F_CP_A_SINGLE0, LPMX_F_CP_A0);
         host_slot_copy(0, 2);
     }
 }
@@ -531,6 +548,38 @@ static void law_outputs(void)
          LPMX_F_J2_ALPHA2, LPMX_F_J2_DLAMBDA, LPMX_F_PL_FLAG);
 }
 
+/* crystal-plasticity state around a plmode 1 evaluation */
+static void cp_inputs(void)
+{
+    PUSH(LPMX_F_CP_GY0, LPMX_F_CP_A_SINGLE0, LPMX_F_CP_A0);
+}
+
+static void cp_outputs(void)
+{
+    PULL(LPMX_F_CP_GY2, LPMX_F_CP_A_SINGLE2, LPMX_F_CP_A2, LPMX_F_CP_DA, LPMX_F_CP_DGY, LPMX_F_CP_DA_SINGLE,
+         LPMX_F_CP_RSS, LPMX_F_CP_JACT);
+    int fails = 0;
+    CHK("crystal plasticity", lpmx_cp_failures(&fails));
+    if (fails > 0)
+    {
+        /* the reference's LAPACKE_dgesv check exits here (constitutive.c:1235-1241) */
+        printf("The diagonal element of the triangular factor of Jacobian matrix for cp_gamma \n");
+        printf("is zero in %d return maps, so that it is singular.\n", fails);
+        exit(1);
+    }
+}
+
+void lpm_dropin_slip_systems(void)
+{
+    if (!d_have_model)
+        return;
+    double *t = dtmp(6L * nslipSys);
+    for (int s = 0; s < nslipSys; s++)
+        memcpy(t + 6 * s, schmid_tensor[s], sizeof(double) * 6);
+    lpmx_cp_params p = {cp_tau0[0], cp_taus[0], cp_h0, cp_q, cp_eta, cp_p, dtime, (int)ceil(cp_maxloop)};
+    CHK("slipSysDefine3D", lpmx_set_slip_systems(nslipSys, t, &p));
+}
+
 static void stress_outputs(void)
 {
     PULL(LPMX_F_STRESS, LPMX_F_J2_STRESSEQ, LPMX_F_J2_STRESSM, LPMX_F_J2_TRIAXIALITY, LPMX_F_BOND_STRESS);
@@ -538,7 +587,7 @@ static void stress_outputs(void)
 
 static void not_on_path(const char *what)
 {
-    fprintf(stderr, "%s is not part of this build (the device path covers plmode 0, 4 and 6)\n", what);
+    fprintf(stderr, "%s is not part of this build (the device path covers plmode 0, 1, 4 and 6)\n", what);
     exit(1);
 }
 
@@ -546,14 +595,18 @@ static void not_on_path(const char *what)
 void computeBondForceGeneral(int pm, int t)
 {
     (void)t;
-    if (pm != 0 && pm != 4 && pm != 6)
+    if (pm != 0 && pm != 1 && pm != 4 && pm != 6)
         not_on_path("computeBondForceGeneral for this plmode");
     need_model("computeBondForceGeneral");
     law_inputs();
+    if (pm == 1)
+        cp_inputs();
     CHK("computeBondForceGeneral", lpmx_bond_force(pm));
     law_outputs();
+    if (pm == 1)
+        cp_outputs();
     stress_outputs();
-    PULL(LPMX_F_DLP0, LPMX_F_J2_BETA0, LPMX_F_J2_ALPHA0);
+    PULL(LPMX_F_DLP0, LPMX_F_J2_BETA0, LPMX_F_J2_ALPHA0, LPMX_F_CP_GY0, LPMX_F_CP_A_SINGLE0, LPMX_F_CP_A0);
     host_slot_copy(0, 2);
 }
 
@@ -575,8 +628,12 @@ void computeBondForceJ2nonlinearIso(int ii)
 }
 void computeBondForceCPMiehe(int ii)
 {
-    (void)ii;
-    not_on_path("computeBondForceCPMiehe (plmode 1)");
+    need_model("computeBondForceCPMiehe");
+    law_inputs();
+    cp_inputs();
+    CHK("computeBondForceCPMiehe", lpmx_bond_force_range(1, ii, 1));
+    law_outputs();
+    cp_outputs();
 }
 void computeBondForceJ2energyReturnMap(int ii, int load_indicator)
 {
@@ -584,10 +641,15 @@ void computeBondForceJ2energyReturnMap(int ii, int load_indicator)
     not_on_path("computeBondForceJ2energyReturnMap (plmode 3)");
 }
 
+/* computeCab (reference src/constitutive.c:1864-1917) on the device. */
 void computeCab()
 {
-    if (nslipSys > 0)
-        not_on_path("computeCab (crystal plasticity)");
+    if (nslipSys <= 0)
+        return;
+    need_model("computeCab");
+    PUSH(LPMX_F_CSX, LPMX_F_CSY, LPMX_F_CSZ, LPMX_F_DISTANCE, LPMX_F_KN, LPMX_F_TV, LPMX_F_NB, LPMX_F_DAMAGE_BROKEN);
+    CHK("computeCab", lpmx_compute_cab());
+    PULL(LPMX_F_CP_CAB);
 }
 
 /* computeStress (reference src/lpm_basic.c:53-125). */

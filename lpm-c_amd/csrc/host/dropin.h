@@ -11,4 +11,7 @@ void lpm_dropin_fatal(const char *where, int rc);
 void lpm_dropin_push(int field);
 void lpm_dropin_pull(int field);
 
+/* Hand slipSysDefine3D's Schmid tensors and the cp_* parameters to the device. */
+void lpm_dropin_slip_systems(void);
+
 #endif

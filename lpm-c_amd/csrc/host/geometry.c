@@ -12,6 +12,7 @@
 #include <string.h>
 
 #include "lpm.h"
+#include "dropin.h"
 
 static void rotation_matrix(void)
 {
@@ -463,22 +464,80 @@ void initMatrices()
     J2_dlambda = allocDouble1D(n, 0.);
 }
 
-/* slipSysDefine3D (reference src/initialization.c:563-827).  The crystal-plasticity
- * law is not part of this build, so no slip systems are defined (nslipSys = 0, as
- * the reference does for lattices without slip systems); the CP arrays exist with
- * zero columns so drivers link and run. */
+/* slipSysDefine3D (reference src/initialization.c:563-827).  FCC (lattice 3): the 4
+ * {111} planes with 6 <110> directions each (both senses), in the reference's order,
+ * which decides ties in the active-set search; normals and directions rotated by
+ * R_matrix (y = R x); Schmid tensors s11 s22 s33 s23 s13 s12 with engineering shear
+ * terms.  Other lattices get no slip systems (BCC's are not part of this build).
+ * Then the crystal-plasticity arrays, and the slip systems go to the device. */
 void slipSysDefine3D()
 {
-    if (lattice == 3 || lattice == 4)
-        fprintf(stderr, "slipSysDefine3D: crystal plasticity is not part of this build; no slip systems defined\n");
-    nslipSys = 0;
-    cp_gy = allocDouble3D(nparticle, 0, 3, 0.0);
-    cp_RSS = allocDouble2D(nparticle, 0, 0.);
-    cp_Jact = allocInt2D(nparticle, 0, 0);
+    static const int pl[4][3] = {{1, 1, 1}, {-1, 1, 1}, {1, -1, 1}, {-1, -1, 1}};
+    static const int dr[4][6][3] = {
+        {{1, -1, 0}, {-1, 1, 0}, {-1, 0, 1}, {1, 0, -1}, {0, 1, -1}, {0, -1, 1}},
+        {{1, 0, 1}, {-1, 0, -1}, {-1, -1, 0}, {1, 1, 0}, {0, 1, -1}, {0, -1, 1}},
+        {{-1, 0, 1}, {1, 0, -1}, {0, -1, -1}, {0, 1, 1}, {1, 1, 0}, {-1, -1, 0}},
+        {{-1, 1, 0}, {1, -1, 0}, {1, 0, 1}, {-1, 0, -1}, {0, -1, -1}, {0, 1, 1}}};
+    if (lattice == 3)
+    {
+        nslip_face = 4;
+        max_nslip_vector = 6;
+        nslipSys = 24;
+        cp_gy = allocDouble3D(nparticle, nslipSys, 3, cp_tau0[0]);
+        nslip_vector = allocInt1D(nslip_face, 6);
+        const double a = 1 / sqrt(3), b = 1 / sqrt(2);
+        double nloc[4][3], dloc[4][6][3];
+        slip_normal = allocDouble2D(nslip_face, 3, 0);
+        slip_vector = allocDouble3D(nslip_face, max_nslip_vector, 3, 0.);
+        for (int p = 0; p < 4; p++)
+        {
+            for (int c = 0; c < 3; c++)
+                nloc[p][c] = pl[p][c] * a;
+            for (int r = 0; r < 3; r++)
+                slip_normal[p][r] = R_matrix[3 * r] * nloc[p][0] + R_matrix[3 * r + 1] * nloc[p][1] +
+                                    R_matrix[3 * r + 2] * nloc[p][2];
+            for (int k = 0; k < 6; k++)
+            {
+                for (int c = 0; c < 3; c++)
+                    dloc[p][k][c] = dr[p][k][c] * b;
+                for (int r = 0; r < 3; r++)
+                    slip_vector[p][k][r] = R_matrix[3 * r] * dloc[p][k][0] + R_matrix[3 * r + 1] * dloc[p][k][1] +
+                                           R_matrix[3 * r + 2] * dloc[p][k][2];
+            }
+        }
+        schmid_tensor_local = allocDouble2D(nslipSys, 6, 0);
+        schmid_tensor = allocDouble2D(nslipSys, 6, 0);
+        int s = 0;
+        for (int p = 0; p < nslip_face; p++)
+            for (int k = 0; k < nslip_vector[p]; k++, s++)
+            {
+                const double *n0 = nloc[p], *d0 = dloc[p][k], *n1 = slip_normal[p], *d1 = slip_vector[p][k];
+                double *tl = schmid_tensor_local[s], *t = schmid_tensor[s];
+                tl[0] = n0[0] * d0[0], tl[1] = n0[1] * d0[1], tl[2] = n0[2] * d0[2];
+                tl[3] = n0[1] * d0[2] + n0[2] * d0[1];
+                tl[4] = n0[0] * d0[2] + n0[2] * d0[0];
+                tl[5] = n0[0] * d0[1] + n0[1] * d0[0];
+                t[0] = n1[0] * d1[0], t[1] = n1[1] * d1[1], t[2] = n1[2] * d1[2];
+                t[3] = n1[1] * d1[2] + n1[2] * d1[1];
+                t[4] = n1[0] * d1[2] + n1[2] * d1[0];
+                t[5] = n1[0] * d1[1] + n1[1] * d1[0];
+            }
+    }
+    else
+    {
+        if (lattice == 4)
+            fprintf(stderr, "slipSysDefine3D: BCC slip systems are not part of this build; none defined\n");
+        nslipSys = 0;
+        cp_gy = allocDouble3D(nparticle, 0, 3, 0.0);
+    }
+    cp_RSS = allocDouble2D(nparticle, nslipSys, 0.);
+    cp_Jact = allocInt2D(nparticle, nslipSys, 0);
     cp_A = allocDouble2D(nparticle, 3, 0.);
-    cp_A_single = allocDouble3D(nparticle, 0, 3, 0.);
-    cp_Cab = allocDouble2D(nparticle, 0, 0.);
-    cp_dA_single = allocDouble2D(nparticle, 0, 0.0);
+    cp_A_single = allocDouble3D(nparticle, nslipSys, 3, 0.);
+    cp_Cab = allocDouble2D(nparticle, nslipSys * nslipSys, 0.);
+    cp_dA_single = allocDouble2D(nparticle, nslipSys, 0.0);
     cp_dA = allocDouble1D(nparticle, 0.0);
-    cp_dgy = allocDouble2D(nparticle, 0, 0.0);
+    cp_dgy = allocDouble2D(nparticle, nslipSys, 0.0);
+    if (nslipSys > 0)
+        lpm_dropin_slip_systems();
 }

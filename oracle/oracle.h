@@ -93,6 +93,21 @@ typedef struct orc_model
     int64_t nK;
     double *K;
     int *IK, *JK;
+
+    /* crystal plasticity (plmode 1); NULL pointers when unused */
+    double *dist;        /* n*nbmax: distance, set by orc_computedL */
+    int nslip;           /* slip systems (24 for FCC: 12 {111}<110> x 2 senses) */
+    double *schmid;      /* nslip*6: s11 s22 s33 s23 s13 s12 (engineering shear) */
+    double cp_tau0, cp_taus, cp_h0, cp_q, cp_eta, cp_p, dtime;
+    int cp_maxloop;
+    double *cp_gy, *cp_A_single; /* n*nslip*3 slots */
+    double *cp_A;        /* n*3 slots */
+    double *cp_dA;       /* n */
+    double *cp_dgy, *cp_dA_single, *cp_RSS; /* n*nslip */
+    int *cp_Jact;        /* n*nslip */
+    double *cp_Cab;      /* n*nslip*nslip */
+    int *state_v;        /* n: return map done in the current sweep */
+    int cp_fail;         /* singular slip Jacobians met */
 } orc_model;
 
 typedef struct orc_dispbc
@@ -157,6 +172,12 @@ void orc_computedL(orc_model *m);
 void orc_bond_force_elastic(orc_model *m, int ii);
 void orc_bond_force_j2(orc_model *m, int ii);
 void orc_bond_force_incremental(orc_model *m, int ii);
+/* crystal plasticity: slip systems, slip-stiffness matrices, return map */
+int orc_slip_systems_fcc(const double *R, double *schmid);
+void orc_compute_cab(orc_model *m);
+void orc_bond_force_cp(orc_model *m, int ii);
+int orc_lu_solve(int n, double *a, double *b);
+void orc_cp_rss(orc_model *m, int i, const double *gamma, double *rss);
 void orc_compute_stress(orc_model *m);
 void orc_switch_state(orc_model *m, int flag);
 void orc_bond_force_general(orc_model *m, int mode);

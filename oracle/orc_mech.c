@@ -102,6 +102,8 @@ void orc_computedL(orc_model *m)
             int64_t b = B(m, i, j);
             int q = m->nbr[b], s = m->nsign[b];
             double dis = dist_ij(m->x, i, q);
+            if (m->dist)
+                m->dist[b] = dis;
             m->dL[b] = dis - m->L0[b];
             m->dL[b] -= m->dLp[3 * b + 0];
             A[s] += m->dL[b];
@@ -408,6 +410,16 @@ void orc_switch_state(orc_model *m, int flag)
             m->beta_eq[3 * i + d] = m->beta_eq[3 * i + s];
             m->dloc[2 * i + d] = m->dloc[2 * i + s];
             m->dnl[2 * i + d] = m->dnl[2 * i + s];
+            if (m->cp_gy)
+            {
+                for (int a = 0; a < m->nslip; a++)
+                {
+                    int64_t o = (int64_t)i * m->nslip + a;
+                    m->cp_gy[3 * o + d] = m->cp_gy[3 * o + s];
+                    m->cp_A_single[3 * o + d] = m->cp_A_single[3 * o + s];
+                }
+                m->cp_A[3 * i + d] = m->cp_A[3 * i + s];
+            }
         }
         else
         {
@@ -420,6 +432,16 @@ void orc_switch_state(orc_model *m, int flag)
                 m->beta[(6 * i + j) * 3 + 0] = m->beta[(6 * i + j) * 3 + 2];
             m->alpha[3 * i + 0] = m->alpha[3 * i + 2];
             m->beta_eq[3 * i + 0] = m->beta_eq[3 * i + 2];
+            if (m->cp_gy)
+            {
+                for (int a = 0; a < m->nslip; a++)
+                {
+                    int64_t o = (int64_t)i * m->nslip + a;
+                    m->cp_gy[3 * o + 0] = m->cp_gy[3 * o + 2];
+                    m->cp_A_single[3 * o + 0] = m->cp_A_single[3 * o + 2];
+                }
+                m->cp_A[3 * i + 0] = m->cp_A[3 * i + 2];
+            }
         }
     }
 }
@@ -428,10 +450,14 @@ void orc_switch_state(orc_model *m, int flag)
  * particle order, then stress and switchStateV(2). */
 void orc_bond_force_general(orc_model *m, int mode)
 {
+    if (mode == 1)
+        memset(m->state_v, 0, sizeof(int) * m->n);
     for (int i = 0; i < m->n; i++)
     {
         if (mode == 0)
             orc_bond_force_j2(m, i);
+        else if (mode == 1)
+            orc_bond_force_cp(m, i);
         else if (mode == 4)
             orc_bond_force_incremental(m, i);
         else if (mode == 6)

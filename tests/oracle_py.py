@@ -38,6 +38,12 @@ class OrcModel(C.Structure):
         ("Pin", _dp), ("Pex", _dp), ("Pex_temp", _dp), ("residual", _dp), ("disp", _dp),
         ("dispBC", _ip), ("fixidx", _ip), ("reaction", _dp), ("nreaction", C.c_int),
         ("nK", C.c_int64), ("K", _dp), ("IK", _ip), ("JK", _ip),
+        ("dist", _dp), ("nslip", C.c_int), ("schmid", _dp),
+        ("cp_tau0", C.c_double), ("cp_taus", C.c_double), ("cp_h0", C.c_double), ("cp_q", C.c_double),
+        ("cp_eta", C.c_double), ("cp_p", C.c_double), ("dtime", C.c_double), ("cp_maxloop", C.c_int),
+        ("cp_gy", _dp), ("cp_A_single", _dp), ("cp_A", _dp), ("cp_dA", _dp),
+        ("cp_dgy", _dp), ("cp_dA_single", _dp), ("cp_RSS", _dp), ("cp_Jact", _ip), ("cp_Cab", _dp),
+        ("state_v", _ip), ("cp_fail", C.c_int),
     ]
 
 
@@ -85,11 +91,18 @@ def lib():
         L.orc_afem_connectivity.argtypes = [P]
         L.orc_afem_connectivity.restype = C.c_int64
         for f in ("orc_kntv", "orc_computedL", "orc_compute_stress", "orc_tangent_fd",
-                  "orc_tangent_analytic", "orc_update_rr", "orc_bc_stiffness_update", "orc_update_crack"):
+                  "orc_tangent_analytic", "orc_update_rr", "orc_bc_stiffness_update", "orc_update_crack",
+                  "orc_compute_cab"):
             getattr(L, f).argtypes = [P]
             getattr(L, f).restype = None
+        L.orc_slip_systems_fcc.argtypes = [_dp, _dp]
+        L.orc_slip_systems_fcc.restype = C.c_int
+        L.orc_cp_rss.argtypes = [P, C.c_int, _dp, _dp]
+        L.orc_cp_rss.restype = None
+        L.orc_lu_solve.argtypes = [C.c_int, _dp, _dp]
+        L.orc_lu_solve.restype = C.c_int
         for f in ("orc_bond_force_elastic", "orc_bond_force_j2", "orc_bond_force_incremental",
-                  "orc_switch_state", "orc_bond_force_general"):
+                  "orc_switch_state", "orc_bond_force_general", "orc_bond_force_cp"):
             getattr(L, f).argtypes = [P, C.c_int]
             getattr(L, f).restype = None
         L.orc_internal_force.argtypes = [P, _dp, _dp]
@@ -152,7 +165,7 @@ class Model:
 
     def __init__(self, x, lattice, radius, types, Ce, box_z=1.0, plmode=0, sigmay=200.0,
                  J2_H=0.0, J2_xi=0.0, damage_L=0.5, damage_threshold=0.9, damagec_A=0.0,
-                 critical_bstrain=1e-2, nbreak=20, emod=None, crit=None):
+                 critical_bstrain=1e-2, nbreak=20, emod=None, crit=None, cp=None):
         lp = lattice_params(lattice, radius, box_z)
         n = x.shape[0]
         d = lp.dim
@@ -204,6 +217,22 @@ class Model:
         A["dispBC"] = np.ones(d * n, i32)
         A["fixidx"] = np.ones(d * n, i32)
         A["reaction"] = np.zeros(d * n, f)
+        A["dist"] = np.zeros((n, nbm), f)
+        if cp is not None:
+            # crystal plasticity: cp = dict(R=3x3, tau0, taus, h0, q, eta, p, dtime, maxloop)
+            sch = np.zeros((64, 6), f)
+            ns = lib().orc_slip_systems_fcc(ptr(np.ascontiguousarray(cp.get("R", np.eye(3)), f).reshape(9)),
+                                             ptr(sch))
+            A["schmid"] = np.ascontiguousarray(sch[:ns])
+            A["cp_gy"] = np.full((n, ns, 3), cp["tau0"], f)
+            A["cp_A_single"] = np.zeros((n, ns, 3), f)
+            A["cp_A"] = np.zeros((n, 3), f)
+            A["cp_dA"] = np.zeros(n, f)
+            for k in ("cp_dgy", "cp_dA_single", "cp_RSS"):
+                A[k] = np.zeros((n, ns), f)
+            A["cp_Jact"] = np.zeros((n, ns), i32)
+            A["cp_Cab"] = np.zeros((n, ns, ns), f)
+            A["state_v"] = np.zeros(n, i32)
         self.A = A
         s = OrcModel()
         s.dim, s.lattice, s.n, s.nbmax = d, lattice, n, nbm
@@ -212,6 +241,10 @@ class Model:
         s.plmode, s.J2_H, s.J2_xi = plmode, J2_H, J2_xi
         s.damage_L, s.damage_threshold, s.damagec_A = damage_L, damage_threshold, damagec_A
         s.critical_bstrain, s.nbreak = critical_bstrain, nbreak
+        if cp is not None:
+            s.nslip = A["schmid"].shape[0]
+            s.cp_tau0, s.cp_taus, s.cp_h0, s.cp_q = cp["tau0"], cp["taus"], cp["h0"], cp["q"]
+            s.cp_eta, s.cp_p, s.dtime, s.cp_maxloop = cp["eta"], cp["p"], cp["dtime"], cp["maxloop"]
         for k, v in A.items():
             setattr(s, k, ptr(v))
         s.nreaction = 0

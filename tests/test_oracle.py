@@ -392,3 +392,121 @@ def test_weibull_bond_critical_is_symmetric_with_the_weibull_mean(oracle):
     # each undirected bond appears twice
     assert abs(vals.mean() - mean) <= 4 * sd / math.sqrt(len(vals) / 2)
     assert not np.array_equal(crit, P.weibull_bond_critical(m.nbr, 2))
+
+
+# ----------------------------------------------------------- crystal plasticity ----
+CP = dict(tau0=10.0, taus=1e20, h0=100.0, q=1.0, eta=1.0, p=1.0, dtime=1.0, maxloop=20)
+
+
+def fcc_cp(O, L, cp=CP):
+    x = O.create_lattice(3, 0.5, [0, L, 0, L, 0, L])
+    c = O.elastic_constants(70e3, 0.3)
+    m = O.Model(x, 3, 0.5, np.zeros(x.shape[0], np.int32), [c], plmode=1, cp=cp)
+    m.setup()
+    O.lib().orc_compute_cab(m.p)
+    return m, c
+
+
+def test_fcc_slip_systems_closed_form(oracle):
+    m, _ = fcc_cp(oracle, 4.3)
+    t = m.schmid
+    assert t.shape == (24, 6)
+    # n . d = 0: the Schmid tensor is traceless; |sym(n x d)|^2 = 1/2 for unit n, d
+    assert np.abs(t[:, :3].sum(axis=1)).max() < 1e-15
+    np.testing.assert_allclose((t[:, :3] ** 2).sum(1) + (t[:, 3:] ** 2).sum(1) / 2, 0.5, rtol=0, atol=1e-15)
+    np.testing.assert_array_equal(t[1::2], -t[0::2])  # both senses of each direction
+    zz = t[:, 2]
+    big = np.isclose(np.abs(zz), 1 / np.sqrt(6), rtol=0, atol=1e-15)
+    assert big.sum() == 16 and (zz[big] > 0).sum() == 8 and np.all(zz[~big] == 0)
+    # a rotation of the crystal rotates every Schmid tensor
+    th = 0.3
+    R = np.array([[np.cos(th), -np.sin(th), 0], [np.sin(th), np.cos(th), 0], [0, 0, 1.0]])
+    sr = np.zeros((24, 6))
+    oracle.lib().orc_slip_systems_fcc(oracle.ptr(np.ascontiguousarray(R.reshape(9))), oracle.ptr(sr))
+
+    def full(v):
+        return np.array([[v[0], v[5] / 2, v[4] / 2], [v[5] / 2, v[1], v[3] / 2], [v[4] / 2, v[3] / 2, v[2]]])
+    for s in range(24):
+        np.testing.assert_allclose(full(sr[s]), R @ full(t[s]) @ R.T, rtol=0, atol=1e-15)
+
+
+def test_lu_solve_matches_dense_solve(oracle):
+    rng = np.random.default_rng(3)
+    for n in (5, 24):
+        a = rng.standard_normal((n, n)) + n * np.eye(n) * rng.choice([-1, 1], n)[:, None] * 0.1
+        a = a[rng.permutation(n)]  # pivoting is needed
+        b = rng.standard_normal(n)
+        aa, bb = np.ascontiguousarray(a.copy()), b.copy()
+        assert oracle.lib().orc_lu_solve(n, oracle.ptr(aa), oracle.ptr(bb)) == 0
+        np.testing.assert_allclose(bb, np.linalg.solve(a, b), rtol=1e-10, atol=1e-12)
+        # the factors reproduce a row permutation of a
+        L = np.tril(aa, -1) + np.eye(n)
+        U = np.triu(aa)
+        LU = L @ U
+        rows = [int(np.argmin(np.abs(a - LU[k]).sum(1))) for k in range(n)]
+        assert sorted(rows) == list(range(n))
+        np.testing.assert_allclose(LU, a[rows], rtol=0, atol=1e-12 * np.abs(a).max())
+    s = np.zeros((3, 3))
+    s[0] = [1.0, 2.0, 3.0]
+    assert oracle.lib().orc_lu_solve(3, oracle.ptr(s), oracle.ptr(np.ones(3))) > 0
+
+
+def test_cab_is_minus_rss_derivative(oracle):
+    """C_ab = -dRSS_a/dgamma_b of the return map's own stress path (linear)."""
+    m, _ = fcc_cp(oracle, 6.5)
+    i = deep_interior(m)[0]
+    cab = m.cp_Cab[i]
+    g0 = np.zeros(24)
+    r0 = np.zeros(24)
+    oracle.lib().orc_cp_rss(m.p, int(i), oracle.ptr(g0), oracle.ptr(r0))
+    h = 1e-4
+    for b in range(24):
+        g = np.zeros(24)
+        g[b] = h
+        r = np.zeros(24)
+        oracle.lib().orc_cp_rss(m.p, int(i), oracle.ptr(g), oracle.ptr(r))
+        np.testing.assert_allclose(-(r - r0) / h, cab[:, b], rtol=0, atol=1e-8 * np.abs(cab).max())
+    assert np.abs(cab - cab.T).max() <= 1e-9 * np.abs(cab).max()
+
+
+def test_single_slip_return_map_closed_form(oracle):
+    """Uniaxial strain along a single-slip direction just past first yield: exactly
+    the most loaded system activates, and its slip solves the scalar residual
+    RSS_tr - C'aa g - (tau0 + h0 g)(1 + g eta/dt) = 0 in closed form."""
+    m, c = fcc_cp(oracle, 6.5)
+    t = np.array([-1.0, 2.0, 3.0]) / np.sqrt(14.0)
+    mu = c[2]
+    ev = np.outer(t, t)
+    fac = np.array([2 * mu * (s[0] * t[0] ** 2 + s[1] * t[1] ** 2 + s[2] * t[2] ** 2 + s[3] * t[1] * t[2] +
+                              s[4] * t[0] * t[2] + s[5] * t[0] * t[1]) for s in m.schmid])
+    a = int(np.argmax(fac))
+    second = np.sort(fac)[-2]
+    assert second < 0.95 * fac[a]
+    e = 1.02 * CP["tau0"] / fac[a]
+    m.x[:] = m.x0 + m.x0 @ (e * ev).T
+    i = deep_interior(m)[0]
+    # trial RSS and the slip stiffness at the deformed geometry, from the stress path
+    oracle.lib().orc_computedL(m.p)
+    r0, r1 = np.zeros(24), np.zeros(24)
+    g = np.zeros(24)
+    oracle.lib().orc_cp_rss(m.p, int(i), oracle.ptr(g), oracle.ptr(r0))
+    g[a] = 1e-4
+    oracle.lib().orc_cp_rss(m.p, int(i), oracle.ptr(g), oracle.ptr(r1))
+    caa = (r0[a] - r1[a]) / 1e-4
+    assert abs(r0[a] - 1.02 * CP["tau0"]) < 1e-3 * CP["tau0"]  # the bond model's stress is C:eps
+    oracle.lib().orc_bond_force_general(m.p, 1)
+    assert m.s.cp_fail == 0
+    assert list(np.nonzero(m.cp_Jact[i])[0]) == [a]
+    k = CP["eta"] / CP["dtime"]
+    qa, qb, qc = k * CP["h0"], caa + CP["h0"] + CP["tau0"] * k, -(r0[a] - CP["tau0"])
+    gam = (-qb + np.sqrt(qb * qb - 4 * qa * qc)) / (2 * qa)
+    assert gam > 0
+    np.testing.assert_allclose(m.cp_dA_single[i, a], gam, rtol=1e-6)
+    # the accumulated slip and the hardening are formed from the slips at the start
+    # of the last inner iteration, before its Newton update (constitutive.c:1111-1131)
+    np.testing.assert_allclose(m.cp_A[i, 2], gam, rtol=1e-3)
+    np.testing.assert_allclose(m.cp_dgy[i, a], CP["h0"] * m.cp_A[i, 2], rtol=1e-12)
+    gy2 = m.cp_gy[i, :, 2]
+    assert abs(gy2[a] - (CP["tau0"] + m.cp_dgy[i, a])) <= 1e-12 * CP["tau0"]
+    np.testing.assert_array_equal(np.delete(gy2, a), CP["tau0"])
+    assert m.pl_flag[i] == 1
