@@ -24,10 +24,11 @@ DISTANCE_INITIAL CSX_INITIAL CSY_INITIAL CSZ_INITIAL KN TV DL DL_AVE DDL DDLP F 
 DISTANCE DAMAGE_BROKEN DAMAGE_W DLP0 DLP1 DLP2 DAMAGE_D0 DAMAGE_D1 DL_TOTAL TDL_TOTAL DDL_TOTAL TDDL_TOTAL
 J2_BETA0 J2_BETA1 J2_BETA2 STRESS J2_ALPHA0 J2_ALPHA1 J2_ALPHA2 J2_DLAMBDA J2_STRESSEQ J2_STRESSM
 J2_TRIAXIALITY DAMAGE_NONLOCAL0 DAMAGE_NONLOCAL1 DAMAGE_VISUAL SIGMAY PIN PEX PEX_TEMP RESIDUAL DISP
-DISPBC_INDEX FIX_INDEX""".split()
+DISPBC_INDEX FIX_INDEX CP_GY0 CP_GY1 CP_GY2 CP_A_SINGLE0 CP_A_SINGLE1 CP_A_SINGLE2 CP_A0 CP_A1 CP_A2 CP_DA
+CP_DGY CP_DA_SINGLE CP_RSS CP_JACT CP_CAB""".split()
 F = {name: k for k, name in enumerate(FIELDS)}
 INT_FIELDS = {"NEIGHBORS", "NSIGN", "NB", "NB_INITIAL", "NB_CONN", "TYPE", "PL_FLAG", "CONN", "K_POINTER",
-              "DISPBC_INDEX", "FIX_INDEX"}
+              "DISPBC_INDEX", "FIX_INDEX", "CP_JACT"}
 
 
 class LpmxError(RuntimeError):
@@ -119,6 +120,8 @@ def lib():
                            C.c_char, C.c_int, C.POINTER(StepStats)],
         "lpmx_bench_spmv": [C.c_int, C.c_int, _dp, _dp],
         "lpmx_spmv_bytes": [C.c_int, _dp],
+        "lpmx_compute_cab": [],
+        "lpmx_cp_failures": [_ip],
         "lpmx_matrix_info": [C.POINTER(C.c_int64), C.POINTER(C.c_int64), _ip],
         "lpmx_spmv": [C.c_int, _dp, _dp],
     }
@@ -476,4 +479,37 @@ def build_hex_model(box, E=70e3, nu=0.25, critical=1e-3, nbreak=20, crit=None):
         Lb.lpm_ext_set_bond_critical(None)
     Lb.calcKnTv()
     Lb.computedL()
+    return n, x, t, c
+
+
+def set_cp_params(tau0, taus, h0, q, eta, p, dtime, maxloop):
+    """The reference's crystal-plasticity globals (cp_tau0[0], cp_taus[0], cp_h0, cp_q,
+    cp_eta, cp_p, dtime, cp_maxloop)."""
+    L = lib()
+    (C.c_double * 3).in_dll(L, "cp_tau0")[0] = tau0
+    (C.c_double * 3).in_dll(L, "cp_taus")[0] = taus
+    for name, v in (("cp_h0", h0), ("cp_q", q), ("cp_eta", eta), ("cp_p", p), ("dtime", dtime),
+                    ("cp_maxloop", float(maxloop))):
+        gdouble(name).value = v
+
+
+def build_fcc_cp_model(L, cp, E=70e3, nu=0.3):
+    """FCC crystal-plasticity model (plmode 1) through the reference API on the
+    device: lattice, lists, uniaxial types, materials, bond constants, slip systems
+    (slipSysDefine3D) and the slip stiffness (computeCab)."""
+    Lb = lib()
+    n = make_lattice(3, 0.5, [0.0, L, 0.0, L, 0.0, L])
+    Lb.searchNormalNeighbor()
+    Lb.searchAFEMNeighbor()
+    x = positions()
+    t = sc_types(x)
+    set_types(t, 5)
+    c = elastic_constants(E, nu)
+    set_materials([c] * 5, 1e30)
+    set_params(plmode=1)
+    set_cp_params(**cp)
+    Lb.calcKnTv()
+    Lb.computedL()
+    Lb.slipSysDefine3D()
+    Lb.computeCab()
     return n, x, t, c

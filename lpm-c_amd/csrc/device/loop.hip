@@ -117,6 +117,13 @@ extern "C" int lpmx_run_steps(const lpmx_solver *s, int nsteps, const int *nbd, 
             }
             break;
         }
+        if (g.plmode == 1)
+        {
+            int fails = 0;
+            RETCHK(lpmx_cp_failures(&fails));
+            if (fails > 0) // the reference exits here (constitutive.c:1235-1241)
+                return lpmx_fail(LPMX_ERR_NOCONV, "singular slip Jacobian in %d return maps of step %d", fails, st);
+        }
         RETCHK(sync());
         o.t_total = now_s() - t0;
         o.t_other = o.t_total - o.t_tangent - o.t_solve - o.t_force;

@@ -141,6 +141,9 @@ typedef struct orc_solver
     int tangent;         /* 0: forward FD (reference), 1: analytic elastic Jacobian */
     int fixed_diag;      /* 1: constrained-DOF diagonal fixed per tangent assembly */
     int nthreads;        /* >1 only for timing (CPU baseline); 1 is the specification */
+    int accel;           /* >0: Anderson acceleration with that many differences, to reach
+                          * the same equilibrium where modified Newton crawls (off in the
+                          * specification runs) */
 } orc_solver;
 
 /* Per-step outputs of orc_run_steps. */

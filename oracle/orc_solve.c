@@ -472,12 +472,97 @@ static int damage_general(orc_model *m)
 /* The load-step loop of the reference driver (src/lpmc_project.c:382-546) with the
  * solver tolerances as parameters.  computeStrain (output only) is not run.
  * Returns 0, or the index+1 of the first step whose Newton loop hit the cap. */
+/* Anderson mixing of the modified-Newton iterates u <- u + d(u) (Walker & Ni 2011,
+ * type II): the last `depth` differences of iterates and corrections, the
+ * least-squares weights from the normal equations (regularised by 1e-12 of their
+ * largest diagonal), u <- u + d - sum_j g_j (dU_j + dF_j). */
+typedef struct
+{
+    int depth, cnt, next, have, nd;
+    double *dU, *dF, *up, *fp, G[8][8];
+} orc_aa;
+
+static void aa_begin(orc_aa *a, int depth, int nd)
+{
+    if (a->nd != nd || a->depth != depth)
+    {
+        free(a->dU), free(a->dF), free(a->up), free(a->fp);
+        a->dU = malloc(sizeof(double) * (size_t)nd * (depth > 0 ? depth : 1));
+        a->dF = malloc(sizeof(double) * (size_t)nd * (depth > 0 ? depth : 1));
+        a->up = malloc(sizeof(double) * nd), a->fp = malloc(sizeof(double) * nd);
+    }
+    a->depth = depth, a->nd = nd, a->cnt = 0, a->next = 0, a->have = 0;
+}
+
+/* x holds u + d after the solve, m->disp holds d; returns 1 when it mixed */
+static int aa_step(orc_aa *a, orc_model *m)
+{
+    const int d = m->dim, nd = a->nd;
+    if (a->depth <= 0)
+        return 0;
+    const int s = a->next;
+    double *dU = a->dU + (size_t)s * nd, *dF = a->dF + (size_t)s * nd;
+    for (int t = 0; t < nd; t++)
+    {
+        const int i = t / d, c = t % d;
+        const double f = m->disp[t], u = m->x[3 * i + c] - f;
+        if (a->have)
+            dU[t] = u - a->up[t], dF[t] = f - a->fp[t];
+        a->up[t] = u, a->fp[t] = f;
+    }
+    if (!a->have)
+    {
+        a->have = 1;
+        return 0;
+    }
+    if (a->cnt < a->depth)
+        a->cnt++;
+    a->next = (a->next + 1) % a->depth;
+    const int k = a->cnt;
+    double h[8], gmat[64], dmax = 0.0;
+    for (int j = 0; j < k; j++)
+    {
+        const double *Fj = a->dF + (size_t)j * nd;
+        double g = 0.0, hh = 0.0;
+        for (int t = 0; t < nd; t++)
+            g += dF[t] * Fj[t], hh += Fj[t] * m->disp[t];
+        a->G[s][j] = a->G[j][s] = g;
+        h[j] = hh;
+    }
+    for (int j = 0; j < k; j++)
+        dmax = a->G[j][j] > dmax ? a->G[j][j] : dmax;
+    if (!(dmax > 0.0))
+    {
+        a->cnt = a->next = 0;
+        return 0;
+    }
+    for (int i = 0; i < k; i++)
+        for (int j = 0; j < k; j++)
+            gmat[i * k + j] = a->G[i][j] + (i == j ? 1e-12 * dmax : 0.0);
+    if (orc_lu_solve(k, gmat, h))
+    {
+        a->cnt = a->next = 0;
+        return 0;
+    }
+    for (int t = 0; t < nd; t++)
+    {
+        double sum = 0.0;
+        for (int j = 0; j < k; j++)
+            sum += h[j] * (a->dU[(size_t)j * nd + t] + a->dF[(size_t)j * nd + t]);
+        const int i = t / d, c = t % d;
+        m->x[3 * i + c] -= sum;
+    }
+    return 1;
+}
+
 int orc_run_steps(orc_model *m, const orc_solver *s, int nsteps,
                   const int *nbd, const orc_dispbc *dbc, const int *nbf, const orc_forcebc *fbc,
                   int maxbc, char out_axis, int out_type, orc_stepout *out)
 {
     const int d = m->dim, n = m->n;
     int first_fail = 0;
+    orc_aa aa;
+    memset(&aa, 0, sizeof(aa));
     for (int st = 0; st < nsteps; st++)
     {
         orc_stepout o;
@@ -500,6 +585,8 @@ int orc_run_steps(orc_model *m, const orc_solver *s, int nsteps,
             double tolm = nres > nrea ? nres : nrea;
             int ni = 0;
             double nd = -1.0;
+            aa_begin(&aa, s->accel > 8 ? 8 : s->accel, d * n);
+            double nprev = nres;
             while (nres > s->newton_tol * tolm && ni < s->newton_maxit)
             {
                 orc_switch_state(m, 0);
@@ -512,10 +599,23 @@ int orc_run_steps(orc_model *m, const orc_solver *s, int nsteps,
                 }
                 int it = orc_solve_cg(m, s, NULL);
                 o.cg_iters += it < 0 ? -it : it;
+                const int mixed = aa_step(&aa, m);
                 orc_bond_force_general(m, m->plmode);
                 orc_update_rr(m);
                 ni++;
                 nres = orc_nrm2(d * n, m->residual);
+                if (mixed && !(nres <= 1e2 * nprev))
+                {
+                    /* back to the plain iterate, history restarted */
+                    for (int t = 0; t < d * n; t++)
+                        m->x[3 * (t / d) + t % d] = aa.up[t] + aa.fp[t];
+                    aa.cnt = aa.next = 0;
+                    orc_switch_state(m, 0);
+                    orc_bond_force_general(m, m->plmode);
+                    orc_update_rr(m);
+                    nres = orc_nrm2(d * n, m->residual);
+                }
+                nprev = nres;
             }
             if (ni >= s->newton_maxit && !first_fail)
                 first_fail = st + 1;
@@ -542,5 +642,6 @@ int orc_run_steps(orc_model *m, const orc_solver *s, int nsteps,
         if (out)
             out[st] = o;
     }
+    free(aa.dU), free(aa.dF), free(aa.up), free(aa.fp);
     return first_fail;
 }

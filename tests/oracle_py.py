@@ -64,7 +64,7 @@ class OrcForceBC(C.Structure):
 class OrcSolver(C.Structure):
     _fields_ = [("rtol2", C.c_double), ("atol2", C.c_double), ("maxit", C.c_int),
                 ("newton_tol", C.c_double), ("newton_maxit", C.c_int), ("tangent", C.c_int),
-                ("fixed_diag", C.c_int), ("nthreads", C.c_int)]
+                ("fixed_diag", C.c_int), ("nthreads", C.c_int), ("accel", C.c_int)]
 
 
 class OrcStepOut(C.Structure):
@@ -312,12 +312,13 @@ def run_steps(model, steps, solver, out_axis="z", out_type=1):
 
 
 def solver(rtol2=1e-8, atol2=1e-12, maxit=None, newton_tol=1e-4, newton_maxit=100, tangent=0, nthreads=1, n=0,
-           fixed_diag=0):
+           fixed_diag=0, accel=0):
     s = OrcSolver()
     s.rtol2, s.atol2 = rtol2, atol2
     s.maxit = maxit if maxit is not None else max(1, n)
     s.newton_tol, s.newton_maxit, s.tangent, s.nthreads = newton_tol, newton_maxit, tangent, nthreads
     s.fixed_diag = fixed_diag
+    s.accel = accel
     return s
 
 

@@ -367,3 +367,90 @@ def test_brittle_shear_steps_same_broken_bonds(oracle, prec):
     assert np.abs(r - ro).max() <= 1e-8 * np.abs(ro).max()
     u, uo = get("XYZ", (n, 3)) - x, m.x - x
     assert np.abs(u - uo).max() <= 1e-9 * np.abs(uo).max()
+
+
+CP = dict(tau0=10.0, taus=1e20, h0=100.0, q=1.0, eta=1.0, p=1.0, dtime=1.0, maxloop=20)
+
+
+def cp_pair(oracle, L):
+    n, x, t, c = P.build_fcc_cp_model(L, CP)
+    m = oracle_model(oracle, x, 3, t, [c] * 5, plmode=1, sigmay=1e30, cp=CP)
+    oracle.lib().orc_compute_cab(m.p)
+    return n, x, t, c, m
+
+
+@pytest.mark.parametrize("case", ["single", "multi", "random"])
+def test_crystal_plasticity_return_map_bit_exact(oracle, case):
+    """computeCab and one plmode-1 force evaluation: slip stiffness, active slip sets,
+    slips, hardening, plastic stretch, forces and stress bit-exact with the oracle,
+    for single slip, [001] multi-slip (eight tied systems) and a random strain."""
+    L = 6.5
+    n, x, t, c, m = cp_pair(oracle, L)
+    np.testing.assert_array_equal(get("CP_CAB", (n, 24 * 24)), m.cp_Cab.reshape(n, -1))
+    mu = c[2]
+    if case == "single":
+        d = np.array([-1.0, 2.0, 3.0]) / np.sqrt(14.0)
+        eps = np.outer(d, d)
+        fac = max(2 * mu * (s[0] * d[0] ** 2 + s[1] * d[1] ** 2 + s[2] * d[2] ** 2 + s[3] * d[1] * d[2] +
+                            s[4] * d[0] * d[2] + s[5] * d[0] * d[1]) for s in m.schmid)
+        eps *= 1.02 * CP["tau0"] / fac
+    elif case == "multi":
+        eps = np.diag([-0.3, -0.3, 1.0]) * 1.5 * CP["tau0"] / (mu * 0.8)
+    else:
+        rng = np.random.default_rng(5)
+        a = rng.standard_normal((3, 3))
+        eps = 2e-3 * (a + a.T) / 2
+    xd = x + x @ eps.T
+    if case == "random":
+        xd = xd + 1e-4 * np.random.default_rng(6).standard_normal(x.shape)
+    push("XYZ", xd)
+    P.check(P.lib().lpmx_bond_force(1), "plmode 1")
+    fails = C.c_int()
+    P.check(P.lib().lpmx_cp_failures(C.byref(fails)), "failures")
+    m.x[:] = xd
+    oracle.lib().orc_bond_force_general(m.p, 1)
+    assert fails.value == 0 and m.s.cp_fail == 0
+    jact = get("CP_JACT", (n, 24))
+    np.testing.assert_array_equal(jact, m.cp_Jact)
+    assert jact.any(), "the case reaches plastic slip"
+    if case == "multi":
+        assert jact.sum(axis=1).max() > 1
+    for name, ref in (("CP_DA_SINGLE", m.cp_dA_single), ("CP_DGY", m.cp_dgy), ("CP_RSS", m.cp_RSS),
+                      ("CP_GY0", m.cp_gy[:, :, 0]), ("CP_A_SINGLE0", m.cp_A_single[:, :, 0])):
+        np.testing.assert_array_equal(get(name, ref.shape), ref, err_msg=name)
+    np.testing.assert_array_equal(get("CP_A0", (n,)), m.cp_A[:, 0])
+    np.testing.assert_array_equal(get("CP_DA", (n,)), m.cp_dA)
+    np.testing.assert_array_equal(get("DLP0", (n, 18)), m.dLp[:, :, 0])
+    np.testing.assert_array_equal(get("F", (n, 18)), m.F)
+    np.testing.assert_array_equal(get("PIN", (3 * n,)), m.Pin)
+    np.testing.assert_array_equal(get("STRESS", (n, 6)), m.stress)
+    np.testing.assert_array_equal(get("PL_FLAG", (n,)), m.pl_flag)
+
+
+@pytest.mark.parametrize("prec", [P.LPMX_FP64, P.LPMX_MIXED])
+def test_crystal_plasticity_load_steps_match_oracle(oracle, prec):
+    """FCC crystal under [001] tension through first slip: the fused device loop and
+    the oracle reach the same state (active slip sets bit-exact at the end).  Both
+    accelerate the modified-Newton iteration (its contraction is ~0.996 once slip
+    starts); the equilibrium reached does not depend on that."""
+    L = 4.3
+    n, x, t, c, m = cp_pair(oracle, L)
+    height = float(x[:, 2].max() - x[:, 2].min())
+    e_y = CP["tau0"] / (c[2] * 0.8)
+    steps = P.sc_uniaxial_steps(height, np.array([0.5, 1.0, 1.3]) * e_y)
+    s = P.solver(precision=prec, rtol2=1e-24 if prec == P.LPMX_FP64 else 1e-16, atol2=0.0, maxit=100 * 3 * n,
+                 newton_tol=1e-11, newton_maxit=800, accel_depth=5)
+    rc, out = P.run_steps(steps, s, out_axis="z", out_type=1)
+    assert rc == 0, (P.lib().lpmx_last_error(), [(o.status, o.newton_iters) for o in out])
+    so = oracle.solver(rtol2=1e-24, atol2=0.0, maxit=100 * 3 * n, newton_tol=1e-11, newton_maxit=800, tangent=1,
+                       fixed_diag=1, accel=5)
+    rco, outo = oracle.run_steps(m, steps, so, out_axis="z", out_type=1)
+    assert rco == 0 and m.s.cp_fail == 0
+    jact = get("CP_JACT", (n, 24))
+    assert jact.any(), "the path reaches slip"
+    np.testing.assert_array_equal(jact, m.cp_Jact)
+    r, ro = np.array([o.reaction for o in out]), np.array([o.reaction for o in outo])
+    assert np.abs(r - ro).max() <= 1e-8 * np.abs(ro).max()
+    u, uo = get("XYZ", (n, 3)) - x, m.x - x
+    # FP64 to 1e-9; the mixed form's stated displacement tolerance is 1e-8
+    assert np.abs(u - uo).max() <= (1e-9 if prec == P.LPMX_FP64 else 1e-8) * np.abs(uo).max()

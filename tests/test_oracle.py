@@ -510,3 +510,28 @@ def test_single_slip_return_map_closed_form(oracle):
     assert abs(gy2[a] - (CP["tau0"] + m.cp_dgy[i, a])) <= 1e-12 * CP["tau0"]
     np.testing.assert_array_equal(np.delete(gy2, a), CP["tau0"])
     assert m.pl_flag[i] == 1
+
+
+def test_accelerated_loop_reaches_the_plain_equilibrium(oracle):
+    """The oracle's optional Anderson mixing changes the path, not the equilibrium:
+    J2 tension through yield, plain modified Newton against the accelerated one."""
+    from lpm_harness import P
+
+    out = {}
+    for acc in (0, 5):
+        L = P.sc_box_for(6)
+        x = oracle.create_lattice(2, 0.5, [0, L, 0, L, 0, L])
+        c = oracle.elastic_constants(E0, NU0)
+        m = oracle.Model(x, 2, 0.5, P.sc_types(x), [c] * 5, plmode=0, sigmay=200.0, J2_H=2000.0)
+        m.setup()
+        n = m.n
+        steps = P.sc_uniaxial_steps(5.0, np.array([0.5e-3, 1e-3, 1.5e-3]))
+        s = oracle.solver(rtol2=1e-24, atol2=0.0, maxit=100 * 3 * n, newton_tol=1e-11, newton_maxit=600,
+                          tangent=1, fixed_diag=1, accel=acc)
+        rc, o = oracle.run_steps(m, steps, s, out_axis="z", out_type=1)
+        assert rc == 0
+        out[acc] = (m.x - x, [k.newton_iters for k in o], [k.reaction for k in o])
+    u0, u5 = out[0][0], out[5][0]
+    assert np.abs(u0 - u5).max() <= 1e-9 * np.abs(u0).max()
+    np.testing.assert_allclose(out[5][2], out[0][2], rtol=1e-9)
+    assert sum(out[5][1]) < 0.5 * sum(out[0][1])
