@@ -153,6 +153,16 @@ int lpmx_compute_cab(void);
 /* Number of return maps that met a singular slip Jacobian (the reference exits). */
 int lpmx_cp_failures(int *count);
 
+/* ---- adjoint of the J2 load path (no counterpart in the reference) --------------------- */
+/* One J2 force evaluation at the current positions and slot-0 state (no stress, no
+ * state switch), then its vector-Jacobian product: seeds on Pin [3n] and on the slot-2
+ * state dLp [n][nneighbors], J2_beta [n][6], J2_alpha [n]; out the adjoints of the
+ * positions [3n], slot-0 dLp, J2_beta, J2_alpha, the per-particle modulus factor [n],
+ * the yield stress [n] and the hardening modulus H. */
+int lpmx_vjp(const double *Pbar, const double *dLp2bar, const double *beta2bar, const double *alpha2bar,
+             double *xbar, double *dLp0bar, double *beta0bar, double *alpha0bar, double *emodbar,
+             double *sigmaybar, double *Hbar);
+
 /* ---- constitutive, residual, boundary conditions, damage ------------------------------ */
 int lpmx_computedL(void);
 /* computeBondForceGeneral(mode): 0 J2, 1 crystal plasticity, 4 predictor, 6 elastic; then stress and

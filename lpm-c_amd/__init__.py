@@ -121,6 +121,7 @@ def lib():
         "lpmx_bench_spmv": [C.c_int, C.c_int, _dp, _dp],
         "lpmx_spmv_bytes": [C.c_int, _dp],
         "lpmx_compute_cab": [],
+        "lpmx_vjp": [_dp] * 11,
         "lpmx_cp_failures": [_ip],
         "lpmx_matrix_info": [C.POINTER(C.c_int64), C.POINTER(C.c_int64), _ip],
         "lpmx_spmv": [C.c_int, _dp, _dp],

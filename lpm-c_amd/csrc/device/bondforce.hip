@@ -460,6 +460,16 @@ extern "C" int lpmx_bond_force_range(int mode, int first, int count)
     return LPMX_OK;
 }
 
+int lpmx_eval_j2(void)
+{
+    const dim3 G(nblocks(g.n)), B(LPMX_BLOCK);
+    hipLaunchKernelGGL(k_j2_particle, G, B, 0, g_stream, g, 0, g.n);
+    LAUNCHCHK();
+    hipLaunchKernelGGL(k_j2_force, G, B, 0, g_stream, g, 0, g.n);
+    LAUNCHCHK();
+    return LPMX_OK;
+}
+
 extern "C" int lpmx_bond_force(int mode)
 {
     RETCHK(lpmx_bond_force_range(mode, 0, g.n));

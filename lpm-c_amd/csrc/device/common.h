@@ -64,6 +64,10 @@ struct DV
     int *cp_Jact;                                            // [nslip][n]
     double *cp_Cab, *cp_D;                                   // [nslip*nslip][n]
     int *cp_fail;                                            // singular slip Jacobians met
+    // per-type constants kept for the adjoint: bond constants per unit modulus
+    // factor [ntype][3], elastic constants [ntype][3], the factor per particle [n]
+    int ntype;
+    double *ktype_t, *ce_t, *emod;
 };
 
 extern DV g;
@@ -164,6 +168,8 @@ void lpmx_solver_free(void);
 void lpmx_damage_free(void);
 void lpmx_accel_free(void);
 void lpmx_cp_free(void);
+void lpmx_adjoint_free(void);
+int lpmx_eval_j2(void); // one J2 evaluation of every particle, no stress, no slot switch
 int lpmx_cp_particles(void);
 
 // Anderson acceleration of the Newton loop (accel.hip)

@@ -140,7 +140,7 @@ static void free_all(void)
     for (void *p : g_allocs)
         hipFree(p);
     g_allocs.clear();
-    void *pat[] = {g.sptr, g.col, g.tpos, g.K, g.K32, g.J, g.Minv, g.Minv32, g.crit};
+    void *pat[] = {g.sptr, g.col, g.tpos, g.K, g.K32, g.J, g.Minv, g.Minv32, g.crit, g.ktype_t, g.ce_t, g.emod};
     for (void *p : pat)
         if (p)
             hipFree(p);
@@ -149,6 +149,7 @@ static void free_all(void)
     lpmx_damage_free();
     lpmx_accel_free();
     lpmx_cp_free();
+    lpmx_adjoint_free();
     memset(&g, 0, sizeof(g));
     g_have_model = g_have_pattern = g_have_nbr = false;
 }
@@ -1177,8 +1178,20 @@ extern "C" int lpmx_set_materials(int ntype, const int *type, const double *Ce, 
     LAUNCHCHK();
     hipLaunchKernelGGL(k_kntv, dim3(nblocks((int64_t)n * g.nb)), dim3(LPMX_BLOCK), 0, g_stream, g);
     LAUNCHCHK();
+    // kept for the adjoint (d Kn / d factor, d c44 / d factor)
+    if (g.ktype_t)
+        hipFree(g.ktype_t), hipFree(g.ce_t);
+    if (!g.emod)
+        HIPCHK(hipMalloc(&g.emod, sizeof(double) * n));
+    g.ktype_t = dk, g.ce_t = dce, g.ntype = ntype;
+    if (dem)
+        HIPCHK(hipMemcpyAsync(g.emod, dem, sizeof(double) * n, hipMemcpyDeviceToDevice, g_stream));
+    else
+    {
+        std::vector<double> ones(n, 1.0);
+        HIPCHK(hipMemcpyAsync(g.emod, ones.data(), sizeof(double) * n, hipMemcpyHostToDevice, g_stream));
+    }
     HIPCHK(hipStreamSynchronize(g_stream));
-    hipFree(dk), hipFree(dce);
     if (dem)
         hipFree(dem);
     return LPMX_OK;

@@ -181,6 +181,16 @@ void orc_compute_cab(orc_model *m);
 void orc_bond_force_cp(orc_model *m, int ii);
 int orc_lu_solve(int n, double *a, double *b);
 void orc_cp_rss(orc_model *m, int i, const double *gamma, double *rss);
+
+/* ---- adjoint of the J2 load path ------------------------------------------------- */
+/* One J2 evaluation (orc_bond_force_j2 for every particle, no stress, no switch). */
+void orc_eval_j2(orc_model *m);
+/* Reverse mode of orc_eval_j2 at the current positions and slot-0 state: seeds on
+ * Pin [3n], dLp slot 2 [n*nbmax], beta slot 2 [n*6], alpha slot 2 [n]; adjoints of
+ * the positions [3n], dLp slot 0, beta slot 0, alpha slot 0, the per-particle
+ * modulus factor [n], the yield stress [n] and H. */
+void orc_vjp_j2(orc_model *m, const double *Pb, const double *dLp2b, const double *beta2b, const double *alpha2b,
+                double *xb, double *dLp0b, double *beta0b, double *alpha0b, double *eb, double *syb, double *Hb);
 void orc_compute_stress(orc_model *m);
 void orc_switch_state(orc_model *m, int flag);
 void orc_bond_force_general(orc_model *m, int mode);

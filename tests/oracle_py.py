@@ -95,6 +95,10 @@ def lib():
                   "orc_compute_cab"):
             getattr(L, f).argtypes = [P]
             getattr(L, f).restype = None
+        L.orc_eval_j2.argtypes = [P]
+        L.orc_eval_j2.restype = None
+        L.orc_vjp_j2.argtypes = [P, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.orc_vjp_j2.restype = None
         L.orc_slip_systems_fcc.argtypes = [_dp, _dp]
         L.orc_slip_systems_fcc.restype = C.c_int
         L.orc_cp_rss.argtypes = [P, C.c_int, _dp, _dp]

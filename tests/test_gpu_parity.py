@@ -454,3 +454,46 @@ def test_crystal_plasticity_load_steps_match_oracle(oracle, prec):
     u, uo = get("XYZ", (n, 3)) - x, m.x - x
     # FP64 to 1e-9; the mixed form's stated displacement tolerance is 1e-8
     assert np.abs(u - uo).max() <= (1e-9 if prec == P.LPMX_FP64 else 1e-8) * np.abs(uo).max()
+
+
+def test_j2_vjp_matches_oracle(oracle):
+    """The device reverse sweep of one J2 force evaluation against the oracle's
+    (itself pinned by central differences): same seeds, same state."""
+    rng = np.random.default_rng(21)
+    k = 6
+    L = P.sc_box_for(k)
+    n0 = k ** 3
+    emod = 1.0 + 0.05 * rng.uniform(-1, 1, n0)
+    sy = 200.0 + 20 * rng.uniform(-1, 1, n0)
+    n, x, t, c = P.build_sc_model(L, sigmay=sy, H=2000.0, emod=emod)
+    P.check(P.lib().lpmx_set_params(C.byref(P.Params(0, 2000.0, 0.3, 0.5, 0.9, 0.0, 1e-2, 20))), "params")
+    m = oracle_model(oracle, x, 2, t, [c] * 5, plmode=0, sigmay=sy, J2_H=2000.0, J2_xi=0.3, emod=emod)
+    nb = m.nbmax
+    live = m.nbr >= 0
+    eps = 1.2e-3 * np.array([[1.0, 0.1, 0], [0.1, -0.4, 0.05], [0, 0.05, -0.3]])
+    xd = x + x @ eps.T + 2e-5 * rng.standard_normal((n, 3))
+    dlp = 1e-5 * rng.standard_normal((n, nb)) * live
+    beta = 5.0 * rng.standard_normal((n, 6))
+    alpha = 1e-3 * rng.uniform(0, 1, n)
+    push("XYZ", xd)
+    push("DLP0", dlp)
+    push("J2_BETA0", beta)
+    push("J2_ALPHA0", alpha)
+    m.x[:] = xd
+    m.dLp[:, :, 0] = dlp
+    m.beta[:, :, 0] = beta
+    m.alpha[:, 0] = alpha
+    seeds = [rng.standard_normal(3 * n), np.ascontiguousarray(rng.standard_normal((n, nb)) * live * 1e5),
+             np.ascontiguousarray(rng.standard_normal((n, 6))), rng.standard_normal(n) * 1e3]
+    dev = [np.zeros(3 * n), np.zeros((n, nb)), np.zeros((n, 6)), np.zeros(n), np.zeros(n), np.zeros(n)]
+    hb = np.zeros(1)
+    P.check(P.lib().lpmx_vjp(*[P.ptr(s) for s in seeds], *[P.ptr(o) for o in dev], P.ptr(hb)), "vjp")
+    ref = [np.zeros(3 * n), np.zeros((n, nb)), np.zeros((n, 6)), np.zeros(n), np.zeros(n), np.zeros(n)]
+    hr = C.c_double()
+    oracle.lib().orc_eval_j2(m.p)
+    assert m.pl_flag.sum() > 0, "the state is past yield"
+    oracle.lib().orc_vjp_j2(m.p, *[oracle.ptr(s) for s in seeds], *[oracle.ptr(o) for o in ref], C.byref(hr))
+    for name, a, b in zip(["positions", "dLp0", "beta0", "alpha0", "modulus", "sigmay"], dev, ref):
+        assert np.abs(b).max() > 0, name
+        assert np.abs(a - b).max() <= 1e-11 * np.abs(b).max(), name
+    assert abs(hb[0] - hr.value) <= 1e-11 * abs(hr.value)

@@ -535,3 +535,97 @@ def test_accelerated_loop_reaches_the_plain_equilibrium(oracle):
     assert np.abs(u0 - u5).max() <= 1e-9 * np.abs(u0).max()
     np.testing.assert_allclose(out[5][2], out[0][2], rtol=1e-9)
     assert sum(out[5][1]) < 0.5 * sum(out[0][1])
+
+
+# ------------------------------------------------------------------ adjoint --------
+def j2_state_model(oracle, seed, xi=0.3):
+    """SC model past yield with a non-trivial slot-0 state and modulus field."""
+    rng = np.random.default_rng(seed)
+    x = oracle.create_lattice(2, 0.5, [0, 5.125, 0, 5.125, 0, 5.125])
+    n = x.shape[0]
+    c = oracle.elastic_constants(E0, NU0)
+    emod = 1.0 + 0.05 * rng.uniform(-1, 1, n)
+    m = oracle.Model(x, 2, 0.5, np.zeros(n, np.int32), [c], plmode=0, sigmay=200.0 + 20 * rng.uniform(-1, 1, n),
+                     J2_H=2000.0, J2_xi=xi, emod=emod)
+    m.setup()
+    eps = 1.2e-3 * np.array([[1.0, 0.1, 0], [0.1, -0.4, 0.05], [0, 0.05, -0.3]])
+    m.x[:] = m.x0 + m.x0 @ eps.T + 2e-5 * rng.standard_normal((n, 3))
+    live = m.nbr >= 0
+    m.dLp[:, :, 0] = 1e-5 * rng.standard_normal(m.nbr.shape) * live
+    m.beta[:, :, 0] = 5.0 * rng.standard_normal((n, 6))
+    m.alpha[:, 0] = 1e-3 * rng.uniform(0, 1, n)
+    return m, rng
+
+
+def j2_objective(oracle, m, seeds):
+    Pb, dLp2b, beta2b, alpha2b = seeds
+    oracle.lib().orc_kntv(m.p)
+    oracle.lib().orc_eval_j2(m.p)
+    return (Pb @ m.Pin + np.sum(dLp2b * m.dLp[:, :, 2]) + np.sum(beta2b * m.beta[:, :, 2]) +
+            alpha2b @ m.alpha[:, 2])
+
+
+def test_j2_vjp_matches_central_differences(oracle):
+    """Reverse mode of one J2 force evaluation against central differences of the
+    oracle's own forward law, along random directions of every input: positions,
+    the slot-0 plastic stretch, back stress and hardening variable, the
+    per-particle modulus factor and yield stress, and H."""
+    m, rng = j2_state_model(oracle, 11)
+    n, nb = m.n, m.nbmax
+    live = m.nbr >= 0
+    seeds = (rng.standard_normal(3 * n), rng.standard_normal((n, nb)) * live * 1e5,
+             rng.standard_normal((n, 6)), rng.standard_normal(n) * 1e3)
+    j2_objective(oracle, m, seeds)
+    assert 0 < m.pl_flag.sum() < n, "some particles yield, some do not"
+    out = [np.zeros(3 * n), np.zeros((n, nb)), np.zeros((n, 6)), np.zeros(n), np.zeros(n), np.zeros(n)]
+    Hb = C.c_double()
+    P = oracle.ptr
+    oracle.lib().orc_vjp_j2(m.p, P(seeds[0]), P(np.ascontiguousarray(seeds[1])), P(np.ascontiguousarray(seeds[2])),
+                            P(seeds[3]), *[P(o) for o in out], C.byref(Hb))
+    xb, dLp0b, beta0b, alpha0b, eb, syb = out
+
+    def fd(setter, direction, h):
+        setter(+h)
+        jp = j2_objective(oracle, m, seeds)
+        setter(-2 * h)
+        jm = j2_objective(oracle, m, seeds)
+        setter(+h)
+        return (jp - jm) / (2 * h)
+
+    checks = []
+    d = rng.standard_normal((n, 3))
+
+    def sx(h):
+        m.x[:] += h * d
+    checks.append(("positions", fd(sx, d, 1e-7), xb @ d.reshape(-1)))
+    d2 = rng.standard_normal((n, nb)) * live
+
+    def sp(h):
+        m.dLp[:, :, 0] += h * d2
+    checks.append(("dLp0", fd(sp, d2, 1e-9), np.sum(dLp0b * d2)))
+    d3 = rng.standard_normal((n, 6))
+
+    def sb(h):
+        m.beta[:, :, 0] += h * d3
+    checks.append(("beta0", fd(sb, d3, 1e-3), np.sum(beta0b * d3)))
+    d4 = rng.uniform(0, 1, n)
+
+    def sa(h):
+        m.alpha[:, 0] += h * d4
+    checks.append(("alpha0", fd(sa, d4, 1e-7), alpha0b @ d4))
+    d5 = rng.standard_normal(n)
+
+    def se(h):
+        m.emod[:] += h * d5
+    checks.append(("modulus", fd(se, d5, 1e-7), eb @ d5))
+    d6 = rng.standard_normal(n)
+
+    def ss(h):
+        m.sigmay[:] += h * d6
+    checks.append(("sigmay", fd(ss, d6, 1e-3), syb @ d6))
+
+    def sh(h):
+        m.s.J2_H += h
+    checks.append(("H", fd(sh, None, 1e-2), Hb.value))
+    for name, f, a in checks:
+        assert abs(f - a) <= 1e-6 * max(abs(f), abs(a)) + 1e-12, (name, f, a)
