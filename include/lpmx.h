@@ -220,6 +220,52 @@ int lpmx_run_steps(const lpmx_solver *s, int nsteps, const int *nbd, const lpmx_
                    const int *nbf, const lpmx_forcebc *fbc, int maxbc, char out_axis, int out_type,
                    lpmx_step_stats *out);
 
+/* ---- adjoint of a J2 load history (no counterpart in the reference) ------------------- */
+/* The forward half: lpmx_run_steps with the damage update switched off (J2, plmode 0,
+ * 3D), recording what the reverse sweep needs.  Every ckpt_every steps the state before
+ * the step is kept (positions, bond forces, Pin, Pex, the converged slot state,
+ * constraint masks); the steps of the last segment are kept whole.  ckpt_every = 0
+ * keeps every step (no recomputation).  reactions[nsteps] gets writeReaction of
+ * (out_axis, out_type) after each step, which is what the reverse sweep differentiates. */
+int lpmx_history_forward(const lpmx_solver *s, int ckpt_every, int nsteps, const int *nbd,
+                         const lpmx_dispbc *dbc, const int *nbf, const lpmx_forcebc *fbc, int maxbc,
+                         char out_axis, int out_type, double *reactions, lpmx_step_stats *out);
+
+typedef struct lpmx_adjoint_opts
+{
+    double tol;             /* relative residual |b - J^T lambda| / |b| of each step's
+                             * adjoint solve (J: consistent Jacobian of the free DOFs) */
+    int maxit;              /* cap on the adjoint iterations of one step */
+    int depth;              /* Anderson depth of the K-preconditioned fixed-point
+                             * iteration lambda <- lambda + K^-1 (b - J^T lambda), 0..8 */
+    int precision;          /* LPMX_FP64 or LPMX_MIXED for the K^-1 solves */
+    double rtol2;           /* CG test of the K^-1 solves (squared, relative) */
+    int cg_maxit;           /* their cap */
+} lpmx_adjoint_opts;
+
+typedef struct lpmx_adjoint_stats
+{
+    double t_forward;       /* lpmx_history_forward, seconds */
+    double t_recompute;     /* forward steps re-run from checkpoints */
+    double t_backward;      /* reverse sweep without the recomputation */
+    int checkpoints, recomputed_steps;
+    int adj_iters, adj_cg_iters; /* totals over the steps */
+    int adj_iters_max;      /* largest per step */
+    double max_rel_res;     /* largest final |b - J^T lambda| / |b| */
+    double max_rel_free;    /* largest |xbar on free DOFs| / |b| after the last VJP of a step */
+    double ckpt_bytes;      /* device memory held by the history */
+} lpmx_adjoint_stats;
+
+/* The reverse half: given Rbar[nsteps] = dJ/dR_n for the reactions of the recorded
+ * history, the gradient of J through every step's equilibrium and return map:
+ * g_emod [n] (per-particle modulus factor), g_sigmay [n] (per-particle yield stress),
+ * *g_H (hardening modulus).  dJ/dE of a global modulus is sum_i emod_i g_emod_i / E.
+ * Leaves the model in the state after the last forward step. */
+int lpmx_history_backward(const lpmx_adjoint_opts *o, const double *Rbar, double *g_emod, double *g_sigmay,
+                          double *g_H, lpmx_adjoint_stats *st);
+/* Release the recorded history. */
+void lpmx_history_free(void);
+
 /* ---- measurement ---------------------------------------------------------------------- */
 /* Time `reps` launches of the CG's block SpMV kernel (w = K z over the gathered columns,
  * p = z + beta p, q = w + beta q, partial p.q; FP64 or FP32) with HIP events on the

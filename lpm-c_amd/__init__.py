@@ -70,6 +70,18 @@ class StepStats(C.Structure):
                 ("accel_mixed", C.c_int), ("accel_reverts", C.c_int)]
 
 
+class AdjointOpts(C.Structure):
+    _fields_ = [("tol", C.c_double), ("maxit", C.c_int), ("depth", C.c_int), ("precision", C.c_int),
+                ("rtol2", C.c_double), ("cg_maxit", C.c_int)]
+
+
+class AdjointStats(C.Structure):
+    _fields_ = [("t_forward", C.c_double), ("t_recompute", C.c_double), ("t_backward", C.c_double),
+                ("checkpoints", C.c_int), ("recomputed_steps", C.c_int), ("adj_iters", C.c_int),
+                ("adj_cg_iters", C.c_int), ("adj_iters_max", C.c_int), ("max_rel_res", C.c_double),
+                ("max_rel_free", C.c_double), ("ckpt_bytes", C.c_double)]
+
+
 # reference API entry points declared in include/*.h (argument types for ctypes)
 _API = {
     "createCuboid": [], "initMatrices": [], "searchNormalNeighbor": [], "searchAFEMNeighbor": [],
@@ -122,6 +134,9 @@ def lib():
         "lpmx_spmv_bytes": [C.c_int, _dp],
         "lpmx_compute_cab": [],
         "lpmx_vjp": [_dp] * 11,
+        "lpmx_history_forward": [C.POINTER(Solver), C.c_int, C.c_int, _ip, C.POINTER(DispBC), _ip,
+                                 C.POINTER(ForceBC), C.c_int, C.c_char, C.c_int, _dp, C.POINTER(StepStats)],
+        "lpmx_history_backward": [C.POINTER(AdjointOpts), _dp, _dp, _dp, _dp, C.POINTER(AdjointStats)],
         "lpmx_cp_failures": [_ip],
         "lpmx_matrix_info": [C.POINTER(C.c_int64), C.POINTER(C.c_int64), _ip],
         "lpmx_spmv": [C.c_int, _dp, _dp],
@@ -130,6 +145,8 @@ def lib():
         fn = getattr(L, name)
         fn.argtypes = args
         fn.restype = C.c_int
+    L.lpmx_history_free.argtypes = []
+    L.lpmx_history_free.restype = None
     L.lpmx_profile_spmv.argtypes = [C.c_int]
     L.lpmx_profile_spmv.restype = None
     L.lpmx_profile_spmv_read.argtypes = [_dp, C.POINTER(C.c_int64)]
@@ -324,6 +341,80 @@ def run_steps(steps, solv, out_axis="z", out_type=1):
     rc = lib().lpmx_run_steps(C.byref(solv), len(steps), ptr(nbd), dbc, ptr(nbf), fbc, maxbc, out_axis.encode(),
                               out_type, out)
     return rc, list(out)
+
+
+# ------------------------------------------------------------------ adjoint -----
+def history_forward(steps, solv, ckpt_every=5, out_axis="z", out_type=1):
+    """lpmx_history_forward: the load steps with damage off, recorded for the reverse
+    sweep (a checkpoint every ckpt_every steps).  Returns (rc, reactions, stats)."""
+    maxbc, nbd, dbc, nbf, fbc = bc_tables(steps)
+    reac = np.zeros(len(steps))
+    out = (StepStats * len(steps))()
+    rc = lib().lpmx_history_forward(C.byref(solv), ckpt_every, len(steps), ptr(nbd), dbc, ptr(nbf), fbc, maxbc,
+                                    out_axis.encode(), out_type, ptr(reac), out)
+    return rc, reac, list(out)
+
+
+def adjoint_opts(tol=1e-10, maxit=400, depth=8, precision=LPMX_FP64, rtol2=1e-8, cg_maxit=None, n=None):
+    o = AdjointOpts()
+    o.tol, o.maxit, o.depth, o.precision, o.rtol2 = tol, maxit, depth, precision, rtol2
+    o.cg_maxit = cg_maxit if cg_maxit is not None else 3 * (n if n is not None else gint("nparticle").value)
+    return o
+
+
+def history_backward(opts, rbar):
+    """lpmx_history_backward for dJ/dR = rbar.  Returns (rc, g_emod [n], g_sigmay [n],
+    g_H, stats)."""
+    n = gint("nparticle").value
+    rb = np.ascontiguousarray(rbar, np.float64)
+    ge, gy, gh = np.zeros(n), np.zeros(n), np.zeros(1)
+    st = AdjointStats()
+    rc = lib().lpmx_history_backward(C.byref(opts), ptr(rb), ptr(ge), ptr(gy), ptr(gh), C.byref(st))
+    return rc, ge, gy, gh[0], st
+
+
+_rh_class = None
+
+
+def reaction_history():
+    """A torch.autograd.Function R = f(E, emod, sigmay, H): the reaction curve of an SC
+    uniaxial J2 history on the device, differentiated by the discrete adjoint.
+
+        R = reaction_history().apply(E, emod, sigmay, H, spec)
+
+    E and H are 0-d float64 tensors, emod and sigmay [n] float64 tensors; spec is a
+    dict with L, nu, steps (BC tables), solver, ckpt_every and opts (adjoint options).
+    Each call builds the model on the device; backward runs the reverse sweep of the
+    history that forward recorded, so a backward belongs to the latest forward."""
+    global _rh_class
+    if _rh_class is not None:
+        return _rh_class
+    import torch
+
+    class ReactionHistory(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, E, emod, sigmay, H, spec):
+            Ev, Hv = float(E), float(H)
+            em = emod.detach().cpu().double().numpy().copy()
+            sy = sigmay.detach().cpu().double().numpy().copy()
+            build_sc_model(spec["L"], E=Ev, nu=spec.get("nu", 0.3), sigmay=sy, H=Hv, emod=em)
+            rc, reac, _ = history_forward(spec["steps"], spec["solver"], spec.get("ckpt_every", 5),
+                                          spec.get("axis", "z"), spec.get("out_type", 1))
+            check(rc, "lpmx_history_forward")
+            ctx.spec, ctx.E, ctx.em = spec, Ev, em
+            return torch.from_numpy(reac)
+
+        @staticmethod
+        def backward(ctx, gR):
+            opts = ctx.spec.get("opts") or adjoint_opts()
+            rc, ge, gy, gh, _ = history_backward(opts, gR.detach().cpu().double().numpy())
+            check(rc, "lpmx_history_backward")
+            gE = float(np.dot(ctx.em, ge)) / ctx.E
+            return (torch.tensor(gE, dtype=torch.float64), torch.from_numpy(ge), torch.from_numpy(gy),
+                    torch.tensor(gh, dtype=torch.float64), None)
+
+    _rh_class = ReactionHistory
+    return _rh_class
 
 
 # ------------------------------------------------------------ configurations -----

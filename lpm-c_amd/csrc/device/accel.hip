@@ -12,6 +12,9 @@
 // DOFs have d = 0 in every iterate, so every difference is 0 there and the
 // prescribed displacements are untouched.
 //
+// The same mixing serves the fixed-point iteration of the adjoint solve on a DOF
+// vector (lpmx_aa_*).
+//
 // Device: the history lives in DOF vectors (n*dim doubles); the 2m dot products of
 // an iteration are one pass with per-block partials and a fixed-order total, so the
 // result is deterministic.  Host: the m x m least-squares system, by Cholesky of the
@@ -28,37 +31,42 @@ struct AaPtrs
     double *dU[AA_MAX], *dF[AA_MAX];
 };
 
-static struct
+// One Anderson history over DOF vectors of n*dim entries.  The iterate itself may
+// be strided: entry (i, c) of the iterate is at xs*i + c (xs = 3 for positions,
+// xs = dim for a DOF vector).
+struct AaHist
 {
     int n = 0, dim = 0, depth = 0, cnt = 0, next = 0, have_prev = 0;
     double *u_prev = nullptr, *f_prev = nullptr, *parts = nullptr, *out = nullptr;
     AaPtrs P{};
     double G[AA_MAX][AA_MAX];
-} A;
+};
+
+static AaHist A; // the Newton loop's history
 
 static const int AA_GRID = 1024;
 
-void lpmx_accel_free(void)
+static void aa_release(AaHist *a)
 {
     for (int j = 0; j < AA_MAX; j++)
     {
-        if (A.P.dU[j])
-            hipFree(A.P.dU[j]);
-        if (A.P.dF[j])
-            hipFree(A.P.dF[j]);
+        if (a->P.dU[j])
+            hipFree(a->P.dU[j]);
+        if (a->P.dF[j])
+            hipFree(a->P.dF[j]);
     }
-    void *ps[] = {A.u_prev, A.f_prev, A.parts, A.out};
+    void *ps[] = {a->u_prev, a->f_prev, a->parts, a->out};
     for (void *p : ps)
         if (p)
             hipFree(p);
-    memset(&A.P, 0, sizeof(A.P));
-    A.u_prev = A.f_prev = A.parts = A.out = nullptr;
-    A.n = A.dim = A.depth = A.cnt = A.next = A.have_prev = 0;
+    *a = AaHist();
 }
 
-// u = xyz - d (the iterate the solve started from); with a previous iterate the new
+void lpmx_accel_free(void) { aa_release(&A); }
+
+// u = x - d (the iterate the update started from); with a previous iterate the new
 // differences go to slot s; then (u, d) become the previous iterate.
-__global__ void k_aa_push(int n, int dim, const double *__restrict__ xyz, const double *__restrict__ d,
+__global__ void k_aa_push(int n, int dim, int xs, const double *__restrict__ x, const double *__restrict__ d,
                           double *__restrict__ u_prev, double *__restrict__ f_prev, double *__restrict__ dU,
                           double *__restrict__ dF, int have_prev)
 {
@@ -67,7 +75,7 @@ __global__ void k_aa_push(int n, int dim, const double *__restrict__ xyz, const 
         return;
     const int64_t i = t / dim;
     const int c = (int)(t - i * dim);
-    const double u = xyz[3 * i + c] - d[t];
+    const double u = x[xs * i + c] - d[t];
     if (have_prev)
     {
         dU[t] = u - u_prev[t];
@@ -123,9 +131,9 @@ __global__ __launch_bounds__(256) void k_aa_dots_total(const double *parts, int 
     }
 }
 
-// xyz -= sum_j gamma_j (dU_j + dF_j)
-__global__ void k_aa_mix(int n, int dim, double *__restrict__ xyz, AaPtrs P, int cnt, double g0, double g1, double g2,
-                         double g3, double g4, double g5, double g6, double g7)
+// x -= sum_j gamma_j (dU_j + dF_j)
+__global__ void k_aa_mix(int n, int dim, int xs, double *__restrict__ x, AaPtrs P, int cnt, double g0, double g1,
+                         double g2, double g3, double g4, double g5, double g6, double g7)
 {
     const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (t >= (int64_t)n * dim)
@@ -138,11 +146,11 @@ __global__ void k_aa_mix(int n, int dim, double *__restrict__ xyz, AaPtrs P, int
             s += gam[j] * (P.dU[j][t] + P.dF[j][t]);
     const int64_t i = t / dim;
     const int c = (int)(t - i * dim);
-    xyz[3 * i + c] -= s;
+    x[xs * i + c] -= s;
 }
 
-// xyz = u_prev + f_prev: the plain modified-Newton iterate of the last solve.
-__global__ void k_aa_plain(int n, int dim, double *__restrict__ xyz, const double *__restrict__ u_prev,
+// x = u_prev + f_prev: the plain iterate of the last update.
+__global__ void k_aa_plain(int n, int dim, int xs, double *__restrict__ x, const double *__restrict__ u_prev,
                            const double *__restrict__ f_prev)
 {
     const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -150,50 +158,48 @@ __global__ void k_aa_plain(int n, int dim, double *__restrict__ xyz, const doubl
         return;
     const int64_t i = t / dim;
     const int c = (int)(t - i * dim);
-    xyz[3 * i + c] = u_prev[t] + f_prev[t];
+    x[xs * i + c] = u_prev[t] + f_prev[t];
 }
 
 // Start a new history (a new load step, or a re-solve after bond breakage: the
 // tangent, and with it the map, has changed).
-int lpmx_accel_begin(int depth)
+static int aa_begin(AaHist *a, int depth)
 {
     if (depth < 0 || depth > AA_MAX)
         return lpmx_fail(LPMX_ERR_ARG, "acceleration depth %d outside 0..%d", depth, AA_MAX);
-    if (A.n != g.n || A.dim != g.dim)
-        lpmx_accel_free();
+    if (a->n != g.n || a->dim != g.dim)
+        aa_release(a);
     const int64_t m = (int64_t)g.n * g.dim;
     for (int j = 0; j < depth; j++)
     {
-        if (!A.P.dU[j])
-            HIPCHK(hipMalloc(&A.P.dU[j], sizeof(double) * m));
-        if (!A.P.dF[j])
-            HIPCHK(hipMalloc(&A.P.dF[j], sizeof(double) * m));
+        if (!a->P.dU[j])
+            HIPCHK(hipMalloc(&a->P.dU[j], sizeof(double) * m));
+        if (!a->P.dF[j])
+            HIPCHK(hipMalloc(&a->P.dF[j], sizeof(double) * m));
     }
-    if (depth > 0 && !A.u_prev)
+    if (depth > 0 && !a->u_prev)
     {
-        HIPCHK(hipMalloc(&A.u_prev, sizeof(double) * m));
-        HIPCHK(hipMalloc(&A.f_prev, sizeof(double) * m));
-        HIPCHK(hipMalloc(&A.parts, sizeof(double) * AA_GRID * 2 * AA_MAX));
-        HIPCHK(hipMalloc(&A.out, sizeof(double) * 2 * AA_MAX));
+        HIPCHK(hipMalloc(&a->u_prev, sizeof(double) * m));
+        HIPCHK(hipMalloc(&a->f_prev, sizeof(double) * m));
+        HIPCHK(hipMalloc(&a->parts, sizeof(double) * AA_GRID * 2 * AA_MAX));
+        HIPCHK(hipMalloc(&a->out, sizeof(double) * 2 * AA_MAX));
     }
-    A.n = g.n, A.dim = g.dim;
-    A.depth = depth;
-    A.cnt = A.next = A.have_prev = 0;
+    a->n = g.n, a->dim = g.dim;
+    a->depth = depth;
+    a->cnt = a->next = a->have_prev = 0;
     return LPMX_OK;
 }
 
-// Forget the differences (keeps the last iterate as the reference of the next one).
-void lpmx_accel_forget(void) { A.cnt = A.next = 0; }
+static void aa_forget(AaHist *a) { a->cnt = a->next = 0; }
 
-// Revert xyz to the plain iterate of the last solve and forget the history.
-int lpmx_accel_revert(void)
+static int aa_revert(AaHist *a, double *x, int xs)
 {
-    if (A.depth == 0 || !A.have_prev)
+    if (a->depth == 0 || !a->have_prev)
         return LPMX_OK;
-    hipLaunchKernelGGL(k_aa_plain, dim3(nblocks((int64_t)g.n * g.dim)), dim3(LPMX_BLOCK), 0, g_stream, g.n, g.dim, g.x,
-                       A.u_prev, A.f_prev);
+    hipLaunchKernelGGL(k_aa_plain, dim3(nblocks((int64_t)g.n * g.dim)), dim3(LPMX_BLOCK), 0, g_stream, g.n, g.dim, xs,
+                       x, a->u_prev, a->f_prev);
     LAUNCHCHK();
-    lpmx_accel_forget();
+    aa_forget(a);
     return LPMX_OK;
 }
 
@@ -242,47 +248,68 @@ static bool small_spd_solve(int cnt, const double G[AA_MAX][AA_MAX], const doubl
     return true;
 }
 
-// After a solve has set xyz = u_k + d_k (d_k in g.disp): record the iterate and, with
-// a history, move xyz to the accelerated iterate.  *mixed = 1 when it did.
-int lpmx_accel_step(int *mixed)
+// After an update has set x = u_k + d_k: record the iterate and, with a history,
+// move x to the accelerated iterate.  *mixed = 1 when it did.
+static int aa_step(AaHist *a, double *x, int xs, const double *d, int *mixed)
 {
     *mixed = 0;
-    if (A.depth == 0)
+    if (a->depth == 0)
         return LPMX_OK;
     const int n = g.n, dim = g.dim;
     const int64_t m = (int64_t)n * dim;
     const int nb = nblocks(m);
-    const int s = A.next;
-    hipLaunchKernelGGL(k_aa_push, dim3(nb), dim3(LPMX_BLOCK), 0, g_stream, n, dim, g.x, g.disp, A.u_prev, A.f_prev,
-                       A.P.dU[s], A.P.dF[s], A.have_prev);
+    const int s = a->next;
+    hipLaunchKernelGGL(k_aa_push, dim3(nb), dim3(LPMX_BLOCK), 0, g_stream, n, dim, xs, x, d, a->u_prev, a->f_prev,
+                       a->P.dU[s], a->P.dF[s], a->have_prev);
     LAUNCHCHK();
-    if (!A.have_prev)
+    if (!a->have_prev)
     {
-        A.have_prev = 1;
+        a->have_prev = 1;
         return LPMX_OK;
     }
-    if (A.cnt < A.depth)
-        A.cnt++;
-    A.next = (A.next + 1) % A.depth;
-    const int cnt = A.cnt;
-    hipLaunchKernelGGL(k_aa_dots, dim3(AA_GRID), dim3(256), 0, g_stream, m, A.P, cnt, s, g.disp, A.parts);
-    hipLaunchKernelGGL(k_aa_dots_total, dim3(1), dim3(256), 0, g_stream, A.parts, AA_GRID, cnt, A.out);
+    if (a->cnt < a->depth)
+        a->cnt++;
+    a->next = (a->next + 1) % a->depth;
+    const int cnt = a->cnt;
+    hipLaunchKernelGGL(k_aa_dots, dim3(AA_GRID), dim3(256), 0, g_stream, m, a->P, cnt, s, d, a->parts);
+    hipLaunchKernelGGL(k_aa_dots_total, dim3(1), dim3(256), 0, g_stream, a->parts, AA_GRID, cnt, a->out);
     LAUNCHCHK();
     double o[2 * AA_MAX];
-    HIPCHK(hipMemcpyAsync(o, A.out, sizeof(o), hipMemcpyDeviceToHost, g_stream));
+    HIPCHK(hipMemcpyAsync(o, a->out, sizeof(o), hipMemcpyDeviceToHost, g_stream));
     HIPCHK(hipStreamSynchronize(g_stream));
     for (int j = 0; j < cnt; j++)
-        A.G[s][j] = A.G[j][s] = o[j];
+        a->G[s][j] = a->G[j][s] = o[j];
     double gam[AA_MAX] = {0, 0, 0, 0, 0, 0, 0, 0};
-    if (!small_spd_solve(cnt, A.G, o + AA_MAX, gam))
+    if (!small_spd_solve(cnt, a->G, o + AA_MAX, gam))
     {
         // degenerate history: keep the plain iterate, start the differences afresh
-        lpmx_accel_forget();
+        aa_forget(a);
         return LPMX_OK;
     }
-    hipLaunchKernelGGL(k_aa_mix, dim3(nb), dim3(LPMX_BLOCK), 0, g_stream, n, dim, g.x, A.P, cnt, gam[0], gam[1],
+    hipLaunchKernelGGL(k_aa_mix, dim3(nb), dim3(LPMX_BLOCK), 0, g_stream, n, dim, xs, x, a->P, cnt, gam[0], gam[1],
                        gam[2], gam[3], gam[4], gam[5], gam[6], gam[7]);
     LAUNCHCHK();
     *mixed = 1;
     return LPMX_OK;
 }
+
+// --------------------------------------------------------- the Newton loop's -----
+int lpmx_accel_begin(int depth) { return aa_begin(&A, depth); }
+void lpmx_accel_forget(void) { aa_forget(&A); }
+// Revert xyz to the plain iterate of the last solve and forget the history.
+int lpmx_accel_revert(void) { return aa_revert(&A, g.x, 3); }
+// After a solve has set xyz = u_k + d_k (d_k in g.disp).
+int lpmx_accel_step(int *mixed) { return aa_step(&A, g.x, 3, g.disp, mixed); }
+
+// ------------------------------------------------------------ other histories ----
+AaHist *lpmx_aa_create(void) { return new AaHist(); }
+void lpmx_aa_destroy(AaHist *a)
+{
+    if (a)
+    {
+        aa_release(a);
+        delete a;
+    }
+}
+int lpmx_aa_begin(AaHist *a, int depth) { return aa_begin(a, depth); }
+int lpmx_aa_step(AaHist *a, double *x, const double *d, int *mixed) { return aa_step(a, x, g.dim, d, mixed); }

@@ -25,15 +25,6 @@
 
 #define BI(j, i) ((int64_t)(j) * v.n + (i))
 
-struct VjpIO
-{
-    const double *Pb;     // [3n], Pin layout
-    const double *dLp2b;  // [nb][n] planes
-    const double *beta2b; // [6][n]
-    const double *alpha2b;
-    double *xb;           // [3n], xyz layout
-    double *dLp0b, *beta0b, *alpha0b, *eb, *syb, *Hb; // planes / [n]
-};
 
 struct AdjWork
 {
@@ -433,4 +424,8 @@ extern "C" int lpmx_vjp(const double *Pb, const double *dLp2b, const double *bet
     return rc;
 }
 
-void lpmx_adjoint_free(void) { lpmx_adjoint_free_work(); }
+void lpmx_adjoint_free(void)
+{
+    lpmx_adjoint_free_work();
+    lpmx_history_free();
+}

@@ -170,6 +170,19 @@ void lpmx_accel_free(void);
 void lpmx_cp_free(void);
 void lpmx_adjoint_free(void);
 int lpmx_eval_j2(void); // one J2 evaluation of every particle, no stress, no slot switch
+
+// Seeds and outputs of the reverse sweep of one J2 evaluation (adjoint.hip); the
+// outputs are overwritten, not accumulated.
+struct VjpIO
+{
+    const double *Pb;     // [3n], Pin layout
+    const double *dLp2b;  // [nb][n] planes
+    const double *beta2b; // [6][n]
+    const double *alpha2b;
+    double *xb;           // [3n], xyz layout
+    double *dLp0b, *beta0b, *alpha0b, *eb, *syb, *Hb; // planes / [n]
+};
+int lpmx_vjp_device(const VjpIO &io);
 int lpmx_cp_particles(void);
 
 // Anderson acceleration of the Newton loop (accel.hip)
@@ -177,3 +190,17 @@ int lpmx_accel_begin(int depth);
 int lpmx_accel_step(int *mixed);
 int lpmx_accel_revert(void);
 void lpmx_accel_forget(void);
+// Further Anderson histories over DOF vectors (x and d both [n*dim])
+struct AaHist;
+AaHist *lpmx_aa_create(void);
+void lpmx_aa_destroy(AaHist *a);
+int lpmx_aa_begin(AaHist *a, int depth);
+int lpmx_aa_step(AaHist *a, double *x, const double *d, int *mixed);
+
+// K x = b on the current constraint-eliminated tangent (solver.hip)
+int lpmx_solve_into(int precision, double rtol2, double atol2, int maxit, double inner_rtol2, int ir_maxit,
+                    const double *b, double *x, int *iters, double *rr, int *status);
+// the load-step loop with or without the damage update after each step (loop.hip)
+int lpmx_run_steps_impl(const lpmx_solver *s, int nsteps, const int *nbd, const lpmx_dispbc *dbc, const int *nbf,
+                        const lpmx_forcebc *fbc, int maxbc, char out_axis, int out_type, lpmx_step_stats *out,
+                        int skip_damage);

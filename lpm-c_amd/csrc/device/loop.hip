@@ -23,9 +23,9 @@ static int sync(void)
     return LPMX_OK;
 }
 
-extern "C" int lpmx_run_steps(const lpmx_solver *s, int nsteps, const int *nbd, const lpmx_dispbc *dbc,
-                              const int *nbf, const lpmx_forcebc *fbc, int maxbc, char out_axis, int out_type,
-                              lpmx_step_stats *out)
+int lpmx_run_steps_impl(const lpmx_solver *s, int nsteps, const int *nbd, const lpmx_dispbc *dbc, const int *nbf,
+                        const lpmx_forcebc *fbc, int maxbc, char out_axis, int out_type, lpmx_step_stats *out,
+                        int skip_damage)
 {
     RETCHK(lpmx_require_pattern());
     if (!s || nsteps < 0 || (nsteps > 0 && (!nbd || !nbf)) || maxbc <= 0)
@@ -100,7 +100,8 @@ extern "C" int lpmx_run_steps(const lpmx_solver *s, int nsteps, const int *nbd, 
             o.res_norm = nres;
             o.tol_mult = tolm;
             int broken = 0;
-            RETCHK(lpmx_damage(g.plmode, &broken));
+            if (!skip_damage)
+                RETCHK(lpmx_damage(g.plmode, &broken));
             RETCHK(lpmx_update_crack());
             RETCHK(lpmx_switch_state(1));
             RETCHK(lpmx_reaction(out_axis, out_type, &o.reaction, &o.disp));
@@ -135,4 +136,11 @@ extern "C" int lpmx_run_steps(const lpmx_solver *s, int nsteps, const int *nbd, 
     if (rc_all != LPMX_OK)
         return lpmx_fail(rc_all, "a solve or Newton loop hit its cap (see lpmx_step_stats.status)");
     return LPMX_OK;
+}
+
+extern "C" int lpmx_run_steps(const lpmx_solver *s, int nsteps, const int *nbd, const lpmx_dispbc *dbc,
+                              const int *nbf, const lpmx_forcebc *fbc, int maxbc, char out_axis, int out_type,
+                              lpmx_step_stats *out)
+{
+    return lpmx_run_steps_impl(s, nsteps, nbd, dbc, nbf, fbc, maxbc, out_axis, out_type, out, 0);
 }

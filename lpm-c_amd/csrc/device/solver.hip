@@ -597,10 +597,11 @@ static int pcg_loop(const TV *Kv, const TM *Minv, TX *x, TX *r, TX *z, TX *q, TX
 }
 
 template <int D>
-static int solve_fp64(double rtol2, double atol2, int maxit, int *iters, double *rr, int *status)
+static int solve_fp64(const double *b, double *x, double rtol2, double atol2, int maxit, int *iters, double *rr,
+                      int *status)
 {
-    return pcg_loop<D, double, double, double>(g.K, g.Minv, g.disp, W.r64, W.z64, W.q64, W.p64, g.res, rtol2, atol2,
-                                               maxit, iters, rr, status);
+    return pcg_loop<D, double, double, double>(g.K, g.Minv, x, W.r64, W.z64, W.q64, W.p64, b, rtol2, atol2, maxit,
+                                               iters, rr, status);
 }
 
 // Mixed form: one FP32 block-Jacobi CG whose residual is replaced by the FP64 one
@@ -609,17 +610,17 @@ static int solve_fp64(double rtol2, double atol2, int maxit, int *iters, double 
 // solution, r = b - K x is formed with the FP64 matrix, and the CG continues from
 // it with its search direction kept.  The outer test is FP64 throughout.
 template <int D>
-static int solve_mixed(double rtol2, double atol2, double seg_r2, int max_replace, int maxit, int *iters,
-                       double *rr, int *status, int *passes)
+static int solve_mixed(const double *b, double *x, double rtol2, double atol2, double seg_r2, int max_replace,
+                       int maxit, int *iters, double *rr, int *status, int *passes)
 {
     const int n = g.n;
     const int64_t m = (int64_t)n * D;
     const int G = nblocks(n, 256);
     double bb = 0.0;
-    RETCHK(lpmx_dev_sumsq(g.res, m, &bb));
+    RETCHK(lpmx_dev_sumsq(b, m, &bb));
     const double thr = rtol2 * bb + atol2;
-    HIPCHK(hipMemsetAsync(g.disp, 0, sizeof(double) * m, g_stream));
-    HIPCHK(hipMemcpyAsync(W.b, g.res, sizeof(double) * m, hipMemcpyDeviceToDevice, g_stream));
+    HIPCHK(hipMemsetAsync(x, 0, sizeof(double) * m, g_stream));
+    HIPCHK(hipMemcpyAsync(W.b, b, sizeof(double) * m, hipMemcpyDeviceToDevice, g_stream));
     *iters = 0, *rr = bb, *status = 1, *passes = 0;
     if (bb <= thr)
         return LPMX_OK;
@@ -634,8 +635,8 @@ static int solve_mixed(double rtol2, double atol2, double seg_r2, int max_replac
         RETCHK((cg_iterate<D, float, float, float>(g.K32, g.Minv32, W.x32, W.r32, W.z32, W.q32, W.p32, maxit)));
         st = W.hsc->done;
         pass++;
-        hipLaunchKernelGGL((k_add_f32<D>), dim3(nblocks(m)), dim3(LPMX_BLOCK), 0, g_stream, n, g.disp, W.x32);
-        hipLaunchKernelGGL((k_residual64<D>), dim3(G), dim3(256), 0, g_stream, n, g.sptr, g.col, g.K, g.disp, g.res,
+        hipLaunchKernelGGL((k_add_f32<D>), dim3(nblocks(m)), dim3(LPMX_BLOCK), 0, g_stream, n, x, W.x32);
+        hipLaunchKernelGGL((k_residual64<D>), dim3(G), dim3(256), 0, g_stream, n, g.sptr, g.col, g.K, x, b,
                            W.b, W.parts);
         hipLaunchKernelGGL(k_parts_total, dim3(1), dim3(256), 0, g_stream, W.parts, G, W.scal);
         LAUNCHCHK();
@@ -664,10 +665,11 @@ static int solve_mixed(double rtol2, double atol2, double seg_r2, int max_replac
 static int g_last_passes = 0;
 int lpmx_last_ir_passes(void) { return g_last_passes; }
 
-// Full solve of the Newton correction: block-Jacobi inverse of the current matrix,
-// PCG (FP64 or mixed), then xyz += disp.  status: 1 converged, 2 cap, 3 breakdown.
-extern "C" int lpmx_solve_ex(int precision, double rtol2, double atol2, int maxit, double inner_rtol2,
-                             int ir_maxit, int *iters, double *rr, int *status)
+// K x = b on the current (constraint-eliminated) matrix: block-Jacobi inverse, then
+// PCG (FP64 or mixed).  b and x are DOF vectors; status: 1 converged, 2 cap,
+// 3 breakdown.
+int lpmx_solve_into(int precision, double rtol2, double atol2, int maxit, double inner_rtol2, int ir_maxit,
+                    const double *b, double *x, int *iters, double *rr, int *status)
 {
     RETCHK(lpmx_require_pattern());
     RETCHK(ensure_work());
@@ -677,18 +679,34 @@ extern "C" int lpmx_solve_ex(int precision, double rtol2, double atol2, int maxi
     if (precision == LPMX_MIXED)
     {
         if (g.dim == 3)
-            RETCHK(solve_mixed<3>(rtol2, atol2, inner_rtol2, ir_maxit, maxit, &it, &r2, &st, &passes));
+            RETCHK(solve_mixed<3>(b, x, rtol2, atol2, inner_rtol2, ir_maxit, maxit, &it, &r2, &st, &passes));
         else
-            RETCHK(solve_mixed<2>(rtol2, atol2, inner_rtol2, ir_maxit, maxit, &it, &r2, &st, &passes));
+            RETCHK(solve_mixed<2>(b, x, rtol2, atol2, inner_rtol2, ir_maxit, maxit, &it, &r2, &st, &passes));
     }
     else
     {
         if (g.dim == 3)
-            RETCHK(solve_fp64<3>(rtol2, atol2, maxit, &it, &r2, &st));
+            RETCHK(solve_fp64<3>(b, x, rtol2, atol2, maxit, &it, &r2, &st));
         else
-            RETCHK(solve_fp64<2>(rtol2, atol2, maxit, &it, &r2, &st));
+            RETCHK(solve_fp64<2>(b, x, rtol2, atol2, maxit, &it, &r2, &st));
     }
     g_last_passes = passes;
+    if (iters)
+        *iters = it;
+    if (rr)
+        *rr = r2;
+    if (status)
+        *status = st;
+    return LPMX_OK;
+}
+
+// Full solve of the Newton correction K disp = residual, then xyz += disp.
+extern "C" int lpmx_solve_ex(int precision, double rtol2, double atol2, int maxit, double inner_rtol2,
+                             int ir_maxit, int *iters, double *rr, int *status)
+{
+    int it = 0, st = 0;
+    double r2 = 0;
+    RETCHK(lpmx_solve_into(precision, rtol2, atol2, maxit, inner_rtol2, ir_maxit, g.res, g.disp, &it, &r2, &st));
     hipLaunchKernelGGL(k_apply_disp, dim3(nblocks(g.n)), dim3(LPMX_BLOCK), 0, g_stream, g.n, g.dim, g.x, g.disp);
     LAUNCHCHK();
     if (iters)

@@ -144,6 +144,7 @@ typedef struct orc_solver
     int accel;           /* >0: Anderson acceleration with that many differences, to reach
                           * the same equilibrium where modified Newton crawls (off in the
                           * specification runs) */
+    int skip_damage;     /* 1: no damage update after a step (the differentiated history) */
 } orc_solver;
 
 /* Per-step outputs of orc_run_steps. */
@@ -191,6 +192,14 @@ void orc_eval_j2(orc_model *m);
  * modulus factor [n], the yield stress [n] and H. */
 void orc_vjp_j2(orc_model *m, const double *Pb, const double *dLp2b, const double *beta2b, const double *alpha2b,
                 double *xb, double *dLp0b, double *beta0b, double *alpha0b, double *eb, double *syb, double *Hb);
+/* Discrete adjoint of a J2 load history (damage off): J = sum_n (R_n - target_n)^2 with
+ * R_n the reaction along `axis` of type t1 after step n; gradients with respect to the
+ * per-particle modulus factor [n], the yield stress [n] and H.  Each step's adjoint
+ * system is formed densely and solved directly (small models only).  Returns 0, or
+ * the first step whose Newton loop failed, +1. */
+int orc_adjoint_run(orc_model *m, const orc_solver *s, int nsteps, const int *nbd, const orc_dispbc *dbc,
+                    const int *nbf, const orc_forcebc *fbc, int maxbc, char axis, int t1, const double *target,
+                    double *J, double *reactions, double *g_emod, double *g_sigmay, double *g_H);
 void orc_compute_stress(orc_model *m);
 void orc_switch_state(orc_model *m, int flag);
 void orc_bond_force_general(orc_model *m, int mode);

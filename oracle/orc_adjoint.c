@@ -350,3 +350,156 @@ void orc_vjp_j2(orc_model *m, const double *Pb, const double *dLp2b, const doubl
     *Hb = Hacc;
     free(kt), free(P), free(Fb), free(gb), free(Knb), free(Tvb), free(c44b);
 }
+
+/* ---------------------------------------------------------------- history adjoint --- */
+/* Gaussian elimination with partial pivoting, row swaps applied to b as they happen;
+ * a (n x n, row-major) is destroyed, b receives the solution.  Returns 1 if singular. */
+static int dense_solve(int n, double *a, double *b)
+{
+    for (int k = 0; k < n; k++)
+    {
+        int p = k;
+        for (int i = k + 1; i < n; i++)
+            if (fabs(a[(size_t)i * n + k]) > fabs(a[(size_t)p * n + k]))
+                p = i;
+        if (a[(size_t)p * n + k] == 0.0)
+            return 1;
+        if (p != k)
+        {
+            for (int j = 0; j < n; j++)
+            {
+                double t = a[(size_t)k * n + j];
+                a[(size_t)k * n + j] = a[(size_t)p * n + j], a[(size_t)p * n + j] = t;
+            }
+            double t = b[k];
+            b[k] = b[p], b[p] = t;
+        }
+        for (int i = k + 1; i < n; i++)
+        {
+            const double l = a[(size_t)i * n + k] / a[(size_t)k * n + k];
+            if (l == 0.0)
+                continue;
+            for (int j = k + 1; j < n; j++)
+                a[(size_t)i * n + j] -= l * a[(size_t)k * n + j];
+            b[i] -= l * b[k];
+        }
+    }
+    for (int k = n - 1; k >= 0; k--)
+    {
+        double t = b[k];
+        for (int j = k + 1; j < n; j++)
+            t -= a[(size_t)k * n + j] * b[j];
+        b[k] = t / a[(size_t)k * n + k];
+    }
+    return 0;
+}
+
+int orc_adjoint_run(orc_model *m, const orc_solver *s, int nsteps, const int *nbd, const orc_dispbc *dbc,
+                    const int *nbf, const orc_forcebc *fbc, int maxbc, char axis, int t1, const double *target,
+                    double *J, double *reactions, double *g_emod, double *g_sigmay, double *g_H)
+{
+    const int n = m->n, nbm = m->nbmax, d = m->dim, nd = d * n;
+    const int ax = axis == 'x' ? 0 : (axis == 'y' ? 1 : 2);
+    orc_solver sf = *s;
+    sf.skip_damage = 1;
+    /* per step: converged positions, incoming state, free-DOF mask */
+    double *X = malloc(sizeof(double) * 3 * n * nsteps);
+    double *Sp = malloc(sizeof(double) * (size_t)n * nbm * nsteps), *Sb = malloc(sizeof(double) * 6 * n * nsteps),
+           *Sa = malloc(sizeof(double) * n * nsteps);
+    int *Fr = malloc(sizeof(int) * nd * nsteps);
+    int fail = 0;
+    *J = 0.0;
+    for (int st = 0; st < nsteps && !fail; st++)
+    {
+        for (int b = 0; b < n * nbm; b++)
+            Sp[(size_t)st * n * nbm + b] = m->dLp[3 * b + 0];
+        for (int k = 0; k < 6 * n; k++)
+            Sb[(size_t)st * 6 * n + k] = m->beta[3 * k + 0];
+        for (int i = 0; i < n; i++)
+            Sa[(size_t)st * n + i] = m->alpha[3 * i + 0];
+        orc_stepout o;
+        if (orc_run_steps(m, &sf, 1, nbd + st, dbc + (size_t)st * maxbc, nbf + st, fbc + (size_t)st * maxbc, maxbc,
+                          axis, t1, &o))
+            fail = st + 1;
+        memcpy(X + (size_t)st * 3 * n, m->x, sizeof(double) * 3 * n);
+        for (int k = 0; k < nd; k++)
+            Fr[(size_t)st * nd + k] = m->dispBC[k] && m->fixidx[k];
+        reactions[st] = o.reaction;
+        *J += (o.reaction - target[st]) * (o.reaction - target[st]);
+    }
+    if (fail)
+    {
+        free(X), free(Sp), free(Sb), free(Sa), free(Fr);
+        return fail;
+    }
+    /* backward sweep */
+    double *sp = calloc((size_t)n * nbm, sizeof(double)), *sb = calloc(6 * n, sizeof(double)),
+           *sa = calloc(n, sizeof(double));
+    double *Pb = malloc(sizeof(double) * 3 * n), *xb = malloc(sizeof(double) * 3 * n);
+    double *o1 = malloc(sizeof(double) * n * nbm), *o2 = malloc(sizeof(double) * 6 * n), *o3 = malloc(sizeof(double) * n);
+    double *eb = malloc(sizeof(double) * n), *yb = malloc(sizeof(double) * n);
+    double *zp = calloc((size_t)n * nbm, sizeof(double)), *zb = calloc(6 * n, sizeof(double)), *za = calloc(n, sizeof(double));
+    int *fi = malloc(sizeof(int) * nd);
+    double *A = NULL, *bv = NULL;
+    memset(g_emod, 0, sizeof(double) * n);
+    memset(g_sigmay, 0, sizeof(double) * n);
+    *g_H = 0.0;
+    for (int st = nsteps - 1; st >= 0; st--)
+    {
+        memcpy(m->x, X + (size_t)st * 3 * n, sizeof(double) * 3 * n);
+        for (int b = 0; b < n * nbm; b++)
+            m->dLp[3 * b + 0] = Sp[(size_t)st * n * nbm + b];
+        for (int k = 0; k < 6 * n; k++)
+            m->beta[3 * k + 0] = Sb[(size_t)st * 6 * n + k];
+        for (int i = 0; i < n; i++)
+            m->alpha[3 * i + 0] = Sa[(size_t)st * n + i];
+        orc_eval_j2(m);
+        int nf = 0;
+        for (int k = 0; k < nd; k++)
+            if (Fr[(size_t)st * nd + k])
+                fi[nf++] = k;
+        const double jr = 2.0 * (reactions[st] - target[st]);
+        /* R = -sum_{type t1} Pin_axis */
+        memset(Pb, 0, sizeof(double) * 3 * n);
+        for (int i = 0; i < n; i++)
+            if (m->type[i] == t1)
+                Pb[3 * i + ax] = -jr;
+        double Hb;
+        orc_vjp_j2(m, Pb, sp, sb, sa, xb, o1, o2, o3, eb, yb, &Hb);
+        A = realloc(A, sizeof(double) * (size_t)nf * nf);
+        bv = realloc(bv, sizeof(double) * nf);
+        for (int r = 0; r < nf; r++)
+            bv[r] = xb[3 * (fi[r] / d) + fi[r] % d];
+        /* A[r][c] = d(P_f[c]) / d(u_f[r]) = column c of the free-DOF Jacobian, transposed */
+        for (int c = 0; c < nf; c++)
+        {
+            memset(Pb, 0, sizeof(double) * 3 * n);
+            Pb[3 * (fi[c] / d) + fi[c] % d] = 1.0;
+            double h2;
+            orc_vjp_j2(m, Pb, zp, zb, za, xb, o1, o2, o3, eb, yb, &h2);
+            for (int r = 0; r < nf; r++)
+                A[(size_t)r * nf + c] = xb[3 * (fi[r] / d) + fi[r] % d];
+        }
+        if (dense_solve(nf, A, bv))
+        {
+            fail = -1;
+            break;
+        }
+        memset(Pb, 0, sizeof(double) * 3 * n);
+        for (int i = 0; i < n; i++)
+            if (m->type[i] == t1)
+                Pb[3 * i + ax] = -jr;
+        for (int r = 0; r < nf; r++)
+            Pb[3 * (fi[r] / d) + fi[r] % d] -= bv[r];
+        orc_vjp_j2(m, Pb, sp, sb, sa, xb, o1, o2, o3, eb, yb, &Hb);
+        memcpy(sp, o1, sizeof(double) * n * nbm);
+        memcpy(sb, o2, sizeof(double) * 6 * n);
+        memcpy(sa, o3, sizeof(double) * n);
+        for (int i = 0; i < n; i++)
+            g_emod[i] += eb[i], g_sigmay[i] += yb[i];
+        *g_H += Hb;
+    }
+    free(X), free(Sp), free(Sb), free(Sa), free(Fr), free(sp), free(sb), free(sa), free(Pb), free(xb);
+    free(o1), free(o2), free(o3), free(eb), free(yb), free(zp), free(zb), free(za), free(fi), free(A), free(bv);
+    return fail;
+}

@@ -622,7 +622,7 @@ int orc_run_steps(orc_model *m, const orc_solver *s, int nsteps,
             o.newton_iters += ni;
             o.res_norm = nres;
             o.tol_mult = tolm;
-            int broken = damage_general(m);
+            int broken = s->skip_damage ? 0 : damage_general(m);
             orc_update_crack(m);
             orc_switch_state(m, 1);
             o.broken += broken;

@@ -64,7 +64,8 @@ class OrcForceBC(C.Structure):
 class OrcSolver(C.Structure):
     _fields_ = [("rtol2", C.c_double), ("atol2", C.c_double), ("maxit", C.c_int),
                 ("newton_tol", C.c_double), ("newton_maxit", C.c_int), ("tangent", C.c_int),
-                ("fixed_diag", C.c_int), ("nthreads", C.c_int), ("accel", C.c_int)]
+                ("fixed_diag", C.c_int), ("nthreads", C.c_int), ("accel", C.c_int),
+                ("skip_damage", C.c_int)]
 
 
 class OrcStepOut(C.Structure):
@@ -99,6 +100,9 @@ def lib():
         L.orc_eval_j2.restype = None
         L.orc_vjp_j2.argtypes = [P, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
         L.orc_vjp_j2.restype = None
+        L.orc_adjoint_run.argtypes = [P, C.POINTER(OrcSolver), C.c_int, _ip, C.POINTER(OrcDispBC), _ip,
+                                      C.POINTER(OrcForceBC), C.c_int, C.c_char, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.orc_adjoint_run.restype = C.c_int
         L.orc_slip_systems_fcc.argtypes = [_dp, _dp]
         L.orc_slip_systems_fcc.restype = C.c_int
         L.orc_cp_rss.argtypes = [P, C.c_int, _dp, _dp]
@@ -315,14 +319,28 @@ def run_steps(model, steps, solver, out_axis="z", out_type=1):
     return rc, list(out)
 
 
+def adjoint_run(model, steps, solver, target, out_axis="z", out_type=1):
+    """Discrete adjoint of a J2 load history with damage off: returns (rc, J, reactions,
+    dJ/d modulus factor [n], dJ/d yield stress [n], dJ/dH) for J = sum (R_n - target_n)^2."""
+    maxbc, nbd, dbc, nbf, fbc = bc_tables(steps)
+    ns, n = len(steps), model.n
+    J = np.zeros(1)
+    reac, ge, gy, gh = np.zeros(ns), np.zeros(n), np.zeros(n), np.zeros(1)
+    tgt = np.ascontiguousarray(target, np.float64)
+    rc = lib().orc_adjoint_run(model.p, C.byref(solver), ns, ptr(nbd), dbc, ptr(nbf), fbc, maxbc,
+                               out_axis.encode(), out_type, ptr(tgt), ptr(J), ptr(reac), ptr(ge), ptr(gy), ptr(gh))
+    return rc, J[0], reac, ge, gy, gh[0]
+
+
 def solver(rtol2=1e-8, atol2=1e-12, maxit=None, newton_tol=1e-4, newton_maxit=100, tangent=0, nthreads=1, n=0,
-           fixed_diag=0, accel=0):
+           fixed_diag=0, accel=0, skip_damage=0):
     s = OrcSolver()
     s.rtol2, s.atol2 = rtol2, atol2
     s.maxit = maxit if maxit is not None else max(1, n)
     s.newton_tol, s.newton_maxit, s.tangent, s.nthreads = newton_tol, newton_maxit, tangent, nthreads
     s.fixed_diag = fixed_diag
     s.accel = accel
+    s.skip_damage = skip_damage
     return s
 
 

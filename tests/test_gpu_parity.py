@@ -497,3 +497,114 @@ def test_j2_vjp_matches_oracle(oracle):
         assert np.abs(b).max() > 0, name
         assert np.abs(a - b).max() <= 1e-11 * np.abs(b).max(), name
     assert abs(hb[0] - hr.value) <= 1e-11 * abs(hr.value)
+
+
+# ------------------------------------------------------------------ history adjoint --
+def _history_case(k, seed, strains):
+    rng = np.random.default_rng(seed)
+    n0 = k ** 3
+    emod = 1.0 + 0.05 * rng.uniform(-1, 1, n0)
+    sy = 200.0 + 30.0 * rng.uniform(-1, 1, n0)
+    L = P.sc_box_for(k)
+    return L, emod, sy, P.sc_uniaxial_steps(L - 1.0, np.array(strains)), rng
+
+
+def _dev_solver(n, tol=1e-12):
+    return P.solver(precision=P.LPMX_FP64, rtol2=1e-26, atol2=0.0, maxit=100 * 3 * n, newton_tol=tol,
+                    newton_maxit=800, accel_depth=5)
+
+
+def test_history_adjoint_matches_oracle(oracle):
+    """Device discrete adjoint of a three-step J2 tension history (damage off) against
+    the oracle's, whose adjoint systems are dense LU solves (and which is pinned by
+    central differences of its own forward history).  With a checkpoint every step
+    the reverse sweep re-runs two of the three steps: the recomputed trajectory is
+    bitwise the recorded one, so the gradients are identical."""
+    L, emod, sy, steps, _ = _history_case(5, 5, [0.6e-3, 1.1e-3, 1.5e-3])
+    n, x, t, c = P.build_sc_model(L, sigmay=sy, H=2000.0, emod=emod)
+    s = _dev_solver(n)
+    res = {}
+    for ck in (0, 1):
+        if ck:
+            P.build_sc_model(L, sigmay=sy, H=2000.0, emod=emod)
+        rc, R, out = P.history_forward(steps, s, ckpt_every=ck)
+        assert rc == 0, (P.lib().lpmx_last_error(), [(o.status, o.newton_iters) for o in out])
+        rbar = 2.0 * (R - 0.9 * R)
+        x_after = get("XYZ", (n, 3)).copy()
+        a_after = get("J2_ALPHA1", (n,)).copy()
+        o = P.adjoint_opts(tol=1e-12, maxit=400, depth=8, rtol2=1e-20, n=n)
+        rc, ge, gy, gh, st = P.history_backward(o, rbar)
+        assert rc == 0, P.lib().lpmx_last_error()
+        print(f"ckpt {ck}: adj iters {st.adj_iters} (max {st.adj_iters_max}), cg {st.adj_cg_iters}, "
+              f"recomputed {st.recomputed_steps}, rel res {st.max_rel_res:.2e}, free {st.max_rel_free:.2e}")
+        assert st.recomputed_steps == (2 if ck else 0)
+        assert st.max_rel_free <= 1e-10
+        res[ck] = (R, ge, gy, gh)
+        # the model is left as the forward left it
+        np.testing.assert_array_equal(get("XYZ", (n, 3)), x_after)
+        np.testing.assert_array_equal(get("J2_ALPHA1", (n,)), a_after)
+    for a, b in zip(res[0], res[1]):
+        np.testing.assert_array_equal(a, b)
+    R, ge, gy, gh = res[0]
+    m = oracle_model(oracle, x, 2, t, [c] * 5, plmode=0, sigmay=sy, J2_H=2000.0, emod=emod)
+    so = oracle.solver(rtol2=1e-26, atol2=0.0, maxit=100 * 3 * n, newton_tol=1e-12, newton_maxit=800, tangent=1,
+                       fixed_diag=1, accel=5, skip_damage=1)
+    rc, J, Ro, geo, gyo, gho = oracle.adjoint_run(m, steps, so, 0.9 * R)
+    assert rc == 0
+    assert m.pl_flag.sum() > 0
+    np.testing.assert_allclose(R, Ro, rtol=1e-9)
+    for name, a, b in (("modulus", ge, geo), ("sigmay", gy, gyo)):
+        assert np.abs(b).max() > 0, name
+        assert np.abs(a - b).max() <= 1e-7 * np.abs(b).max(), (name, np.abs(a - b).max(), np.abs(b).max())
+    assert abs(gh - gho) <= 1e-7 * abs(gho), (gh, gho)
+
+
+def test_history_adjoint_central_differences_and_autograd():
+    """Gradients of J = sum (R_n - t_n)^2 over a six-step J2 tension history on the
+    device (checkpoint every 2 steps), through the torch.autograd harness, against
+    central differences of the device forward with h = 1e-6 relative: the global
+    modulus E, a random direction of the per-particle modulus field, the yield
+    stress field along a random direction, and H."""
+    import torch
+
+    k = 7
+    L, emod, sy, steps, rng = _history_case(k, 8, [0.5e-3, 1.0e-3, 1.4e-3, 1.8e-3, 2.2e-3, 2.6e-3])
+    n = k ** 3
+    E0, H0 = 200e3, 2000.0
+    spec = dict(L=L, nu=0.3, steps=steps, solver=_dev_solver(n), ckpt_every=2,
+                opts=P.adjoint_opts(tol=1e-11, maxit=400, depth=8, rtol2=1e-16, n=n))
+    F = P.reaction_history()
+
+    def reactions(E, em, s_, H):
+        with torch.no_grad():
+            return F.apply(torch.tensor(E, dtype=torch.float64), torch.from_numpy(em), torch.from_numpy(s_),
+                           torch.tensor(H, dtype=torch.float64), spec).numpy()
+
+    target = 0.9 * reactions(E0, emod, sy, H0)
+    tE = torch.tensor(E0, dtype=torch.float64, requires_grad=True)
+    tm = torch.from_numpy(emod.copy()).requires_grad_(True)
+    ts = torch.from_numpy(sy.copy()).requires_grad_(True)
+    tH = torch.tensor(H0, dtype=torch.float64, requires_grad=True)
+    R = F.apply(tE, tm, ts, tH, spec)
+    J = ((R - torch.from_numpy(target)) ** 2).sum()
+    J.backward()
+    assert (P.field("PL_FLAG", n) > 0).sum() > 0, "the history reaches the plastic regime"
+
+    def Jof(E, em, s_, H):
+        return float(np.sum((reactions(E, em, s_, H) - target) ** 2))
+
+    h = 1e-6
+    d1 = rng.standard_normal(n)
+    d2 = rng.standard_normal(n)
+    checks = [
+        ("E", (Jof(E0 * (1 + h), emod, sy, H0) - Jof(E0 * (1 - h), emod, sy, H0)) / (2 * h * E0), tE.grad.item()),
+        ("modulus field", (Jof(E0, emod * (1 + h * d1), sy, H0) - Jof(E0, emod * (1 - h * d1), sy, H0)) / (2 * h),
+         float(tm.grad.numpy() @ (emod * d1))),
+        ("yield stress", (Jof(E0, emod, sy * (1 + h * d2), H0) - Jof(E0, emod, sy * (1 - h * d2), H0)) / (2 * h),
+         float(ts.grad.numpy() @ (sy * d2))),
+        ("H", (Jof(E0, emod, sy, H0 * (1 + h)) - Jof(E0, emod, sy, H0 * (1 - h))) / (2 * h * H0), tH.grad.item()),
+    ]
+    for name, f, a in checks:
+        print(f"{name}: FD {f:.10e} adjoint {a:.10e} rel {abs(f - a) / abs(a):.2e}")
+    for name, f, a in checks:
+        assert abs(f - a) <= 2e-5 * abs(a), (name, f, a)

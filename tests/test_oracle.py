@@ -629,3 +629,65 @@ def test_j2_vjp_matches_central_differences(oracle):
     checks.append(("H", fd(sh, None, 1e-2), Hb.value))
     for name, f, a in checks:
         assert abs(f - a) <= 1e-6 * max(abs(f), abs(a)) + 1e-12, (name, f, a)
+
+
+def j2_history_model(oracle, emod, sy, H):
+    from lpm_harness import P
+
+    L = P.sc_box_for(5)
+    x = oracle.create_lattice(2, 0.5, [0, L, 0, L, 0, L])
+    c = oracle.elastic_constants(E0, NU0)
+    m = oracle.Model(x, 2, 0.5, P.sc_types(x), [c] * 5, plmode=0, sigmay=sy, J2_H=H, emod=emod)
+    m.setup()
+    steps = P.sc_uniaxial_steps(L - 1.0, np.array([0.6e-3, 1.1e-3, 1.5e-3]))
+    return m, steps
+
+
+def history_solver(oracle, n):
+    return oracle.solver(rtol2=1e-26, atol2=0.0, maxit=100 * 3 * n, newton_tol=1e-12, newton_maxit=800,
+                         tangent=1, fixed_diag=1, accel=5, skip_damage=1)
+
+
+def test_history_adjoint_matches_central_differences(oracle):
+    """Discrete adjoint of a three-step J2 tension history (damage off) against
+    central differences of the whole forward history, for the per-particle modulus
+    factor and yield stress and for H.  J = sum_n (R_n - t_n)^2 over the reactions
+    of the moved face.  The adjoint solves are dense LU on the free DOFs."""
+    from lpm_harness import P
+
+    n = oracle.create_lattice(2, 0.5, [0, P.sc_box_for(5)] * 3).shape[0]
+    rng = np.random.default_rng(5)
+    e0 = 1.0 + 0.05 * rng.uniform(-1, 1, n)
+    y0 = 200.0 + 30.0 * rng.uniform(-1, 1, n)
+    H0 = 2000.0
+    s = history_solver(oracle, n)
+
+    def forward(emod, sy, H):
+        m, steps = j2_history_model(oracle, emod, sy, H)
+        rc, o = oracle.run_steps(m, steps, s)
+        assert rc == 0
+        return m, np.array([k.reaction for k in o])
+
+    m, R0 = forward(e0, y0, H0)
+    assert 0 < m.pl_flag.sum() < n
+    target = 0.9 * R0
+    m, steps = j2_history_model(oracle, e0, y0, H0)
+    rc, J, reac, ge, gy, gh = oracle.adjoint_run(m, steps, s, target)
+    assert rc == 0
+    np.testing.assert_allclose(reac, R0, rtol=1e-10)
+    assert abs(J - np.sum((R0 - target) ** 2)) <= 1e-10 * J
+
+    def Jof(*a):
+        return float(np.sum((forward(*a)[1] - target) ** 2))
+
+    d1 = rng.standard_normal(n)
+    d2 = rng.standard_normal(n)
+    checks = []
+    h = 1e-4
+    checks.append(("modulus", (Jof(e0 + h * d1, y0, H0) - Jof(e0 - h * d1, y0, H0)) / (2 * h), ge @ d1))
+    h = 1e-3
+    checks.append(("sigmay", (Jof(e0, y0 + h * d2, H0) - Jof(e0, y0 - h * d2, H0)) / (2 * h), gy @ d2))
+    h = 5.0
+    checks.append(("H", (Jof(e0, y0, H0 + h) - Jof(e0, y0, H0 - h)) / (2 * h), gh))
+    for name, f, a in checks:
+        assert abs(f - a) <= 1e-6 * max(abs(f), abs(a)), (name, f, a)
