@@ -10,9 +10,19 @@ untimed, the next `--steps K` are timed.  Both precision forms run the same step
 from a fresh model: the FP64 form (block-Jacobi PCG in FP64) and the mixed form
 (FP32 matrix values, vectors and SpMV, compensated FP32 dots, FP64 iterative
 refinement); both use the reference's CG stopping rule |r|^2 <= 1e-8 |r0|^2 + 1e-12
-and Anderson-accelerated modified Newton (depth 8) to 1e-11 relative residual, tight
+and Anderson-accelerated modified Newton (depth 16) to 1e-11 relative residual, tight
 enough that the mixed form's displacements can be compared with the FP64 form's at
 the 1e-8 level.  `value` is the mixed form's load steps/s.
+
+`--workload adjoint` measures the gradient instead: SC 60 x 60 x 60 = 216,000
+particles, the same material, 30 monotonic tension steps to 1.5% (damage off),
+J = sum_n (R_n - T_n)^2 against the reaction curve T of a perturbed parameter set
+(modulus field of seed 2, E +2%, yield +5%, H -10%); gradients with respect to E,
+the yield stress, H and the 216,000-entry modulus field by the discrete adjoint
+through the whole history with a checkpoint every 5 steps.  Its line reports the
+forward time, the time of the reverse sweep (recomputation included) and their
+ratio, and, with --adjoint-fd, central differences (h = 1e-6 relative) of J along
+E, H, the yield stress and a random direction of the modulus field.
 
 roofline: the CG SpMV kernel (k_cg_spmv), timed with HIP events on the library's
 stream during the timed steps; achieved = algorithmic bytes per launch / mean
@@ -203,6 +213,87 @@ def copy_probe():
     print(json.dumps({"copy_probe_bytes_each_way": a.numel() * 8, "launches": 5}))
 
 
+ADJ_EDGE, ADJ_BOX, ADJ_STEPS, ADJ_STRAIN = 60, 59.3, 30, 0.015
+
+
+def adjoint_workload(P, a):
+    """The history-gradient workload (see the module docstring)."""
+    os.environ.setdefault("LPMX_PROGRESS", "1")
+    E0, NU, SY0, H0 = 200e3, 0.3, 200.0, 2000.0
+    strains = np.arange(1, ADJ_STEPS + 1) * (ADJ_STRAIN / ADJ_STEPS)
+    steps = P.sc_uniaxial_steps(ADJ_EDGE - 1.0, strains)
+    n0 = ADJ_EDGE ** 3
+    prec = P.LPMX_FP64 if a.forms.split(",")[0].strip() == "fp64" else P.LPMX_MIXED
+    s = P.solver(precision=prec, rtol2=RTOL2, atol2=ATOL2, maxit=20000, newton_tol=a.newton_tol,
+                 newton_maxit=max(a.newton_maxit, 400), inner_rtol2=1e-2, ir_maxit=100, accel_depth=a.accel_depth)
+    opts = P.adjoint_opts(tol=a.adjoint_tol, maxit=400, depth=16, precision=prec, rtol2=RTOL2, cg_maxit=20000)
+    em0 = P.uniform_modulus(n0, 0)
+    sy0 = np.full(n0, SY0)
+
+    def forward(E, em, sy, H, ck):
+        P.build_sc_model(ADJ_BOX, E=E, nu=NU, sigmay=sy, H=H, emod=em)
+        t0 = time.perf_counter()
+        rc, R, out = P.history_forward(steps, s, ckpt_every=ck)
+        P.check(rc, "history forward")
+        return R, out, time.perf_counter() - t0
+
+    log("adjoint: target curve (perturbed parameters, seed 2)")
+    T, _, _ = forward(E0 * 1.02, P.uniform_modulus(n0, 2), sy0 * 1.05, H0 * 0.9, ADJ_STEPS)
+    log("adjoint: forward with checkpoints")
+    R, out, t_fwd = forward(E0, em0, sy0, H0, a.ckpt_every)
+    J = float(np.sum((R - T) ** 2))
+    t0 = time.perf_counter()
+    rc, ge, gy, gh, st = P.history_backward(opts, 2.0 * (R - T))
+    P.check(rc, "history backward")
+    t_bwd = time.perf_counter() - t0
+    gE = float(em0 @ ge) / E0
+    line = {
+        "metric": "adjoint gradient cost / forward cost, 216k-particle SC J2 history (30 steps)",
+        "value": (t_fwd + t_bwd) / t_fwd,
+        "unit": "x forward",
+        "n_gpus": 1, "steps": ADJ_STEPS, "warmup": 0,
+        "ms_per_step": 1e3 * (t_fwd + t_bwd) / ADJ_STEPS,
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if prec == P.LPMX_FP64 else "f32+f64", "data": "synthetic",
+        "config": {"workload": f"SC lattice {ADJ_EDGE}^3 = {n0} particles, J2, uniaxial tension to "
+                               f"{100 * ADJ_STRAIN:.1f}% in {ADJ_STEPS} steps, damage off; J = sum (R - T)^2, "
+                               f"T from modulus seed 2, E +2%, yield +5%, H -10%",
+                   "newton_tol": a.newton_tol, "newton_accel_depth": a.accel_depth, "ckpt_every": a.ckpt_every,
+                   "adjoint_tol": a.adjoint_tol, "adjoint_depth": 16, "linear_rtol2": RTOL2},
+        "t_forward_s": t_fwd, "t_backward_s": st.t_backward, "t_recompute_s": st.t_recompute,
+        "t_gradient_s": t_fwd + t_bwd,
+        "forward_newton_iters": int(sum(o.newton_iters for o in out)),
+        "forward_cg_iters": int(sum(o.cg_iters for o in out)),
+        "adjoint_iters": st.adj_iters, "adjoint_iters_max": st.adj_iters_max, "adjoint_cg_iters": st.adj_cg_iters,
+        "adjoint_max_rel_res": st.max_rel_res, "checkpoints": st.checkpoints,
+        "recomputed_steps": st.recomputed_steps, "history_bytes": st.ckpt_bytes,
+        "J": J, "dJ_dE": gE, "dJ_dsigmay_total": float(gy.sum()), "dJ_dH": gh,
+        "dJ_demod_norm": float(np.linalg.norm(ge)),
+        "plastic_particles": int((P.field("PL_FLAG", n0) > 0).sum()),
+    }
+    if a.adjoint_fd:
+        h = 1e-6
+        rng = np.random.default_rng(3)
+        d1 = rng.standard_normal(n0)
+
+        def Jof(E, em, sy, H):
+            return float(np.sum((forward(E, em, sy, H, ADJ_STEPS)[0] - T) ** 2))
+
+        fd = {}
+        for name, plus, minus, adj in (
+                ("E", (E0 * (1 + h), em0, sy0, H0), (E0 * (1 - h), em0, sy0, H0), gE * E0),
+                ("H", (E0, em0, sy0, H0 * (1 + h)), (E0, em0, sy0, H0 * (1 - h)), gh * H0),
+                ("sigmay", (E0, em0, sy0 * (1 + h), H0), (E0, em0, sy0 * (1 - h), H0), float(gy @ sy0)),
+                ("emod_dir", (E0, em0 * (1 + h * d1), sy0, H0), (E0, em0 * (1 - h * d1), sy0, H0),
+                 float(ge @ (em0 * d1)))):
+            log(f"adjoint: central difference along {name}")
+            f = (Jof(*plus) - Jof(*minus)) / (2 * h)
+            fd[name] = {"fd": f, "adjoint": adj, "rel_err": abs(f - adj) / max(abs(adj), 1e-300)}
+            log(f"adjoint: {name}: FD {f:.10e} adjoint {adj:.10e}")
+        line["fd_check_h_rel_1e-6"] = fd
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -214,9 +305,14 @@ def main():
     ap.add_argument("--copy-probe", action="store_true")
     ap.add_argument("--newton-tol", type=float, default=1e-11,
                     help="Newton test |R| <= tol * max(|R0|, |reaction|) (reference TOLITER semantics)")
-    ap.add_argument("--newton-maxit", type=int, default=100, help="Newton cap (reference MAXITER = 100)")
-    ap.add_argument("--accel-depth", type=int, default=8,
+    ap.add_argument("--newton-maxit", type=int, default=100,
+                    help="Newton cap (reference MAXITER = 100; the adjoint workload uses at least 400)")
+    ap.add_argument("--accel-depth", type=int, default=16,
                     help="Anderson acceleration depth of the Newton loop (0: modified Newton)")
+    ap.add_argument("--workload", default="loadsteps", choices=["loadsteps", "adjoint"])
+    ap.add_argument("--ckpt-every", type=int, default=5, help="adjoint workload: checkpoint spacing")
+    ap.add_argument("--adjoint-tol", type=float, default=1e-8, help="adjoint workload: per-step solve tolerance")
+    ap.add_argument("--adjoint-fd", action="store_true", help="adjoint workload: central-difference checks")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {a.gpus}): the load-step loop here is a single-GPU "
@@ -224,6 +320,9 @@ def main():
         sys.exit(2)
     if a.copy_probe:
         copy_probe()
+        return
+    if a.workload == "adjoint":
+        adjoint_workload(product(), a)
         return
     if a.warmup + a.steps > NSTEPS_SCHEDULE or a.steps < 1 or a.warmup < 0:
         sys.exit(f"--warmup + --steps must lie in 1..{NSTEPS_SCHEDULE}")

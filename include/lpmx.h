@@ -201,7 +201,7 @@ typedef struct lpmx_solver
     double inner_rtol2;     /* mixed form: squared reduction of an FP32 segment between
                              * FP64 residual replacements (1e-2) */
     int ir_maxit;           /* mixed form: cap on residual replacements per solve */
-    int accel_depth;        /* 0: the reference's modified Newton; 1..8: Anderson
+    int accel_depth;        /* 0: the reference's modified Newton; 1..16: Anderson
                              * acceleration of it with that many differences (same
                              * fixed points, far fewer iterations in plastic flow) */
 } lpmx_solver;
@@ -237,7 +237,7 @@ typedef struct lpmx_adjoint_opts
                              * adjoint solve (J: consistent Jacobian of the free DOFs) */
     int maxit;              /* cap on the adjoint iterations of one step */
     int depth;              /* Anderson depth of the K-preconditioned fixed-point
-                             * iteration lambda <- lambda + K^-1 (b - J^T lambda), 0..8 */
+                             * iteration lambda <- lambda + K^-1 (b - J^T lambda), 0..16 */
     int precision;          /* LPMX_FP64 or LPMX_MIXED for the K^-1 solves */
     double rtol2;           /* CG test of the K^-1 solves (squared, relative) */
     int cg_maxit;           /* their cap */

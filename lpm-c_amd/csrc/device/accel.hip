@@ -24,11 +24,16 @@
 
 #include "common.h"
 
-#define AA_MAX 8
+#define AA_MAX 16
 
 struct AaPtrs
 {
     double *dU[AA_MAX], *dF[AA_MAX];
+};
+
+struct AaGamma
+{
+    double g[AA_MAX];
 };
 
 // One Anderson history over DOF vectors of n*dim entries.  The iterate itself may
@@ -132,18 +137,16 @@ __global__ __launch_bounds__(256) void k_aa_dots_total(const double *parts, int 
 }
 
 // x -= sum_j gamma_j (dU_j + dF_j)
-__global__ void k_aa_mix(int n, int dim, int xs, double *__restrict__ x, AaPtrs P, int cnt, double g0, double g1,
-                         double g2, double g3, double g4, double g5, double g6, double g7)
+__global__ void k_aa_mix(int n, int dim, int xs, double *__restrict__ x, AaPtrs P, int cnt, AaGamma gam)
 {
     const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (t >= (int64_t)n * dim)
         return;
-    const double gam[AA_MAX] = {g0, g1, g2, g3, g4, g5, g6, g7};
     double s = 0.0;
 #pragma unroll
     for (int j = 0; j < AA_MAX; j++)
         if (j < cnt)
-            s += gam[j] * (P.dU[j][t] + P.dF[j][t]);
+            s += gam.g[j] * (P.dU[j][t] + P.dF[j][t]);
     const int64_t i = t / dim;
     const int c = (int)(t - i * dim);
     x[xs * i + c] -= s;
@@ -279,15 +282,16 @@ static int aa_step(AaHist *a, double *x, int xs, const double *d, int *mixed)
     HIPCHK(hipStreamSynchronize(g_stream));
     for (int j = 0; j < cnt; j++)
         a->G[s][j] = a->G[j][s] = o[j];
-    double gam[AA_MAX] = {0, 0, 0, 0, 0, 0, 0, 0};
-    if (!small_spd_solve(cnt, a->G, o + AA_MAX, gam))
+    AaGamma gam;
+    for (int j = 0; j < AA_MAX; j++)
+        gam.g[j] = 0.0;
+    if (!small_spd_solve(cnt, a->G, o + AA_MAX, gam.g))
     {
         // degenerate history: keep the plain iterate, start the differences afresh
         aa_forget(a);
         return LPMX_OK;
     }
-    hipLaunchKernelGGL(k_aa_mix, dim3(nb), dim3(LPMX_BLOCK), 0, g_stream, n, dim, xs, x, a->P, cnt, gam[0], gam[1],
-                       gam[2], gam[3], gam[4], gam[5], gam[6], gam[7]);
+    hipLaunchKernelGGL(k_aa_mix, dim3(nb), dim3(LPMX_BLOCK), 0, g_stream, n, dim, xs, x, a->P, cnt, gam);
     LAUNCHCHK();
     *mixed = 1;
     return LPMX_OK;

@@ -27,6 +27,8 @@
 // (bitwise the same trajectory: every reduction has a fixed order).
 #include <chrono>
 #include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 #include <vector>
 
@@ -35,6 +37,16 @@
 static double now_s(void)
 {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// A line per step on stderr when the environment variable LPMX_PROGRESS is set
+// (long histories otherwise run silently inside one call).
+static bool progress(void)
+{
+    static int on = -1;
+    if (on < 0)
+        on = getenv("LPMX_PROGRESS") != nullptr;
+    return on == 1;
 }
 
 // State before a step (a checkpoint) or after the last one.
@@ -244,6 +256,9 @@ extern "C" int lpmx_history_forward(const lpmx_solver *s, int ckpt_every, int ns
         }
         lpmx_step_stats o;
         RETCHK(forward_step(st, &H.rec[st - H.rec_first], &o));
+        if (progress())
+            fprintf(stderr, "[lpmx] history step %d/%d: Newton %d, CG %d, %.2f s, reaction %.6e\n", st + 1, nsteps,
+                    o.newton_iters, o.cg_iters, o.t_total, o.reaction);
         H.rec_count++;
         H.reac[st] = o.reaction;
         reactions[st] = o.reaction;
@@ -441,7 +456,7 @@ extern "C" int lpmx_history_backward(const lpmx_adjoint_opts *o, const double *R
     RETCHK(lpmx_require_pattern());
     if (!H.valid || H.n != g.n || H.nb != g.nb)
         return lpmx_fail(LPMX_ERR_STATE, "no recorded history for this model (lpmx_history_forward first)");
-    if (!o || !Rbar || !g_emod || !g_sigmay || !g_H || o->depth < 0 || o->depth > 8 || o->maxit < 0 ||
+    if (!o || !Rbar || !g_emod || !g_sigmay || !g_H || o->depth < 0 || o->depth > 16 || o->maxit < 0 ||
         !(o->tol > 0.0))
         return lpmx_fail(LPMX_ERR_ARG, "invalid arguments to lpmx_history_backward");
     lpmx_adjoint_stats S;
@@ -492,7 +507,11 @@ extern "C" int lpmx_history_backward(const lpmx_adjoint_opts *o, const double *R
                 const double tr = now_s();
                 RETCHK(load_snap(H.ck[seg]));
                 for (int s2 = a; s2 < e; s2++)
+                {
                     RETCHK(forward_step(s2, &H.rec[s2 - a], nullptr));
+                    if (progress())
+                        fprintf(stderr, "[lpmx] recomputed step %d\n", s2 + 1);
+                }
                 H.rec_first = a;
                 H.rec_count = e - a;
                 S.recomputed_steps += e - a;
@@ -501,9 +520,14 @@ extern "C" int lpmx_history_backward(const lpmx_adjoint_opts *o, const double *R
             }
             for (int s2 = e - 1; s2 >= a; s2--)
             {
+                const double ts = now_s();
+                const int i0 = S.adj_iters, c0 = S.adj_cg_iters;
                 RETCHK(restore_rec(H.rec[s2 - a]));
                 RETCHK(reverse_step(R, cur, o, Rbar[s2], &S));
                 cur = 1 - cur;
+                if (progress())
+                    fprintf(stderr, "[lpmx] adjoint step %d: %d iterations, CG %d, %.2f s\n", s2 + 1,
+                            S.adj_iters - i0, S.adj_cg_iters - c0, now_s() - ts);
             }
         }
         HIPCHK(hipMemcpyAsync(g_emod, R.ge, sizeof(double) * n, hipMemcpyDeviceToHost, g_stream));
