@@ -22,7 +22,8 @@ the yield stress, H and the 216,000-entry modulus field by the discrete adjoint
 through the whole history with a checkpoint every 5 steps.  Its line reports the
 forward time, the time of the reverse sweep (recomputation included) and their
 ratio, and, with --adjoint-fd, central differences (h = 1e-6 relative) of J along
-E, H, the yield stress and a random direction of the modulus field.
+any of E, H, the yield stress and a random direction of the modulus field (each the
+logarithmic derivative d J / d log theta, compared with the adjoint's).
 
 roofline: the CG SpMV kernel (k_cg_spmv), timed with HIP events on the library's
 stream during the timed steps; achieved = algorithmic bytes per launch / mean
@@ -237,8 +238,17 @@ def adjoint_workload(P, a):
         P.check(rc, "history forward")
         return R, out, time.perf_counter() - t0
 
-    log("adjoint: target curve (perturbed parameters, seed 2)")
-    T, _, _ = forward(E0 * 1.02, P.uniform_modulus(n0, 2), sy0 * 1.05, H0 * 0.9, ADJ_STEPS)
+    if a.adjoint_target:
+        T = np.array([float(v) for v in a.adjoint_target.split(",")])
+        if T.size != ADJ_STEPS:
+            sys.exit(f"--adjoint-target needs {ADJ_STEPS} values")
+    else:
+        log("adjoint: target curve (perturbed parameters, seed 2)")
+        T, _, tT = forward(E0 * 1.02, P.uniform_modulus(n0, 2), sy0 * 1.05, H0 * 0.9, ADJ_STEPS)
+        log(f"adjoint: target curve in {tT:.1f} s: " + ",".join(f"{v:.17g}" for v in T))
+        if a.adjoint_target_only:
+            print(json.dumps({"adjoint_target": list(map(float, T)), "t_forward_s": tT}), flush=True)
+            return
     log("adjoint: forward with checkpoints")
     R, out, t_fwd = forward(E0, em0, sy0, H0, a.ckpt_every)
     J = float(np.sum((R - T) ** 2))
@@ -280,12 +290,15 @@ def adjoint_workload(P, a):
             return float(np.sum((forward(E, em, sy, H, ADJ_STEPS)[0] - T) ** 2))
 
         fd = {}
+        want = [w.strip() for w in a.adjoint_fd.split(",") if w.strip()]
         for name, plus, minus, adj in (
                 ("E", (E0 * (1 + h), em0, sy0, H0), (E0 * (1 - h), em0, sy0, H0), gE * E0),
                 ("H", (E0, em0, sy0, H0 * (1 + h)), (E0, em0, sy0, H0 * (1 - h)), gh * H0),
                 ("sigmay", (E0, em0, sy0 * (1 + h), H0), (E0, em0, sy0 * (1 - h), H0), float(gy @ sy0)),
                 ("emod_dir", (E0, em0 * (1 + h * d1), sy0, H0), (E0, em0 * (1 - h * d1), sy0, H0),
                  float(ge @ (em0 * d1)))):
+            if name not in want:
+                continue
             log(f"adjoint: central difference along {name}")
             f = (Jof(*plus) - Jof(*minus)) / (2 * h)
             fd[name] = {"fd": f, "adjoint": adj, "rel_err": abs(f - adj) / max(abs(adj), 1e-300)}
@@ -312,7 +325,14 @@ def main():
     ap.add_argument("--workload", default="loadsteps", choices=["loadsteps", "adjoint"])
     ap.add_argument("--ckpt-every", type=int, default=5, help="adjoint workload: checkpoint spacing")
     ap.add_argument("--adjoint-tol", type=float, default=1e-8, help="adjoint workload: per-step solve tolerance")
-    ap.add_argument("--adjoint-fd", action="store_true", help="adjoint workload: central-difference checks")
+    ap.add_argument("--adjoint-fd", default="",
+                    help="adjoint workload: central-difference checks along these of E,H,sigmay,emod_dir "
+                         "(comma-separated; two forward histories each)")
+    ap.add_argument("--adjoint-target", default="",
+                    help="adjoint workload: the target reaction curve (comma-separated, one value per step) "
+                         "as printed by --adjoint-target-only, instead of computing it in the same run")
+    ap.add_argument("--adjoint-target-only", action="store_true",
+                    help="adjoint workload: compute and print the target curve only")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {a.gpus}): the load-step loop here is a single-GPU "

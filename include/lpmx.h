@@ -214,6 +214,7 @@ typedef struct lpmx_step_stats
     double t_tangent, t_solve, t_force, t_other, t_total; /* seconds */
     int accel_mixed;        /* Newton iterations that took the accelerated iterate */
     int accel_reverts;      /* accelerated iterates rejected for the plain one */
+    int accel_restarts;     /* histories restarted because the residual stagnated */
 } lpmx_step_stats;
 
 int lpmx_run_steps(const lpmx_solver *s, int nsteps, const int *nbd, const lpmx_dispbc *dbc,
@@ -226,7 +227,9 @@ int lpmx_run_steps(const lpmx_solver *s, int nsteps, const int *nbd, const lpmx_
  * the step is kept (positions, bond forces, Pin, Pex, the converged slot state,
  * constraint masks); the steps of the last segment are kept whole.  ckpt_every = 0
  * keeps every step (no recomputation).  reactions[nsteps] gets writeReaction of
- * (out_axis, out_type) after each step, which is what the reverse sweep differentiates. */
+ * (out_axis, out_type) after each step, which is what the reverse sweep differentiates.
+ * Fails (LPMX_ERR_NOCONV) when a Newton loop hits its cap; a linear solve that hits
+ * its own cap inside a Newton loop that then converges is only reported in out[].status. */
 int lpmx_history_forward(const lpmx_solver *s, int ckpt_every, int nsteps, const int *nbd,
                          const lpmx_dispbc *dbc, const int *nbf, const lpmx_forcebc *fbc, int maxbc,
                          char out_axis, int out_type, double *reactions, lpmx_step_stats *out);

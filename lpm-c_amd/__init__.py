@@ -67,7 +67,7 @@ class StepStats(C.Structure):
                 ("resolves", C.c_int), ("broken", C.c_int), ("ir_passes", C.c_int), ("status", C.c_int),
                 ("res_norm", C.c_double), ("tol_mult", C.c_double), ("t_tangent", C.c_double),
                 ("t_solve", C.c_double), ("t_force", C.c_double), ("t_other", C.c_double), ("t_total", C.c_double),
-                ("accel_mixed", C.c_int), ("accel_reverts", C.c_int)]
+                ("accel_mixed", C.c_int), ("accel_reverts", C.c_int), ("accel_restarts", C.c_int)]
 
 
 class AdjointOpts(C.Structure):

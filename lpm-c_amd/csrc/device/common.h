@@ -200,7 +200,9 @@ int lpmx_aa_step(AaHist *a, double *x, const double *d, int *mixed);
 // K x = b on the current constraint-eliminated tangent (solver.hip)
 int lpmx_solve_into(int precision, double rtol2, double atol2, int maxit, double inner_rtol2, int ir_maxit,
                     const double *b, double *x, int *iters, double *rr, int *status);
-// the load-step loop with or without the damage update after each step (loop.hip)
+// the load-step loop with or without the damage update after each step; newton_only:
+// only a Newton loop that hit its cap fails the call, not a linear solve that did
+// (loop.hip)
 int lpmx_run_steps_impl(const lpmx_solver *s, int nsteps, const int *nbd, const lpmx_dispbc *dbc, const int *nbf,
                         const lpmx_forcebc *fbc, int maxbc, char out_axis, int out_type, lpmx_step_stats *out,
-                        int skip_damage);
+                        int skip_damage, int newton_only);

@@ -200,7 +200,7 @@ static int forward_step(int st, Rec *r, lpmx_step_stats *o)
     }
     lpmx_step_stats so;
     RETCHK(lpmx_run_steps_impl(&H.s, 1, &H.nbd[st], &H.dbc[(size_t)st * H.maxbc], &H.nbf[st],
-                               &H.fbc[(size_t)st * H.maxbc], H.maxbc, H.axis, H.t1, &so, 1));
+                               &H.fbc[(size_t)st * H.maxbc], H.maxbc, H.axis, H.t1, &so, 1, 1));
     if (r)
     {
         RETCHK(dcopy(r->x, g.x, 3 * n));
@@ -257,8 +257,9 @@ extern "C" int lpmx_history_forward(const lpmx_solver *s, int ckpt_every, int ns
         lpmx_step_stats o;
         RETCHK(forward_step(st, &H.rec[st - H.rec_first], &o));
         if (progress())
-            fprintf(stderr, "[lpmx] history step %d/%d: Newton %d, CG %d, %.2f s, reaction %.6e\n", st + 1, nsteps,
-                    o.newton_iters, o.cg_iters, o.t_total, o.reaction);
+            fprintf(stderr, "[lpmx] history step %d/%d: Newton %d (restarts %d), CG %d, %.2f s, status %d, "
+                    "reaction %.6e\n", st + 1, nsteps, o.newton_iters, o.accel_restarts, o.cg_iters, o.t_total,
+                    o.status, o.reaction);
         H.rec_count++;
         H.reac[st] = o.reaction;
         reactions[st] = o.reaction;

@@ -3,7 +3,11 @@
 // decisions: two FP64 norms per Newton iteration come back, nothing else.
 // With accel_depth > 0 the modified-Newton iterates are Anderson-accelerated
 // (accel.hip); an accelerated iterate whose residual norm is not finite or grew
-// more than 100-fold is replaced by the plain iterate and the history restarts.
+// more than 100-fold is replaced by the plain iterate and the history restarts, and
+// the history also restarts when the residual has not halved over the last `depth`
+// iterations (differences taken before particles switched between elastic and
+// plastic mislead the mixing; on the hard steps of long plastic histories this
+// halves the iteration count).
 #include <chrono>
 #include <math.h>
 #include <string.h>
@@ -17,6 +21,8 @@ static double now_s(void)
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
+#define RING 64 // residual norms of the last iterations (> the largest Anderson depth)
+
 static int sync(void)
 {
     HIPCHK(hipStreamSynchronize(g_stream));
@@ -25,7 +31,7 @@ static int sync(void)
 
 int lpmx_run_steps_impl(const lpmx_solver *s, int nsteps, const int *nbd, const lpmx_dispbc *dbc, const int *nbf,
                         const lpmx_forcebc *fbc, int maxbc, char out_axis, int out_type, lpmx_step_stats *out,
-                        int skip_damage)
+                        int skip_damage, int newton_only)
 {
     RETCHK(lpmx_require_pattern());
     if (!s || nsteps < 0 || (nsteps > 0 && (!nbd || !nbf)) || maxbc <= 0)
@@ -58,6 +64,9 @@ int lpmx_run_steps_impl(const lpmx_solver *s, int nsteps, const int *nbd, const 
             int ni = 0;
             RETCHK(lpmx_accel_begin(s->accel_depth));
             double nprev = nres;
+            double ring[RING];
+            int since = 0;
+            ring[0] = nres;
             while (nres > s->newton_tol * tolm && ni < s->newton_maxit)
             {
                 RETCHK(lpmx_switch_state(0));
@@ -93,6 +102,15 @@ int lpmx_run_steps_impl(const lpmx_solver *s, int nsteps, const int *nbd, const 
                 o.t_force += now_s() - tf;
                 nprev = nres;
                 ni++;
+                since++;
+                if (s->accel_depth > 0 && since > s->accel_depth &&
+                    nres > 0.5 * ring[(since - s->accel_depth) % RING])
+                {
+                    lpmx_accel_forget();
+                    o.accel_restarts++;
+                    since = 0;
+                }
+                ring[since % RING] = nres;
             }
             if (ni >= s->newton_maxit)
                 o.status |= 1;
@@ -128,7 +146,7 @@ int lpmx_run_steps_impl(const lpmx_solver *s, int nsteps, const int *nbd, const 
         RETCHK(sync());
         o.t_total = now_s() - t0;
         o.t_other = o.t_total - o.t_tangent - o.t_solve - o.t_force;
-        if (o.status)
+        if (newton_only ? (o.status & 1) : o.status)
             rc_all = LPMX_ERR_NOCONV;
         if (out)
             out[st] = o;
@@ -142,5 +160,5 @@ extern "C" int lpmx_run_steps(const lpmx_solver *s, int nsteps, const int *nbd, 
                               const int *nbf, const lpmx_forcebc *fbc, int maxbc, char out_axis, int out_type,
                               lpmx_step_stats *out)
 {
-    return lpmx_run_steps_impl(s, nsteps, nbd, dbc, nbf, fbc, maxbc, out_axis, out_type, out, 0);
+    return lpmx_run_steps_impl(s, nsteps, nbd, dbc, nbf, fbc, maxbc, out_axis, out_type, out, 0, 0);
 }

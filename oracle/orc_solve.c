@@ -475,7 +475,10 @@ static int damage_general(orc_model *m)
 /* Anderson mixing of the modified-Newton iterates u <- u + d(u) (Walker & Ni 2011,
  * type II): the last `depth` differences of iterates and corrections, the
  * least-squares weights from the normal equations (regularised by 1e-12 of their
- * largest diagonal), u <- u + d - sum_j g_j (dU_j + dF_j). */
+ * largest diagonal), u <- u + d - sum_j g_j (dU_j + dF_j).  An accelerated iterate
+ * whose residual grew more than 100-fold is replaced by the plain one; when the
+ * residual has not halved over the last `depth` iterations (a stale history, as
+ * particles switch between elastic and plastic) the differences start afresh. */
 typedef struct
 {
     int depth, cnt, next, have, nd;
@@ -555,6 +558,8 @@ static int aa_step(orc_aa *a, orc_model *m)
     return 1;
 }
 
+#define AA_RING 64
+
 int orc_run_steps(orc_model *m, const orc_solver *s, int nsteps,
                   const int *nbd, const orc_dispbc *dbc, const int *nbf, const orc_forcebc *fbc,
                   int maxbc, char out_axis, int out_type, orc_stepout *out)
@@ -587,6 +592,9 @@ int orc_run_steps(orc_model *m, const orc_solver *s, int nsteps,
             double nd = -1.0;
             aa_begin(&aa, s->accel > 8 ? 8 : s->accel, d * n);
             double nprev = nres;
+            double ring[AA_RING];
+            int since = 0;
+            ring[0] = nres;
             while (nres > s->newton_tol * tolm && ni < s->newton_maxit)
             {
                 orc_switch_state(m, 0);
@@ -616,6 +624,15 @@ int orc_run_steps(orc_model *m, const orc_solver *s, int nsteps,
                     nres = orc_nrm2(d * n, m->residual);
                 }
                 nprev = nres;
+                /* stagnation: less than a halving over the last `depth` iterations
+                 * starts the differences afresh */
+                since++;
+                if (aa.depth > 0 && since > aa.depth && nres > 0.5 * ring[(since - aa.depth) % AA_RING])
+                {
+                    aa.cnt = aa.next = 0;
+                    since = 0;
+                }
+                ring[since % AA_RING] = nres;
             }
             if (ni >= s->newton_maxit && !first_fail)
                 first_fail = st + 1;
