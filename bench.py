@@ -6,13 +6,17 @@ per-particle modulus 1 + 0.05 U(-1, 1) (splitmix64, seed 0), uniaxial tension to
 in 50 load steps (bottom face held in z, 3-2-1 corner constraints, top face moved).
 
 One bench step is one load step of that schedule: `--warmup W` load steps run
-untimed, the next `--steps K` are timed.  Both precision forms run the same steps
-from a fresh model: the FP64 form (block-Jacobi PCG in FP64) and the mixed form
-(FP32 matrix values, vectors and SpMV, compensated FP32 dots, FP64 iterative
-refinement); both use the reference's CG stopping rule |r|^2 <= 1e-8 |r0|^2 + 1e-12
-and Anderson-accelerated modified Newton (depth 16) to 1e-11 relative residual, tight
-enough that the mixed form's displacements can be compared with the FP64 form's at
-the 1e-8 level.  `value` is the mixed form's load steps/s.
+untimed, the next `--steps K` are timed (defaults: steps 1-2 untimed, 3-7 timed; step
+3 is the first plastic one).  Both precision forms run the same steps from a fresh
+model: the FP64 form (PCG in FP64) and the mixed form (FP32 matrix values, vectors
+and SpMV, compensated FP32 dots, FP64 iterative refinement); both use the
+reference's CG stopping rule |r|^2 <= 1e-8 |r0|^2 + 1e-12 and Anderson-accelerated
+modified Newton (depth 16) to 1e-11 relative residual, tight enough that the mixed
+form's displacements can be compared with the FP64 form's at the 1e-8 level.  The CG
+is preconditioned by a smoothed-aggregation AMG V-cycle (`--precond bj`: 3x3
+block-Jacobi); its hierarchy is built once per model and constraint set, on the first
+solve of the untimed steps, and its set-up time is reported.  `value` is the mixed
+form's load steps/s.
 
 `--workload adjoint` measures the gradient instead: SC 60 x 60 x 60 = 216,000
 particles, the same material, 30 monotonic tension steps to 1.5% (damage off),
@@ -25,9 +29,11 @@ ratio, and, with --adjoint-fd, central differences (h = 1e-6 relative) of J alon
 any of E, H, the yield stress and a random direction of the modulus field (each the
 logarithmic derivative d J / d log theta, compared with the adjoint's).
 
-roofline: the CG SpMV kernel (k_cg_spmv), timed with HIP events on the library's
-stream during the timed steps; achieved = algorithmic bytes per launch / mean
-launch time.  cpu_baseline: the oracle's threaded CG (reference solverCG algorithm)
+roofline: the dominant kernel, timed with HIP events on the library's stream during
+the timed steps; achieved = algorithmic bytes per launch / mean launch time.  With
+AMG that is the finest level's smoother SpMV (k_slice_mv, two launches per CG
+iteration; the CG SpMV k_cg_spmv is reported beside it), with block-Jacobi the CG
+SpMV.  cpu_baseline: the oracle's threaded CG (reference solverCG algorithm)
 on the same 1M matrix, a bounded number of iterations on the host cores, in
 DOF-iters/s.
 """
@@ -51,6 +57,11 @@ EDGE = 100
 BOX = 99.195  # SC box edge giving 100 particles per edge (radius 0.5)
 # linear solves: the reference's CG stopping rule |r|^2 <= 1e-8 |r0|^2 + 1e-12
 RTOL2, ATOL2 = 1e-8, 1e-12
+PRECOND_TEXT = {
+    "amg": "smoothed-aggregation AMG V-cycle (rigid-body modes, degree-1 Chebyshev block-Jacobi smoothing; "
+           "hierarchy built once per model and constraint set, before the timed steps)",
+    "bj": "block-Jacobi 3x3",
+}
 
 
 def log(msg):
@@ -68,15 +79,19 @@ def schedule(P):
     return P.sc_uniaxial_steps(EDGE - 1.0, strains)
 
 
-def run_form(P, prec, warmup, steps, profile, accel_depth, newton_tol, newton_maxit):
+def run_form(P, prec, warmup, steps, profile, accel_depth, newton_tol, newton_maxit, precond):
     L = P.lib()
     t0 = time.time()
     emod = P.uniform_modulus(EDGE ** 3, 0)
     n, x, _, _ = P.build_sc_model(BOX, emod=emod)
     log(f"form {prec}: model of {n} particles built in {time.time() - t0:.1f} s")
     sched = schedule(P)
+    amg = precond == "amg"
+    # mixed form: with block-Jacobi the FP32 residual is replaced by the FP64 one after
+    # every 10x drop (long solves accumulate FP32 drift); an AMG solve is a few tens of
+    # iterations, so its FP32 CG runs to the outer threshold in one segment
     s = P.solver(precision=prec, rtol2=RTOL2, atol2=ATOL2, maxit=20000, newton_tol=newton_tol, newton_maxit=newton_maxit,
-                 inner_rtol2=1e-2, ir_maxit=100, accel_depth=accel_depth)
+                 inner_rtol2=RTOL2 if amg else 1e-2, ir_maxit=100, accel_depth=accel_depth, precond=1 if amg else 0)
     if warmup:
         rc, wout = P.run_steps(sched[:warmup], s)
         P.check(rc, "warmup steps")
@@ -95,19 +110,28 @@ def run_form(P, prec, warmup, steps, profile, accel_depth, newton_tol, newton_ma
             f"reverted {o[0].accel_reverts}, residual {o[0].res_norm / max(o[0].tol_mult, 1e-300):.2e}")
     sync()
     wall = time.perf_counter() - t1
-    prof = None
+    prof = prof_sm = None
     if profile:
         ms, cnt = C.c_double(), C.c_int64()
         L.lpmx_profile_spmv_read(C.byref(ms), C.byref(cnt))
-        L.lpmx_profile_spmv(0)
         by = C.c_double()
         P.check(L.lpmx_spmv_bytes(prec, C.byref(by)), "spmv bytes")
         prof = (ms.value, cnt.value, by.value)
+        if amg:
+            L.lpmx_profile_amg_read(C.byref(ms), C.byref(cnt))
+            P.check(L.lpmx_amg_smoother_bytes(prec, C.byref(by)), "smoother bytes")
+            prof_sm = (ms.value, cnt.value, by.value)
+        L.lpmx_profile_spmv(0)
+    amg_info = None
+    if amg:
+        lev, t_setup, dev = C.c_int(), C.c_double(), C.c_int64()
+        P.check(L.lpmx_amg_info(C.byref(lev), C.byref(t_setup), C.byref(dev)), "amg info")
+        amg_info = {"levels": lev.value, "setup_s": t_setup.value, "device_bytes": dev.value}
     if rc != 0:
         log(f"form {prec}: {L.lpmx_last_error().decode()} "
             f"{[(o.status, o.newton_iters, o.cg_iters) for o in out]}")
     u = P.field("XYZ", 3 * n).reshape(n, 3) - x
-    return dict(n=n, wall=wall, out=out, rc=rc, prof=prof, u=u)
+    return dict(n=n, wall=wall, out=out, rc=rc, prof=prof, prof_sm=prof_sm, amg=amg_info, u=u)
 
 
 def sync():
@@ -140,14 +164,14 @@ def summarize(r, n):
     }
 
 
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_cg_spmv_sc1m_hbm_traffic.md")
+TRAFFIC_FILES = {"k_cg_spmv": "k_cg_spmv_sc1m_hbm_traffic.md", "k_slice_mv": "k_slice_mv_sc1m_hbm_traffic.md"}
 
 
-def measured_traffic(form):
-    """HBM bytes per k_cg_spmv launch from the committed PMC summary (FETCH_SIZE and
-    WRITE_SIZE, scaled by a copy of known size), or None."""
+def measured_traffic(kernel, form):
+    """HBM bytes per launch of the kernel from its committed PMC summary (FETCH_SIZE
+    and WRITE_SIZE, scaled by a copy of known size), or None."""
     try:
-        for line in open(TRAFFIC_FILE):
+        for line in open(os.path.join(ROOT, "profiles", TRAFFIC_FILES[kernel])):
             f = line.split()
             if len(f) == 3 and f[0] == "traffic_bytes" and f[1] == form:
                 return float(f[2])
@@ -156,16 +180,24 @@ def measured_traffic(form):
     return None
 
 
-def roofline(prof, form):
+def roofline(prof, form, kernel="k_cg_spmv"):
     ms, cnt, by = prof
     if cnt <= 0:
         return None
     avg = ms / cnt
     ach = by / (avg * 1e-3) / 1e9
     return {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
-            "traffic": measured_traffic(form), "kernel": "k_cg_spmv", "algorithmic_bytes": by,
+            "traffic": measured_traffic(kernel, form), "kernel": kernel, "algorithmic_bytes": by,
             "avg_launch_ms": avg, "launches_timed": cnt,
-            "traffic_source": "profiles/k_cg_spmv_sc1m_hbm_traffic.md (bytes per launch)"}
+            "traffic_source": f"profiles/{TRAFFIC_FILES[kernel]} (bytes per launch)"}
+
+
+def form_roofline(r, form):
+    """The dominant kernel of the form: with AMG the finest level's smoother SpMV
+    (k_slice_mv, two launches per CG iteration), else the CG SpMV (k_cg_spmv)."""
+    if r.get("prof_sm"):
+        return roofline(r["prof_sm"], form, "k_slice_mv")
+    return roofline(r["prof"], form) if r["prof"] else None
 
 
 def cpu_baseline(P, budget_s):
@@ -315,7 +347,7 @@ def adjoint_workload(P, a):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -327,6 +359,8 @@ def main():
                     help="Newton cap (reference MAXITER = 100; the adjoint workload uses at least 400)")
     ap.add_argument("--accel-depth", type=int, default=16,
                     help="Anderson acceleration depth of the Newton loop (0: modified Newton)")
+    ap.add_argument("--precond", default="amg", choices=["amg", "bj"],
+                    help="CG preconditioner: smoothed-aggregation AMG or block-Jacobi (3x3 blocks)")
     ap.add_argument("--workload", default="loadsteps", choices=["loadsteps", "adjoint"])
     ap.add_argument("--ckpt-every", type=int, default=5, help="adjoint workload: checkpoint spacing")
     ap.add_argument("--adjoint-tol", type=float, default=1e-8, help="adjoint workload: per-step solve tolerance")
@@ -357,7 +391,7 @@ def main():
     for f in forms:
         prec = P.LPMX_FP64 if f == "fp64" else P.LPMX_MIXED
         res[f] = run_form(P, prec, a.warmup, a.steps, profile=True, accel_depth=a.accel_depth,
-                          newton_tol=a.newton_tol, newton_maxit=a.newton_maxit)
+                          newton_tol=a.newton_tol, newton_maxit=a.newton_maxit, precond=a.precond)
         log(f"{f}: {a.steps} steps in {res[f]['wall']:.2f} s")
     main_form = "mixed" if "mixed" in res else forms[0]
     r = res[main_form]
@@ -380,7 +414,7 @@ def main():
                                f"uniaxial tension to 2% in {NSTEPS_SCHEDULE} load steps, E +/-5% (seed 0); "
                                f"timed load steps {a.warmup + 1}..{a.warmup + a.steps}",
                    "form": main_form, "newton_tol": a.newton_tol, "newton_maxit": a.newton_maxit, "linear_rtol2": RTOL2,
-                   "preconditioner": "block-Jacobi 3x3", "newton_accel_depth": a.accel_depth},
+                   "preconditioner": PRECOND_TEXT[a.precond], "newton_accel_depth": a.accel_depth},
         "dof_iters_per_s": sm[main_form]["dof_iters_per_s"],
         "forms": sm,
     }
@@ -391,9 +425,15 @@ def main():
         r64 = np.array(sm["fp64"]["reaction"])
         rm = np.array(sm["mixed"]["reaction"])
         line["mixed_reaction_rel_err"] = float(np.abs(rm - r64).max() / np.abs(r64).max())
-    line["roofline"] = roofline(r["prof"], main_form) if r["prof"] else None
-    if "fp64" in res and main_form != "fp64" and res["fp64"]["prof"]:
-        line["roofline_fp64"] = roofline(res["fp64"]["prof"], "fp64")
+    line["roofline"] = form_roofline(r, main_form)
+    if r.get("prof_sm") and r["prof"]:
+        line["roofline_cg_spmv"] = roofline(r["prof"], main_form)
+    if "fp64" in res and main_form != "fp64":
+        line["roofline_fp64"] = form_roofline(res["fp64"], "fp64")
+        if res["fp64"].get("prof_sm") and res["fp64"]["prof"]:
+            line["roofline_cg_spmv_fp64"] = roofline(res["fp64"]["prof"], "fp64")
+    if r.get("amg"):
+        line["amg"] = {f: res[f]["amg"] for f in res}
     if not a.no_cpu:
         t0 = time.time()
         line["cpu_baseline"] = cpu_baseline(P, a.cpu_seconds)

@@ -14,13 +14,16 @@
 extern "C" {
 #endif
 
-/* Per-particle factor on the elastic constants (E_i = factor_i * E), applied by the
- * next calcKnTv(); NULL switches it off.  The array must outlive that call. */
+/* Per-particle factor on the elastic constants (E_i = factor_i * E) of the current
+ * lattice, applied by every following calcKnTv(); NULL switches it off.  The nparticle
+ * values are copied; a new lattice (searchNormalNeighbor) drops them.  Needs the
+ * lattice (call after searchNormalNeighbor); LPMX_ERR_STATE otherwise. */
 int lpm_ext_set_modulus(const double *factor);
 
-/* Per-bond critical stretch [nparticle][nneighbors] of the brittle law (plmode 6),
- * symmetric in (i, j); NULL restores the global critical_bstrain.  The array is read
- * again by every updateBrittleDamage() call and must stay valid until it is replaced. */
+/* Per-bond critical stretch [nparticle][nneighbors] of the brittle law (plmode 6) of
+ * the current lattice, symmetric in (i, j); NULL restores the global
+ * critical_bstrain.  The values are copied; a new lattice drops them.  Needs the
+ * lattice (call after searchNormalNeighbor); LPMX_ERR_STATE otherwise. */
 int lpm_ext_set_bond_critical(const double *crit);
 
 /* Precision and tolerances of solverCG(); NULL restores the reference settings

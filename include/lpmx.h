@@ -204,6 +204,10 @@ typedef struct lpmx_solver
     int accel_depth;        /* 0: the reference's modified Newton; 1..16: Anderson
                              * acceleration of it with that many differences (same
                              * fixed points, far fewer iterations in plastic flow) */
+    int precond;            /* CG preconditioner: 0 block-Jacobi, 1 smoothed-aggregation
+                             * AMG (V-cycle, block-Jacobi Chebyshev smoothing; hierarchy
+                             * built on the first solve, kept while the pattern and the
+                             * constraint set stay) */
 } lpmx_solver;
 
 typedef struct lpmx_step_stats
@@ -216,6 +220,10 @@ typedef struct lpmx_step_stats
     int accel_reverts;      /* accelerated iterates rejected for the plain one */
     int accel_restarts;     /* histories restarted because the residual stagnated */
 } lpmx_step_stats;
+
+/* Preconditioner of the following solves (lpmx_solve*, solverCG): 0 block-Jacobi
+ * (default), 1 smoothed-aggregation AMG.  lpmx_run_steps sets it from its solver. */
+int lpmx_set_preconditioner(int kind);
 
 int lpmx_run_steps(const lpmx_solver *s, int nsteps, const int *nbd, const lpmx_dispbc *dbc,
                    const int *nbf, const lpmx_forcebc *fbc, int maxbc, char out_axis, int out_type,
@@ -288,6 +296,16 @@ int lpmx_spmv(int precision, const double *x, double *y);
  * switching on resets the totals. */
 void lpmx_profile_spmv(int on);
 void lpmx_profile_spmv_read(double *total_ms, int64_t *launches);
+/* With the AMG preconditioner: HIP-event totals of the finest level's smoother SpMV
+ * launches (y = b - K x, y -= K x) in the CG iterations that ran, while
+ * lpmx_profile_spmv is on. */
+void lpmx_profile_amg_read(double *total_ms, int64_t *launches);
+/* Algorithmic HBM bytes of one such launch: nnzb*(dim^2*s_val + 4) + 4*(nslices+1)
+ * + 3*n*dim*s_vec (x read once, b or y read, y written; s = 8 FP64, 4 mixed). */
+int lpmx_amg_smoother_bytes(int precision, double *bytes);
+/* The current AMG hierarchy: levels (0 when none), host set-up time of its last
+ * build, device memory it holds. */
+int lpmx_amg_info(int *levels, double *setup_seconds, int64_t *device_bytes);
 
 #ifdef __cplusplus
 }

@@ -59,7 +59,7 @@ class ForceBC(C.Structure):
 class Solver(C.Structure):
     _fields_ = [("precision", C.c_int), ("rtol2", C.c_double), ("atol2", C.c_double), ("maxit", C.c_int),
                 ("newton_tol", C.c_double), ("newton_maxit", C.c_int), ("inner_rtol2", C.c_double),
-                ("ir_maxit", C.c_int), ("accel_depth", C.c_int)]
+                ("ir_maxit", C.c_int), ("accel_depth", C.c_int), ("precond", C.c_int)]
 
 
 class StepStats(C.Structure):
@@ -140,6 +140,10 @@ def lib():
         "lpmx_cp_failures": [_ip],
         "lpmx_matrix_info": [C.POINTER(C.c_int64), C.POINTER(C.c_int64), _ip],
         "lpmx_spmv": [C.c_int, _dp, _dp],
+        "lpmx_set_preconditioner": [C.c_int],
+        "lpmx_profile_amg_read": [C.POINTER(C.c_double), C.POINTER(C.c_int64)],
+        "lpmx_amg_smoother_bytes": [C.c_int, C.POINTER(C.c_double)],
+        "lpmx_amg_info": [_ip, C.POINTER(C.c_double), C.POINTER(C.c_int64)],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -151,6 +155,7 @@ def lib():
     L.lpmx_profile_spmv.restype = None
     L.lpmx_profile_spmv_read.argtypes = [_dp, C.POINTER(C.c_int64)]
     L.lpmx_profile_spmv_read.restype = None
+    L.lpmx_profile_amg_read.restype = None
     L.lpm_ext_set_modulus.argtypes = [_dp]
     L.lpm_ext_set_bond_critical.argtypes = [_dp]
     L.lpm_ext_set_solver.argtypes = [C.POINTER(Solver)]
@@ -308,10 +313,10 @@ def set_params(plmode=0, J2_H=0.0, J2_xi=0.0, damage_L=0.5, damage_threshold=0.9
 
 
 def solver(precision=LPMX_FP64, rtol2=1e-8, atol2=1e-12, maxit=None, newton_tol=1e-4, newton_maxit=100,
-           inner_rtol2=1e-2, ir_maxit=100, accel_depth=0):
+           inner_rtol2=1e-2, ir_maxit=100, accel_depth=0, precond=0):
     n = gint("nparticle").value * gint("dim").value
     return Solver(precision, rtol2, atol2, maxit if maxit is not None else n, newton_tol, newton_maxit,
-                  inner_rtol2, ir_maxit, accel_depth)
+                  inner_rtol2, ir_maxit, accel_depth, precond)
 
 
 def bc_tables(steps):
@@ -492,14 +497,14 @@ def build_sc_model(L, E=200e3, nu=0.3, sigmay=200.0, H=2000.0, emod=None, damage
     set_materials([c] * 5, sigmay)
     set_params(plmode=0, J2_H=H, J2_xi=0.0, damage_L=0.5, damage_threshold=0.9, damagec_A=0.0,
                critical_bstrain=1e-2, nbreak=20)
-    if emod is not None:
-        Lb.lpm_ext_set_modulus(ptr(emod))
+    # the library copies the field; a new lattice has none until it is set
+    check(Lb.lpm_ext_set_modulus(ptr(np.ascontiguousarray(emod, np.float64)) if emod is not None else None),
+          "modulus field")
     Lb.calcKnTv()
     Lb.computedL()
     return n, x, t, c
 
 
-_ext_keep = {}
 
 
 def hex_types(x, radius=0.5):
@@ -562,13 +567,9 @@ def build_hex_model(box, E=70e3, nu=0.25, critical=1e-3, nbreak=20, crit=None):
     c = elastic_constants(E, nu)
     set_materials([c] * 3, 1e30)
     set_params(plmode=6, critical_bstrain=critical, nbreak=nbreak)
-    if crit is not None:
-        # the library keeps the pointer (updateBrittleDamage reads it each call)
-        _ext_keep["crit"] = np.ascontiguousarray(crit, np.float64)
-        Lb.lpm_ext_set_bond_critical(ptr(_ext_keep["crit"]))
-    else:
-        _ext_keep.pop("crit", None)
-        Lb.lpm_ext_set_bond_critical(None)
+    # the library copies the values (a new lattice drops them)
+    check(Lb.lpm_ext_set_bond_critical(ptr(np.ascontiguousarray(crit, np.float64)) if crit is not None else None),
+          "critical stretch")
     Lb.calcKnTv()
     Lb.computedL()
     return n, x, t, c

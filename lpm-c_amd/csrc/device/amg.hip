@@ -1,0 +1,1382 @@
+// Smoothed-aggregation algebraic multigrid as the CG preconditioner (no counterpart
+// in the reference, whose solverCG is unpreconditioned).
+//
+// Setup (host, from the current constraint-eliminated tangent):
+//   - aggregates: greedy in index order over a graph (a root and its free
+//     neighbours), then leftovers join a neighbouring aggregate; the graph is the
+//     bond graph on the finest level and, on each coarser level, the adjacency of the
+//     aggregates of the level above (a and b adjacent when a member of a has a
+//     neighbour in b), so every level coarsens by about the size of a neighbourhood
+//     and the Galerkin stencils stay compact;
+//   - tentative prolongator: the rigid-body modes of each aggregate (3 translations
+//     and 3 rotations in 3D, 2 + 1 in 2D) about its centroid, zero on constrained
+//     DOFs, orthonormalised per aggregate (modified Gram-Schmidt; a dependent mode
+//     gets a zero column); the triangular factor carries the modes to the next level;
+//   - smoothed prolongator P = (I - w D^-1 A) P0, w = 4 / (3 lambda_max(D^-1 A));
+//   - Galerkin coarse matrix P^T A P, down to at most ~1200 coarse DOFs, which are
+//     inverted densely.
+// The hierarchy is built on the first preconditioned solve and kept while the
+// pattern and the constraint set stay the same: the smoother of the finest level
+// always uses the current tangent and its block-Jacobi inverse, the coarse levels
+// the tangent of the build (the elastic tangent moves with the geometry only).
+// Solve (device): one V-cycle per CG iteration with a Chebyshev polynomial smoother
+// (block-Jacobi scaled, interval [1.15 lambda_max / ratio, 1.15 lambda_max]) of the
+// same degree before and after the coarse correction, so the preconditioner is
+// symmetric positive definite.  Every level is stored like the fine tangent: sliced
+// ELL blocks, 64 rows per slice, slot-major within a slice, so one kernel serves
+// the level matrices, P and R = P^T in FP64 and FP32.  Everything is per row, in a
+// fixed order: results are deterministic.
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <thread>
+#include <vector>
+
+#include "common.h"
+
+// ------------------------------------------------------------------ host side ----
+namespace
+{
+struct HB // block CSR, blocks row-major br x bc
+{
+    int nr = 0, nc = 0, br = 0, bc = 0;
+    std::vector<int64_t> ptr;
+    std::vector<int> idx;
+    std::vector<double> val;
+    int64_t nnzb() const { return ptr.empty() ? 0 : ptr[nr]; }
+};
+
+int host_threads(void)
+{
+    unsigned h = std::thread::hardware_concurrency();
+    int t = h ? (int)h : 4;
+    const char *e = getenv("OMP_NUM_THREADS");
+    if (e && atoi(e) > 0)
+        t = std::min(t, atoi(e));
+    return std::max(1, std::min(t, 32));
+}
+
+template <typename F>
+void parallel_for(int n, F f)
+{
+    const int T = std::min(host_threads(), std::max(1, n / 256));
+    if (T <= 1)
+    {
+        f(0, n, 0);
+        return;
+    }
+    std::vector<std::thread> th;
+    for (int t = 0; t < T; t++)
+    {
+        const int a = (int)((int64_t)n * t / T), b = (int)((int64_t)n * (t + 1) / T);
+        th.emplace_back([=, &f] { f(a, b, t); });
+    }
+    for (auto &x : th)
+        x.join();
+}
+
+// C = A B (A: br x bk blocks, B: bk x bc blocks); per row the columns ascend.
+HB spgemm(const HB &A, const HB &B)
+{
+    HB C;
+    C.nr = A.nr, C.nc = B.nc, C.br = A.br, C.bc = B.bc;
+    const int br = A.br, bk = A.bc, bc = B.bc, bs = br * bc;
+    const int T = std::min(host_threads(), std::max(1, A.nr / 256));
+    std::vector<std::vector<int>> ri(T), rn(T);      // per thread: row lengths, columns
+    std::vector<std::vector<double>> rv(T);
+    std::vector<int> first(T + 1);
+    for (int t = 0; t <= T; t++)
+        first[t] = (int)((int64_t)A.nr * t / T);
+    std::vector<std::thread> th;
+    for (int t = 0; t < T; t++)
+        th.emplace_back([&, t] {
+            std::vector<int> mark(B.nc, -1), cols;
+            std::vector<double> acc;
+            for (int i = first[t]; i < first[t + 1]; i++)
+            {
+                cols.clear();
+                for (int64_t ka = A.ptr[i]; ka < A.ptr[i + 1]; ka++)
+                {
+                    const int j = A.idx[ka];
+                    for (int64_t kb = B.ptr[j]; kb < B.ptr[j + 1]; kb++)
+                    {
+                        const int c = B.idx[kb];
+                        if (mark[c] < 0)
+                        {
+                            mark[c] = (int)cols.size();
+                            cols.push_back(c);
+                        }
+                    }
+                }
+                std::sort(cols.begin(), cols.end());
+                for (size_t q = 0; q < cols.size(); q++)
+                    mark[cols[q]] = (int)q;
+                acc.assign(cols.size() * bs, 0.0);
+                for (int64_t ka = A.ptr[i]; ka < A.ptr[i + 1]; ka++)
+                {
+                    const int j = A.idx[ka];
+                    const double *a = &A.val[ka * br * bk];
+                    for (int64_t kb = B.ptr[j]; kb < B.ptr[j + 1]; kb++)
+                    {
+                        const double *b = &B.val[kb * bk * bc];
+                        double *cblk = &acc[(size_t)mark[B.idx[kb]] * bs];
+                        for (int r = 0; r < br; r++)
+                            for (int s = 0; s < bk; s++)
+                            {
+                                const double ars = a[r * bk + s];
+                                if (ars == 0.0)
+                                    continue;
+                                for (int c = 0; c < bc; c++)
+                                    cblk[r * bc + c] += ars * b[s * bc + c];
+                            }
+                    }
+                }
+                rn[t].push_back((int)cols.size());
+                ri[t].insert(ri[t].end(), cols.begin(), cols.end());
+                rv[t].insert(rv[t].end(), acc.begin(), acc.end());
+                for (int c : cols)
+                    mark[c] = -1;
+            }
+        });
+    for (auto &x : th)
+        x.join();
+    C.ptr.assign(C.nr + 1, 0);
+    int row = 0;
+    for (int t = 0; t < T; t++)
+        for (int len : rn[t])
+        {
+            C.ptr[row + 1] = C.ptr[row] + len;
+            row++;
+        }
+    C.idx.reserve(C.ptr[C.nr]);
+    C.val.reserve(C.ptr[C.nr] * bs);
+    for (int t = 0; t < T; t++)
+    {
+        C.idx.insert(C.idx.end(), ri[t].begin(), ri[t].end());
+        C.val.insert(C.val.end(), rv[t].begin(), rv[t].end());
+    }
+    return C;
+}
+
+HB transpose(const HB &A)
+{
+    HB T;
+    T.nr = A.nc, T.nc = A.nr, T.br = A.bc, T.bc = A.br;
+    const int bs = A.br * A.bc;
+    T.ptr.assign(T.nr + 1, 0);
+    for (int64_t k = 0; k < A.nnzb(); k++)
+        T.ptr[A.idx[k] + 1]++;
+    for (int i = 0; i < T.nr; i++)
+        T.ptr[i + 1] += T.ptr[i];
+    std::vector<int64_t> pos(T.ptr.begin(), T.ptr.end() - 1);
+    T.idx.resize(A.nnzb());
+    T.val.resize(A.nnzb() * bs);
+    for (int i = 0; i < A.nr; i++) // ascending rows: columns of T ascend
+        for (int64_t k = A.ptr[i]; k < A.ptr[i + 1]; k++)
+        {
+            const int64_t d = pos[A.idx[k]]++;
+            T.idx[d] = i;
+            for (int r = 0; r < A.br; r++)
+                for (int c = 0; c < A.bc; c++)
+                    T.val[d * bs + c * A.br + r] = A.val[k * bs + r * A.bc + c];
+        }
+    return T;
+}
+
+// In-place inverse of a small dense matrix (Gauss-Jordan, partial pivoting); a
+// singular matrix gets the pseudo-inverse of its nonzero diagonal part.
+void small_inverse(int d, double *a)
+{
+    std::vector<double> m(a, a + d * d), inv(d * d, 0.0);
+    for (int i = 0; i < d; i++)
+        inv[i * d + i] = 1.0;
+    double scale = 0.0;
+    for (int i = 0; i < d * d; i++)
+        scale = std::max(scale, fabs(m[i]));
+    for (int c = 0; c < d; c++)
+    {
+        int p = c;
+        for (int r = c + 1; r < d; r++)
+            if (fabs(m[r * d + c]) > fabs(m[p * d + c]))
+                p = r;
+        if (!(fabs(m[p * d + c]) > 1e-14 * scale))
+        {
+            // a decoupled zero row and column (an unused coarse mode): identity there
+            for (int k = 0; k < d; k++)
+                m[c * d + k] = m[k * d + c] = 0.0, inv[c * d + k] = inv[k * d + c] = 0.0;
+            m[c * d + c] = 1.0, inv[c * d + c] = 1.0;
+            continue;
+        }
+        if (p != c)
+            for (int k = 0; k < d; k++)
+                std::swap(m[p * d + k], m[c * d + k]), std::swap(inv[p * d + k], inv[c * d + k]);
+        const double piv = m[c * d + c];
+        for (int k = 0; k < d; k++)
+            m[c * d + k] /= piv, inv[c * d + k] /= piv;
+        for (int r = 0; r < d; r++)
+            if (r != c && m[r * d + c] != 0.0)
+            {
+                const double f = m[r * d + c];
+                for (int k = 0; k < d; k++)
+                    m[r * d + k] -= f * m[c * d + k], inv[r * d + k] -= f * inv[c * d + k];
+            }
+    }
+    memcpy(a, inv.data(), sizeof(double) * d * d);
+}
+
+std::vector<double> diag_inverses(const HB &A)
+{
+    const int d = A.br, bs = d * d;
+    std::vector<double> Dinv((size_t)A.nr * bs, 0.0);
+    parallel_for(A.nr, [&](int a, int b, int) {
+        for (int i = a; i < b; i++)
+        {
+            double *o = &Dinv[(size_t)i * bs];
+            for (int64_t k = A.ptr[i]; k < A.ptr[i + 1]; k++)
+                if (A.idx[k] == i)
+                    memcpy(o, &A.val[k * bs], sizeof(double) * bs);
+            small_inverse(d, o);
+        }
+    });
+    return Dinv;
+}
+
+// y = A x
+void host_mv(const HB &A, const std::vector<double> &x, std::vector<double> &y)
+{
+    y.assign((size_t)A.nr * A.br, 0.0);
+    parallel_for(A.nr, [&](int a, int b, int) {
+        for (int i = a; i < b; i++)
+            for (int64_t k = A.ptr[i]; k < A.ptr[i + 1]; k++)
+            {
+                const double *blk = &A.val[k * A.br * A.bc];
+                const double *xv = &x[(size_t)A.idx[k] * A.bc];
+                for (int r = 0; r < A.br; r++)
+                {
+                    double s = 0;
+                    for (int c = 0; c < A.bc; c++)
+                        s += blk[r * A.bc + c] * xv[c];
+                    y[(size_t)i * A.br + r] += s;
+                }
+            }
+    });
+}
+
+// lambda_max(D^-1 A) by power iteration (A symmetric, D^-1 A similar to symmetric).
+double lambda_max(const HB &A, const std::vector<double> &Dinv, int iters)
+{
+    const int d = A.br;
+    const size_t m = (size_t)A.nr * d;
+    std::vector<double> x(m), y, z(m);
+    uint64_t s = 0x9E3779B97F4A7C15ull;
+    for (size_t i = 0; i < m; i++)
+    {
+        s ^= s << 13, s ^= s >> 7, s ^= s << 17;
+        x[i] = (double)(s >> 11) * (1.0 / 9007199254740992.0) - 0.5;
+    }
+    double lam = 0.0;
+    for (int it = 0; it < iters; it++)
+    {
+        double nx = 0;
+        for (double v : x)
+            nx += v * v;
+        nx = sqrt(nx);
+        if (!(nx > 0))
+            return 1.0;
+        for (double &v : x)
+            v /= nx;
+        host_mv(A, x, y);
+        for (int i = 0; i < A.nr; i++)
+            for (int r = 0; r < d; r++)
+            {
+                double t = 0;
+                for (int c = 0; c < d; c++)
+                    t += Dinv[(size_t)i * d * d + r * d + c] * y[(size_t)i * d + c];
+                z[(size_t)i * d + r] = t;
+            }
+        double nz = 0;
+        for (double v : z)
+            nz += v * v;
+        lam = sqrt(nz);
+        x.swap(z);
+    }
+    return lam;
+}
+
+// Greedy aggregation over an adjacency list (ascending neighbours, self excluded).
+std::vector<int> aggregate(int n, const std::vector<int64_t> &aptr, const std::vector<int> &adj, int *nagg)
+{
+    std::vector<int> agg(n, -1);
+    int na = 0;
+    for (int i = 0; i < n; i++)
+    {
+        if (agg[i] >= 0)
+            continue;
+        bool free_ = true;
+        for (int64_t k = aptr[i]; k < aptr[i + 1] && free_; k++)
+            free_ = agg[adj[k]] < 0;
+        if (!free_)
+            continue;
+        agg[i] = na;
+        for (int64_t k = aptr[i]; k < aptr[i + 1]; k++)
+            agg[adj[k]] = na;
+        na++;
+    }
+    // leftovers join the first aggregated neighbour's aggregate (of the first pass)
+    std::vector<int> a1 = agg;
+    for (int i = 0; i < n; i++)
+        if (a1[i] < 0)
+            for (int64_t k = aptr[i]; k < aptr[i + 1]; k++)
+                if (a1[adj[k]] >= 0)
+                {
+                    agg[i] = a1[adj[k]];
+                    break;
+                }
+    // isolated remainder: new aggregates with their unaggregated neighbours
+    for (int i = 0; i < n; i++)
+        if (agg[i] < 0)
+        {
+            agg[i] = na;
+            for (int64_t k = aptr[i]; k < aptr[i + 1]; k++)
+                if (agg[adj[k]] < 0)
+                    agg[adj[k]] = na;
+            na++;
+        }
+    *nagg = na;
+    return agg;
+}
+
+// Tentative prolongator: B (fine null space, [nr*d][nm] row-major) restricted to each
+// aggregate and orthonormalised; Bc receives the coarse null space [na*nm][nm].
+HB tentative(const std::vector<int> &agg, int na, int d, int nm, const std::vector<double> &B,
+             std::vector<double> &Bc)
+{
+    const int n = (int)agg.size();
+    std::vector<std::vector<int>> members(na);
+    for (int i = 0; i < n; i++)
+        members[agg[i]].push_back(i);
+    HB P;
+    P.nr = n, P.nc = na, P.br = d, P.bc = nm;
+    P.ptr.resize(n + 1);
+    for (int i = 0; i <= n; i++)
+        P.ptr[i] = i;
+    P.idx.assign(agg.begin(), agg.end());
+    P.val.assign((size_t)n * d * nm, 0.0);
+    Bc.assign((size_t)na * nm * nm, 0.0);
+    parallel_for(na, [&](int a0, int a1, int) {
+        std::vector<double> Q;
+        for (int a = a0; a < a1; a++)
+        {
+            const auto &mem = members[a];
+            const int rows = (int)mem.size() * d;
+            Q.assign((size_t)rows * nm, 0.0);
+            for (size_t q = 0; q < mem.size(); q++)
+                for (int r = 0; r < d; r++)
+                    for (int c = 0; c < nm; c++)
+                        Q[(q * d + r) * nm + c] = B[((size_t)mem[q] * d + r) * nm + c];
+            double *R = &Bc[(size_t)a * nm * nm];
+            double ref = 0.0;
+            for (int c = 0; c < nm; c++)
+            {
+                double s = 0;
+                for (int r = 0; r < rows; r++)
+                    s += Q[(size_t)r * nm + c] * Q[(size_t)r * nm + c];
+                ref = std::max(ref, sqrt(s));
+            }
+            for (int c = 0; c < nm; c++)
+            {
+                for (int p = 0; p < c; p++)
+                {
+                    double s = 0;
+                    for (int r = 0; r < rows; r++)
+                        s += Q[(size_t)r * nm + p] * Q[(size_t)r * nm + c];
+                    R[p * nm + c] = s;
+                    for (int r = 0; r < rows; r++)
+                        Q[(size_t)r * nm + c] -= s * Q[(size_t)r * nm + p];
+                }
+                double s = 0;
+                for (int r = 0; r < rows; r++)
+                    s += Q[(size_t)r * nm + c] * Q[(size_t)r * nm + c];
+                s = sqrt(s);
+                if (s > 1e-10 * ref && ref > 0)
+                {
+                    R[c * nm + c] = s;
+                    for (int r = 0; r < rows; r++)
+                        Q[(size_t)r * nm + c] /= s;
+                }
+                else
+                {
+                    R[c * nm + c] = 0.0;
+                    for (int r = 0; r < rows; r++)
+                        Q[(size_t)r * nm + c] = 0.0;
+                }
+            }
+            for (size_t q = 0; q < mem.size(); q++)
+                for (int r = 0; r < d; r++)
+                    for (int c = 0; c < nm; c++)
+                        P.val[((size_t)mem[q] * d + r) * nm + c] = Q[(q * d + r) * nm + c];
+        }
+    });
+    return P;
+}
+
+// P = P0 - w D^-1 (A P0)
+HB smooth_prolongator(const HB &A, const std::vector<double> &Dinv, const HB &P0, double w)
+{
+    HB AP = spgemm(A, P0);
+    const int d = A.br, nm = P0.bc;
+    // the result has AP's pattern (which contains P0's: A has its diagonal)
+    parallel_for(AP.nr, [&](int a, int b, int) {
+        std::vector<double> t(d * nm);
+        for (int i = a; i < b; i++)
+        {
+            const double *Di = &Dinv[(size_t)i * d * d];
+            for (int64_t k = AP.ptr[i]; k < AP.ptr[i + 1]; k++)
+            {
+                double *blk = &AP.val[k * d * nm];
+                for (int r = 0; r < d; r++)
+                    for (int c = 0; c < nm; c++)
+                    {
+                        double s = 0;
+                        for (int q = 0; q < d; q++)
+                            s += Di[r * d + q] * blk[q * nm + c];
+                        t[r * nm + c] = -w * s;
+                    }
+                if (AP.idx[k] == P0.idx[i])
+                    for (int e = 0; e < d * nm; e++)
+                        t[e] += P0.val[(size_t)i * d * nm + e];
+                memcpy(blk, t.data(), sizeof(double) * d * nm);
+            }
+        }
+    });
+    return AP;
+}
+
+// Graph of the aggregates: a and b are adjacent when a member of a has a neighbour in
+// b (ascending, self excluded).  It is the aggregation graph of the next level.
+void quotient_graph(const std::vector<int> &agg, int na, std::vector<int64_t> &aptr, std::vector<int> &adj)
+{
+    const int n = (int)agg.size();
+    std::vector<std::vector<int>> members(na);
+    for (int i = 0; i < n; i++)
+        members[agg[i]].push_back(i);
+    std::vector<int64_t> qptr(na + 1, 0);
+    std::vector<int> qadj, mark(na, -1), nb;
+    for (int a = 0; a < na; a++)
+    {
+        nb.clear();
+        for (int i : members[a])
+            for (int64_t k = aptr[i]; k < aptr[i + 1]; k++)
+            {
+                const int b = agg[adj[k]];
+                if (b != a && mark[b] != a)
+                {
+                    mark[b] = a;
+                    nb.push_back(b);
+                }
+            }
+        std::sort(nb.begin(), nb.end());
+        qadj.insert(qadj.end(), nb.begin(), nb.end());
+        qptr[a + 1] = (int64_t)qadj.size();
+    }
+    aptr.swap(qptr);
+    adj.swap(qadj);
+}
+} // namespace
+
+// ---------------------------------------------------------------- device side ----
+// A level operator on the device, in one of two layouts:
+//   SLICE: sliced ELL, 64 rows per slice, slot-major within a slice (the layout of
+//          the model's tangent): entry e of lane l's block in slot k at
+//          val[(k*BR*BC + e)*64 + l]; one thread per row.  For matrices with many
+//          rows (enough threads to fill the device).
+//   WROW:  block CSR, one wavefront per row: entry e of the row's j-th block at
+//          val[rptr[i]*BR*BC + e*len + j] (len = row length), so the row's lanes read
+//          consecutive blocks of one entry in one access.  For matrices with few,
+//          long rows (the coarse Galerkin operators and the restrictions).
+enum
+{
+    LAYOUT_SLICE = 0,
+    LAYOUT_WROW = 1
+};
+struct Mat
+{
+    int layout = LAYOUT_SLICE;
+    int nr = 0, nc = 0, br = 0, bc = 0, nslices = 0;
+    int64_t nslots = 0, nnzb = 0;
+    int *sptr = nullptr, *col = nullptr;
+    int64_t *rptr = nullptr;
+    double *val = nullptr;
+    float *val32 = nullptr;
+};
+
+struct Level
+{
+    int n = 0, d = 0;   // block rows and block size
+    Mat A;              // level matrix (level 0: the model's tangent, not owned)
+    double *Minv = nullptr;
+    float *Minv32 = nullptr;
+    Mat P, R;           // to and from the next level
+    double *b = nullptr, *x = nullptr, *r = nullptr, *dd = nullptr;
+    float *b32 = nullptr, *x32 = nullptr, *r32 = nullptr, *d32 = nullptr;
+    double lmax = 2.0;
+};
+
+static struct Amg
+{
+    bool ready = false;
+    int nlev = 0;           // levels with a smoother; the last one is solved densely
+    std::vector<Level> L;
+    int nc_dense = 0;       // DOFs of the dense coarsest level
+    double *Ainv = nullptr; // dense inverse
+    float *Ainv32 = nullptr;
+    std::vector<void *> mem;
+    int64_t n_model = -1;
+    uint64_t bc_hash = 0;
+    double t_setup = 0.0;
+    int degree = 1;         // Chebyshev degree on the finest level
+    int cdegree = 1;        // ... and on the coarser levels
+    double ratio = 10.0;
+    int64_t bytes = 0;      // device memory of the hierarchy
+} AM;
+
+// rows from which one thread per row fills the device (256 CUs x 1024 threads)
+static const int SLICE_MIN_ROWS = 262144;
+
+// Copies between the host and the model's data, ordered on the library's stream (a
+// non-blocking stream: a copy on the null stream would not wait for the kernels
+// queued on it, e.g. the constraint elimination right before a solve).
+static int d2h(void *dst, const void *src, size_t bytes)
+{
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, g_stream));
+    HIPCHK(hipStreamSynchronize(g_stream));
+    return LPMX_OK;
+}
+
+static int h2d(void *dst, const void *src, size_t bytes)
+{
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, g_stream));
+    HIPCHK(hipStreamSynchronize(g_stream));
+    return LPMX_OK;
+}
+
+template <typename T>
+static int amg_alloc(T **p, int64_t count)
+{
+    const size_t by = sizeof(T) * (size_t)std::max<int64_t>(count, 1);
+    HIPCHK(hipMalloc((void **)p, by));
+    AM.mem.push_back(*p);
+    AM.bytes += (int64_t)by;
+    return LPMX_OK;
+}
+
+void lpmx_amg_free(void)
+{
+    for (void *p : AM.mem)
+        hipFree(p);
+    const int deg = AM.degree, cdeg = AM.cdegree;
+    const double ratio = AM.ratio;
+    AM = Amg();
+    AM.degree = deg, AM.cdegree = cdeg, AM.ratio = ratio;
+}
+
+template <typename T>
+static int upload(T *dst, const std::vector<T> &src)
+{
+    RETCHK(h2d(dst, src.data(), sizeof(T) * src.size()));
+    return LPMX_OK;
+}
+
+static int upload_vals(const std::vector<double> &val, Mat &B)
+{
+    std::vector<float> v32(val.size());
+    for (size_t q = 0; q < val.size(); q++)
+        v32[q] = (float)val[q];
+    RETCHK(amg_alloc(&B.val, (int64_t)val.size()));
+    RETCHK(amg_alloc(&B.val32, (int64_t)v32.size()));
+    RETCHK(upload(B.val, val));
+    RETCHK(upload(B.val32, v32));
+    return LPMX_OK;
+}
+
+static int upload_mat(const HB &H, Mat &B)
+{
+    B.nr = H.nr, B.nc = H.nc, B.br = H.br, B.bc = H.bc, B.nnzb = H.nnzb();
+    const int bs = H.br * H.bc;
+    if (H.nr < SLICE_MIN_ROWS)
+    {
+        B.layout = LAYOUT_WROW;
+        std::vector<double> val((size_t)B.nnzb * bs);
+        parallel_for(H.nr, [&](int a, int b, int) {
+            for (int i = a; i < b; i++)
+            {
+                const int64_t k0 = H.ptr[i], len = H.ptr[i + 1] - k0;
+                for (int64_t j = 0; j < len; j++)
+                    for (int e = 0; e < bs; e++)
+                        val[(size_t)(k0 * bs + e * len + j)] = H.val[(size_t)((k0 + j) * bs + e)];
+            }
+        });
+        RETCHK(amg_alloc(&B.rptr, (int64_t)H.nr + 1));
+        RETCHK(amg_alloc(&B.col, B.nnzb));
+        RETCHK(upload(B.rptr, H.ptr));
+        RETCHK(upload(B.col, H.idx));
+        return upload_vals(val, B);
+    }
+    B.layout = LAYOUT_SLICE;
+    B.nslices = (H.nr + 63) / 64;
+    std::vector<int> sptr(B.nslices + 1, 0);
+    for (int s = 0; s < B.nslices; s++)
+    {
+        int w = 0;
+        for (int l = 0; l < 64 && s * 64 + l < H.nr; l++)
+        {
+            const int i = s * 64 + l;
+            w = std::max<int>(w, (int)(H.ptr[i + 1] - H.ptr[i]));
+        }
+        sptr[s + 1] = sptr[s] + w;
+    }
+    B.nslots = sptr[B.nslices];
+    std::vector<int> col((size_t)B.nslots * 64, 0);
+    std::vector<double> val((size_t)B.nslots * bs * 64, 0.0);
+    for (int i = 0; i < H.nr; i++)
+    {
+        const int s = i / 64, l = i % 64;
+        int k = 0;
+        for (int64_t q = H.ptr[i]; q < H.ptr[i + 1]; q++, k++)
+        {
+            const int64_t slot = sptr[s] + k;
+            col[slot * 64 + l] = H.idx[q];
+            for (int e = 0; e < bs; e++)
+                val[(slot * bs + e) * 64 + l] = H.val[q * bs + e];
+        }
+        for (; sptr[s] + k < sptr[s + 1]; k++) // padding: a zero block at a valid column
+            col[(int64_t)(sptr[s] + k) * 64 + l] = H.nr == H.nc ? i : 0;
+    }
+    RETCHK(amg_alloc(&B.sptr, B.nslices + 1));
+    RETCHK(amg_alloc(&B.col, (int64_t)col.size()));
+    RETCHK(upload(B.sptr, sptr));
+    RETCHK(upload(B.col, col));
+    return upload_vals(val, B);
+}
+
+static int alloc_vectors(Level &v)
+{
+    const int64_t m = (int64_t)v.n * v.d;
+    RETCHK(amg_alloc(&v.b, m));
+    RETCHK(amg_alloc(&v.x, m));
+    RETCHK(amg_alloc(&v.r, m));
+    RETCHK(amg_alloc(&v.dd, m));
+    RETCHK(amg_alloc(&v.b32, m));
+    RETCHK(amg_alloc(&v.x32, m));
+    RETCHK(amg_alloc(&v.r32, m));
+    RETCHK(amg_alloc(&v.d32, m));
+    return LPMX_OK;
+}
+
+static uint64_t mask_hash(const std::vector<int> &db, const std::vector<int> &fx)
+{
+    uint64_t h = 1469598103934665603ull;
+    for (size_t k = 0; k < db.size(); k++)
+    {
+        h ^= (uint64_t)(db[k] != 0 && fx[k] != 0);
+        h *= 1099511628211ull;
+    }
+    return h;
+}
+
+static int read_masks(std::vector<int> &db, std::vector<int> &fx)
+{
+    const int64_t m = (int64_t)g.n * g.dim;
+    db.resize(m), fx.resize(m);
+    RETCHK(d2h(db.data(), g.dispBC, sizeof(int) * m));
+    RETCHK(d2h(fx.data(), g.fix, sizeof(int) * m));
+    return LPMX_OK;
+}
+
+// Whether the hierarchy fits the current model and constraint set.
+bool lpmx_amg_current(void)
+{
+    if (!AM.ready || AM.n_model != g.n)
+        return false;
+    std::vector<int> db, fx;
+    if (read_masks(db, fx) != LPMX_OK)
+        return false;
+    return mask_hash(db, fx) == AM.bc_hash;
+}
+
+double lpmx_amg_setup_seconds(void) { return AM.t_setup; }
+int lpmx_amg_levels(void) { return AM.ready ? AM.nlev + 1 : 0; }
+
+// Build the hierarchy from the current (constraint-eliminated) tangent.
+int lpmx_amg_build(void)
+{
+    RETCHK(lpmx_require_pattern());
+    const auto t0 = std::chrono::steady_clock::now();
+    lpmx_amg_free();
+    if (const char *e = getenv("LPMX_AMG_DEGREE"))
+        AM.degree = std::max(1, std::min(6, atoi(e)));
+    if (const char *e = getenv("LPMX_AMG_CDEGREE"))
+        AM.cdegree = std::max(1, std::min(6, atoi(e)));
+    if (const char *e = getenv("LPMX_AMG_RATIO"))
+        AM.ratio = std::max(2.0, atof(e));
+    const int n = g.n, D = g.dim, D2 = D * D, nm = D == 3 ? 6 : 3;
+    const bool progress = getenv("LPMX_PROGRESS") != nullptr;
+    auto phase = [&](const char *what) {
+        if (progress)
+            fprintf(stderr, "[lpmx] AMG setup: %s at %.2f s\n", what,
+                    std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
+    };
+    // the fine tangent in block CSR
+    std::vector<int> sptr(g.nslices + 1), col((size_t)g.nslots * 64), rowlen(n), nbr((size_t)n * g.nb), nb0(n);
+    std::vector<double> K((size_t)g.nslots * D2 * 64), x((size_t)3 * n);
+    RETCHK(d2h(sptr.data(), g.sptr, sizeof(int) * sptr.size()));
+    RETCHK(d2h(col.data(), g.col, sizeof(int) * col.size()));
+    RETCHK(d2h(K.data(), g.K, sizeof(double) * K.size()));
+    RETCHK(d2h(rowlen.data(), g.nb_conn, sizeof(int) * n));
+    RETCHK(d2h(nbr.data(), g.nbr, sizeof(int) * nbr.size()));
+    RETCHK(d2h(nb0.data(), g.nb0, sizeof(int) * n));
+    RETCHK(d2h(x.data(), g.x, sizeof(double) * x.size()));
+    std::vector<int> db, fx;
+    RETCHK(read_masks(db, fx));
+    HB A;
+    A.nr = A.nc = n, A.br = A.bc = D;
+    A.ptr.assign(n + 1, 0);
+    for (int i = 0; i < n; i++)
+        A.ptr[i + 1] = A.ptr[i] + rowlen[i];
+    A.idx.resize(A.ptr[n]);
+    A.val.resize(A.ptr[n] * D2);
+    parallel_for(n, [&](int a, int b, int) {
+        std::vector<std::pair<int, int>> ord;
+        for (int i = a; i < b; i++)
+        {
+            const int s = i / 64, l = i % 64;
+            ord.clear();
+            for (int k = 0; k < rowlen[i]; k++)
+                ord.push_back({col[(size_t)(sptr[s] + k) * 64 + l], k});
+            std::sort(ord.begin(), ord.end());
+            for (int q = 0; q < rowlen[i]; q++)
+            {
+                const int64_t slot = sptr[s] + ord[q].second, o = A.ptr[i] + q;
+                A.idx[o] = ord[q].first;
+                for (int e = 0; e < D2; e++)
+                    A.val[o * D2 + e] = K[(slot * D2 + e) * 64 + l];
+            }
+        }
+    });
+    std::vector<double>().swap(K);
+    // fine null space: rigid-body modes, zero on constrained DOFs (centred later)
+    std::vector<double> B((size_t)n * D * nm, 0.0);
+    // bond graph (ascending, self excluded)
+    std::vector<int64_t> aptr(n + 1, 0);
+    std::vector<int> adj;
+    for (int i = 0; i < n; i++)
+    {
+        std::vector<int> nb;
+        for (int j = 0; j < nb0[i]; j++)
+        {
+            const int q = nbr[(size_t)j * n + i];
+            if (q >= 0 && q != i)
+                nb.push_back(q);
+        }
+        std::sort(nb.begin(), nb.end());
+        nb.erase(std::unique(nb.begin(), nb.end()), nb.end());
+        adj.insert(adj.end(), nb.begin(), nb.end());
+        aptr[i + 1] = (int64_t)adj.size();
+    }
+    int na = 0;
+    phase("fine matrix copied");
+    std::vector<int> agg = aggregate(n, aptr, adj, &na);
+    {
+        std::vector<double> cx((size_t)na * 3, 0.0);
+        std::vector<int> cnt(na, 0);
+        for (int i = 0; i < n; i++)
+        {
+            for (int c = 0; c < 3; c++)
+                cx[(size_t)agg[i] * 3 + c] += x[(size_t)3 * i + c];
+            cnt[agg[i]]++;
+        }
+        for (int a = 0; a < na; a++)
+            for (int c = 0; c < 3; c++)
+                cx[(size_t)a * 3 + c] /= cnt[a];
+        for (int i = 0; i < n; i++)
+        {
+            const double px = x[3 * i] - cx[(size_t)agg[i] * 3], py = x[3 * i + 1] - cx[(size_t)agg[i] * 3 + 1],
+                         pz = x[3 * i + 2] - cx[(size_t)agg[i] * 3 + 2];
+            double *b = &B[(size_t)i * D * nm];
+            if (D == 3)
+            {
+                const double m6[3][6] = {{1, 0, 0, 0, pz, -py}, {0, 1, 0, -pz, 0, px}, {0, 0, 1, py, -px, 0}};
+                for (int r = 0; r < 3; r++)
+                    for (int c = 0; c < 6; c++)
+                        b[r * 6 + c] = m6[r][c];
+            }
+            else
+            {
+                const double m3[2][3] = {{1, 0, -py}, {0, 1, px}};
+                for (int r = 0; r < 2; r++)
+                    for (int c = 0; c < 3; c++)
+                        b[r * 3 + c] = m3[r][c];
+            }
+            for (int r = 0; r < D; r++)
+                if (!(db[(size_t)i * D + r] && fx[(size_t)i * D + r]))
+                    for (int c = 0; c < nm; c++)
+                        b[r * nm + c] = 0.0;
+        }
+    }
+    // levels
+    std::vector<HB> As, Ps;
+    std::vector<std::vector<double>> Dinvs;
+    std::vector<double> lm;
+    As.push_back(std::move(A));
+    const int max_levels = 8, dense_max = 1200;
+    for (int lev = 0; lev < max_levels; lev++)
+    {
+        HB &Al = As[lev];
+        std::vector<double> Dinv = diag_inverses(Al);
+        const double lmax = lambda_max(Al, Dinv, 20);
+        lm.push_back(lmax);
+        if ((int64_t)Al.nr * Al.br <= dense_max || lev == max_levels - 1)
+        {
+            Dinvs.push_back(std::move(Dinv));
+            break;
+        }
+        if (lev > 0)
+            agg = aggregate(Al.nr, aptr, adj, &na);
+        if (na >= Al.nr) // no coarsening possible
+        {
+            Dinvs.push_back(std::move(Dinv));
+            break;
+        }
+        std::vector<double> Bc;
+        HB P0 = tentative(agg, na, Al.br, nm, B, Bc);
+        HB P = smooth_prolongator(Al, Dinv, P0, 4.0 / (3.0 * lmax));
+        HB R = transpose(P);
+        HB AP = spgemm(Al, P);
+        HB Ac = spgemm(R, AP);
+        Dinvs.push_back(std::move(Dinv));
+        Ps.push_back(std::move(P));
+        As.push_back(std::move(Ac));
+        B.swap(Bc);
+        quotient_graph(agg, na, aptr, adj);
+        if (progress)
+            fprintf(stderr, "[lpmx] AMG setup: level %d -> %d block rows of %d\n", lev, As.back().nr, As.back().br);
+        phase("Galerkin product done");
+    }
+    // upload: levels 0 .. nlev-1 smoothed, level nlev dense
+    AM.nlev = (int)As.size() - 1;
+    AM.L.resize(AM.nlev + 1);
+    for (int lev = 0; lev <= AM.nlev; lev++)
+    {
+        Level &v = AM.L[lev];
+        v.n = As[lev].nr, v.d = As[lev].br;
+        v.lmax = lm[lev];
+        if (lev == AM.nlev)
+            break;
+        if (lev > 0)
+        {
+            RETCHK(upload_mat(As[lev], v.A));
+            const int64_t bs = (int64_t)v.d * v.d;
+            std::vector<double> mp((size_t)bs * v.n);
+            std::vector<float> mp32((size_t)bs * v.n);
+            for (int i = 0; i < v.n; i++)
+                for (int e = 0; e < bs; e++)
+                {
+                    mp[(size_t)e * v.n + i] = Dinvs[lev][(size_t)i * bs + e];
+                    mp32[(size_t)e * v.n + i] = (float)mp[(size_t)e * v.n + i];
+                }
+            RETCHK(amg_alloc(&v.Minv, bs * v.n));
+            RETCHK(amg_alloc(&v.Minv32, bs * v.n));
+            RETCHK(h2d(v.Minv, mp.data(), sizeof(double) * mp.size()));
+            RETCHK(h2d(v.Minv32, mp32.data(), sizeof(float) * mp32.size()));
+        }
+        RETCHK(upload_mat(Ps[lev], v.P));
+        RETCHK(upload_mat(transpose(Ps[lev]), v.R));
+        RETCHK(alloc_vectors(v));
+    }
+    // dense coarsest level
+    {
+        const HB &Ac = As[AM.nlev];
+        const int m = Ac.nr * Ac.br;
+        if (m > 6000)
+        {
+            lpmx_amg_free();
+            return lpmx_fail(LPMX_ERR_STATE, "AMG coarsening stalled at %d DOFs", m);
+        }
+        std::vector<double> M((size_t)m * m, 0.0);
+        for (int i = 0; i < Ac.nr; i++)
+            for (int64_t k = Ac.ptr[i]; k < Ac.ptr[i + 1]; k++)
+                for (int r = 0; r < Ac.br; r++)
+                    for (int c = 0; c < Ac.bc; c++)
+                        M[(size_t)(i * Ac.br + r) * m + Ac.idx[k] * Ac.bc + c] = Ac.val[k * Ac.br * Ac.bc + r * Ac.bc + c];
+        small_inverse(m, M.data());
+        std::vector<float> M32(M.size());
+        for (size_t q = 0; q < M.size(); q++)
+            M32[q] = (float)M[q];
+        AM.nc_dense = m;
+        RETCHK(amg_alloc(&AM.Ainv, (int64_t)M.size()));
+        RETCHK(amg_alloc(&AM.Ainv32, (int64_t)M.size()));
+        RETCHK(h2d(AM.Ainv, M.data(), sizeof(double) * M.size()));
+        RETCHK(h2d(AM.Ainv32, M32.data(), sizeof(float) * M32.size()));
+        Level &v = AM.L[AM.nlev];
+        RETCHK(alloc_vectors(v));
+    }
+    AM.n_model = n;
+    AM.bc_hash = mask_hash(db, fx);
+    AM.ready = true;
+    AM.t_setup = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (getenv("LPMX_PROGRESS"))
+    {
+        fprintf(stderr, "[lpmx] AMG setup %.2f s:", AM.t_setup);
+        for (int lev = 0; lev <= AM.nlev; lev++)
+            fprintf(stderr, " L%d %d x %d (%lld blocks, lmax %.3f)", lev, AM.L[lev].n, AM.L[lev].d,
+                    (long long)As[lev].nnzb(), AM.L[lev].lmax);
+        fprintf(stderr, "\n");
+    }
+    return LPMX_OK;
+}
+
+// ------------------------------------------------------------------ kernels ------
+// y = A x (MODE 0), y = b - A x (1), y += A x (2), y -= A x (3) for the lane's row.
+template <int BR, typename TX, int MODE>
+__device__ __forceinline__ void mv_store(int64_t i, const TX *acc, TX *__restrict__ y, const TX *__restrict__ b)
+{
+#pragma unroll
+    for (int r = 0; r < BR; r++)
+    {
+        const int64_t o = i * BR + r;
+        if (MODE == 0)
+            y[o] = acc[r];
+        else if (MODE == 1)
+            y[o] = b[o] - acc[r];
+        else if (MODE == 2)
+            y[o] += acc[r];
+        else
+            y[o] -= acc[r];
+    }
+}
+
+template <int BR, int BC, typename TV, typename TX>
+__device__ __forceinline__ void slice_mac(const TV *__restrict__ kb, const TX *__restrict__ x, int m, TX *acc)
+{
+    TX xm[BC];
+#pragma unroll
+    for (int c = 0; c < BC; c++)
+        xm[c] = x[(int64_t)m * BC + c];
+#pragma unroll
+    for (int r = 0; r < BR; r++)
+#pragma unroll
+        for (int c = 0; c < BC; c++)
+            acc[r] += (TX)__builtin_nontemporal_load(kb + (r * BC + c) * 64) * xm[c];
+}
+
+// SLICE layout, one thread per row; U slots loaded ahead per step; work-groups
+// remapped so that each XCD walks a contiguous range of rows.
+template <int BR, int BC, typename TV, typename TX, int MODE, int U>
+__global__ __launch_bounds__(256) void k_slice_mv(int nr, const int *__restrict__ sptr, const int *__restrict__ col,
+                                                  const TV *__restrict__ val, const TX *__restrict__ x,
+                                                  TX *__restrict__ y, const TX *__restrict__ b, const int *done)
+{
+    if (done && *done)
+        return;
+    const int i = xcd_block() * 256 + threadIdx.x;
+    if (i >= nr)
+        return;
+    const int s = i >> 6, l = i & 63;
+    const int k1 = sptr[s + 1];
+    TX acc[BR];
+#pragma unroll
+    for (int r = 0; r < BR; r++)
+        acc[r] = 0;
+    int k = sptr[s];
+    for (; k + U <= k1; k += U)
+    {
+        int m[U];
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            m[u] = __builtin_nontemporal_load(col + (int64_t)(k + u) * 64 + l);
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            slice_mac<BR, BC, TV, TX>(val + (int64_t)(k + u) * BR * BC * 64 + l, x, m[u], acc);
+    }
+    for (; k < k1; k++)
+        slice_mac<BR, BC, TV, TX>(val + (int64_t)k * BR * BC * 64 + l, x,
+                                  __builtin_nontemporal_load(col + (int64_t)k * 64 + l), acc);
+    mv_store<BR, TX, MODE>(i, acc, y, b);
+}
+
+// WROW layout, one wavefront per row (4 rows per work-group); the lanes' partial
+// sums are combined in a fixed butterfly order.
+template <int BR, int BC, typename TV, typename TX, int MODE>
+__global__ __launch_bounds__(256) void k_wrow_mv(int nr, const int64_t *__restrict__ rptr, const int *__restrict__ col,
+                                                 const TV *__restrict__ val, const TX *__restrict__ x,
+                                                 TX *__restrict__ y, const TX *__restrict__ b, const int *done)
+{
+    if (done && *done)
+        return;
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (i >= nr) // uniform over the wavefront
+        return;
+    const int64_t k0 = rptr[i];
+    const int len = (int)(rptr[i + 1] - k0);
+    const TV *v = val + k0 * (BR * BC);
+    TX acc[BR];
+#pragma unroll
+    for (int r = 0; r < BR; r++)
+        acc[r] = 0;
+    for (int j = lane; j < len; j += 64)
+    {
+        const int m = col[k0 + j];
+        TX xm[BC];
+#pragma unroll
+        for (int c = 0; c < BC; c++)
+            xm[c] = x[(int64_t)m * BC + c];
+#pragma unroll
+        for (int r = 0; r < BR; r++)
+#pragma unroll
+            for (int c = 0; c < BC; c++)
+                acc[r] += (TX)__builtin_nontemporal_load(v + (int64_t)(r * BC + c) * len + j) * xm[c];
+    }
+#pragma unroll
+    for (int r = 0; r < BR; r++)
+        acc[r] = wave_sum(acc[r]);
+    if (lane == 0)
+        mv_store<BR, TX, MODE>(i, acc, y, b);
+}
+
+// Chebyshev start: d = c M f; x = d (SET) or x += d; r = f (SET only).
+template <int D, typename TM, typename TX, bool SET>
+__global__ __launch_bounds__(256) void k_cheb_start(int n, const TM *__restrict__ Minv, const TX *__restrict__ f,
+                                                    TX *__restrict__ d, TX *__restrict__ x, TX *__restrict__ r,
+                                                    double c, const int *done)
+{
+    if (done && *done)
+        return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    TX fv[D];
+#pragma unroll
+    for (int q = 0; q < D; q++)
+        fv[q] = f[(int64_t)i * D + q];
+#pragma unroll
+    for (int a = 0; a < D; a++)
+    {
+        TX z = 0;
+#pragma unroll
+        for (int q = 0; q < D; q++)
+            z += (TX)Minv[(int64_t)(a * D + q) * n + i] * fv[q];
+        const int64_t o = (int64_t)i * D + a;
+        const TX dv = (TX)c * z;
+        d[o] = dv;
+        if (SET)
+        {
+            x[o] = dv;
+            r[o] = fv[a];
+        }
+        else
+            x[o] += dv;
+    }
+}
+
+// Chebyshev step: d = c1 d + c2 M r; x += d.
+template <int D, typename TM, typename TX>
+__global__ __launch_bounds__(256) void k_cheb_step(int n, const TM *__restrict__ Minv, const TX *__restrict__ r,
+                                                   TX *__restrict__ d, TX *__restrict__ x, double c1, double c2,
+                                                   const int *done)
+{
+    if (done && *done)
+        return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    TX rv[D];
+#pragma unroll
+    for (int q = 0; q < D; q++)
+        rv[q] = r[(int64_t)i * D + q];
+#pragma unroll
+    for (int a = 0; a < D; a++)
+    {
+        TX z = 0;
+#pragma unroll
+        for (int q = 0; q < D; q++)
+            z += (TX)Minv[(int64_t)(a * D + q) * n + i] * rv[q];
+        const int64_t o = (int64_t)i * D + a;
+        const TX dv = (TX)c1 * d[o] + (TX)c2 * z;
+        d[o] = dv;
+        x[o] += dv;
+    }
+}
+
+// x = Ainv b (dense, row-major m x m), one wavefront per row.
+template <typename TX>
+__global__ __launch_bounds__(256) void k_dense_mv(int m, const TX *__restrict__ A, const TX *__restrict__ b,
+                                                  TX *__restrict__ x, const int *done)
+{
+    if (done && *done)
+        return;
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (i >= m)
+        return;
+    TX s = 0;
+    for (int k = lane; k < m; k += 64)
+        s += A[(int64_t)i * m + k] * b[k];
+    s = wave_sum(s);
+    if (lane == 0)
+        x[i] = s;
+}
+
+template <int BR, int BC, typename TX, int MODE>
+static void launch_mv_t(const Mat &B, const TX *x, TX *y, const TX *b, const int *done)
+{
+    if (B.layout == LAYOUT_WROW)
+    {
+        const dim3 G(nblocks(B.nr, 4)), T(256);
+        if (sizeof(TX) == 8)
+            hipLaunchKernelGGL((k_wrow_mv<BR, BC, double, TX, MODE>), G, T, 0, g_stream, B.nr, B.rptr, B.col,
+                               (const double *)B.val, x, y, b, done);
+        else
+            hipLaunchKernelGGL((k_wrow_mv<BR, BC, float, TX, MODE>), G, T, 0, g_stream, B.nr, B.rptr, B.col,
+                               (const float *)B.val32, x, y, b, done);
+        return;
+    }
+    const dim3 G(nblocks(B.nr, 256)), T(256);
+    if (sizeof(TX) == 8)
+        hipLaunchKernelGGL((k_slice_mv<BR, BC, double, TX, MODE, 1>), G, T, 0, g_stream, B.nr, B.sptr, B.col,
+                           (const double *)B.val, x, y, b, done);
+    else
+        hipLaunchKernelGGL((k_slice_mv<BR, BC, float, TX, MODE, 4>), G, T, 0, g_stream, B.nr, B.sptr, B.col,
+                           (const float *)B.val32, x, y, b, done);
+}
+
+template <typename TX, int MODE>
+static void launch_mv(const Mat &B, const TX *x, TX *y, const TX *b, const int *done)
+{
+#define LPMX_MV(BR_, BC_)                                                                                           \
+    if (B.br == BR_ && B.bc == BC_)                                                                                 \
+        return launch_mv_t<BR_, BC_, TX, MODE>(B, x, y, b, done);
+    LPMX_MV(3, 3)
+    LPMX_MV(6, 6)
+    LPMX_MV(3, 6)
+    LPMX_MV(6, 3)
+    LPMX_MV(2, 2)
+    LPMX_MV(2, 3)
+    LPMX_MV(3, 2)
+#undef LPMX_MV
+}
+
+template <typename TX>
+static TX *lv(double *a, float *b)
+{
+    return sizeof(TX) == 8 ? (TX *)a : (TX *)b;
+}
+
+// Level matrix of level l (the model's tangent, SLICE layout, on level 0).
+static Mat level_matrix(int l)
+{
+    if (l > 0)
+        return AM.L[l].A;
+    Mat B;
+    B.layout = LAYOUT_SLICE;
+    B.nr = B.nc = g.n, B.br = B.bc = g.dim, B.nslices = g.nslices, B.nslots = g.nslots;
+    B.sptr = g.sptr, B.col = g.col, B.val = g.K, B.val32 = g.K32;
+    return B;
+}
+
+template <typename TX>
+static const void *level_minv(int l)
+{
+    if (l > 0)
+        return sizeof(TX) == 8 ? (const void *)AM.L[l].Minv : (const void *)AM.L[l].Minv32;
+    return sizeof(TX) == 8 ? (const void *)g.Minv : (const void *)g.Minv32;
+}
+
+template <typename TX, bool SET>
+static void cheb_start(int d, int n, const void *Minv, const TX *f, TX *dd, TX *x, TX *r, double c, const int *done)
+{
+    const dim3 G(nblocks(n)), T(LPMX_BLOCK);
+    typedef typename std::conditional<sizeof(TX) == 8, double, float>::type TM;
+    if (d == 3)
+        hipLaunchKernelGGL((k_cheb_start<3, TM, TX, SET>), G, T, 0, g_stream, n, (const TM *)Minv, f, dd, x, r, c, done);
+    else if (d == 6)
+        hipLaunchKernelGGL((k_cheb_start<6, TM, TX, SET>), G, T, 0, g_stream, n, (const TM *)Minv, f, dd, x, r, c, done);
+    else
+        hipLaunchKernelGGL((k_cheb_start<2, TM, TX, SET>), G, T, 0, g_stream, n, (const TM *)Minv, f, dd, x, r, c, done);
+}
+
+template <typename TX>
+static void cheb_step(int d, int n, const void *Minv, const TX *r, TX *dd, TX *x, double c1, double c2, const int *done)
+{
+    const dim3 G(nblocks(n)), T(LPMX_BLOCK);
+    typedef typename std::conditional<sizeof(TX) == 8, double, float>::type TM;
+    if (d == 3)
+        hipLaunchKernelGGL((k_cheb_step<3, TM, TX>), G, T, 0, g_stream, n, (const TM *)Minv, r, dd, x, c1, c2, done);
+    else if (d == 6)
+        hipLaunchKernelGGL((k_cheb_step<6, TM, TX>), G, T, 0, g_stream, n, (const TM *)Minv, r, dd, x, c1, c2, done);
+    else
+        hipLaunchKernelGGL((k_cheb_step<2, TM, TX>), G, T, 0, g_stream, n, (const TM *)Minv, r, dd, x, c1, c2, done);
+}
+
+// ---------------------------------------------------------------- measuring ----
+static struct AmgProf
+{
+    bool on = false, have_ev = false;
+    enum { MAXEV = 128 };
+    hipEvent_t ev[2 * MAXEV];
+    int tag[MAXEV];
+    int used = 0, cur = -1;
+    double ms = 0.0;
+    int64_t n = 0;
+} AP;
+
+void lpmx_amg_prof_enable(bool on)
+{
+    AP.on = on, AP.used = 0, AP.cur = -1, AP.ms = 0.0, AP.n = 0;
+}
+
+void lpmx_amg_prof_tag(int k) { AP.cur = k; }
+
+int lpmx_amg_prof_flush(int nreal)
+{
+    if (AP.on)
+        for (int u = 0; u < AP.used; u++)
+            if (AP.tag[u] >= 0 && AP.tag[u] < nreal)
+            {
+                float ms = 0.0f;
+                HIPCHK(hipEventElapsedTime(&ms, AP.ev[2 * u], AP.ev[2 * u + 1]));
+                AP.ms += ms, AP.n++;
+            }
+    AP.used = 0, AP.cur = -1;
+    return LPMX_OK;
+}
+
+// A finest-level SpMV of the smoother, timed when profiling is on.
+template <typename TX, int MODE>
+static void fine_mv(const Mat &A, const TX *x, TX *y, const TX *b, const int *done)
+{
+    const bool timed = AP.on && AP.cur >= 0 && AP.used < AmgProf::MAXEV;
+    if (timed)
+    {
+        if (!AP.have_ev)
+        {
+            for (int e = 0; e < 2 * AmgProf::MAXEV; e++)
+                hipEventCreate(&AP.ev[e]);
+            AP.have_ev = true;
+        }
+        hipEventRecord(AP.ev[2 * AP.used], g_stream);
+    }
+    launch_mv<TX, MODE>(A, x, y, b, done);
+    if (timed)
+    {
+        hipEventRecord(AP.ev[2 * AP.used + 1], g_stream);
+        AP.tag[AP.used++] = AP.cur;
+    }
+}
+
+extern "C" void lpmx_profile_amg_read(double *total_ms, int64_t *launches)
+{
+    *total_ms = AP.ms;
+    *launches = AP.n;
+}
+
+// Algorithmic HBM bytes of one finest-level smoother SpMV (y = b - K x or y -= K x):
+// every stored block and its column index once, the slice pointers, x read once (its
+// gathers are re-reads from cache), and two vector streams (b or y read, y written).
+extern "C" int lpmx_amg_smoother_bytes(int precision, double *bytes)
+{
+    RETCHK(lpmx_require_pattern());
+    const double sv = precision == LPMX_MIXED ? 4.0 : 8.0;
+    const int D = g.dim;
+    *bytes = (double)g.nnzb * (D * D * sv + 4) + 4.0 * (g.nslices + 1) + 3.0 * g.n * D * sv;
+    return LPMX_OK;
+}
+
+extern "C" int lpmx_amg_info(int *levels, double *setup_seconds, int64_t *device_bytes)
+{
+    *levels = AM.ready ? AM.nlev + 1 : 0;
+    *setup_seconds = AM.t_setup;
+    *device_bytes = AM.bytes;
+    return LPMX_OK;
+}
+
+// Chebyshev smoothing of A x = f on level l (Saad, Iterative Methods for Sparse
+// Linear Systems, alg. 12.1): `degree` corrections, from x = 0 (SET) or from the
+// current x.  With SET (the pre-smoother) r holds f - A x on return, for the
+// restriction; the post-smoother leaves r stale (nothing reads it).
+template <typename TX, bool SET>
+static void smooth(int l, const TX *f, TX *x, TX *r, TX *dd, const int *done)
+{
+    const Level &v = AM.L[l];
+    const Mat A = level_matrix(l);
+    const void *Minv = level_minv<TX>(l);
+    const double hi = 1.15 * v.lmax, lo = hi / AM.ratio;
+    const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo), sigma = theta / delta;
+    double rho = 1.0 / sigma;
+    auto mv = [&](auto mode, const TX *xin, TX *y, const TX *b) {
+        if (l == 0)
+            fine_mv<TX, decltype(mode)::value>(A, xin, y, b, done);
+        else
+            launch_mv<TX, decltype(mode)::value>(A, xin, y, b, done);
+    };
+    if (SET)
+        cheb_start<TX, true>(v.d, v.n, Minv, f, dd, x, r, 1.0 / theta, done);
+    else
+    {
+        mv(std::integral_constant<int, 1>(), x, r, f); // r = f - A x
+        cheb_start<TX, false>(v.d, v.n, Minv, r, dd, x, nullptr, 1.0 / theta, done);
+    }
+    const int degree = l == 0 ? AM.degree : AM.cdegree;
+    for (int k = 1; k < degree; k++)
+    {
+        mv(std::integral_constant<int, 3>(), dd, r, nullptr); // r -= A d
+        const double rn = 1.0 / (2.0 * sigma - rho);
+        cheb_step<TX>(v.d, v.n, Minv, r, dd, x, rn * rho, 2.0 * rn / delta, done);
+        rho = rn;
+    }
+    if (SET)
+        mv(std::integral_constant<int, 3>(), dd, r, nullptr); // r = f - A x
+}
+
+template <typename TX>
+static void vcycle(int l, const TX *f, TX *x, const int *done)
+{
+    Level &v = AM.L[l];
+    if (l == AM.nlev)
+    {
+        const TX *Ai = sizeof(TX) == 8 ? (const TX *)AM.Ainv : (const TX *)AM.Ainv32;
+        hipLaunchKernelGGL(k_dense_mv<TX>, dim3(nblocks(AM.nc_dense, 4)), dim3(256), 0, g_stream, AM.nc_dense, Ai, f,
+                           x, done);
+        return;
+    }
+    TX *r = lv<TX>(v.r, v.r32), *dd = lv<TX>(v.dd, v.d32);
+    smooth<TX, true>(l, f, x, r, dd, done);
+    Level &c = AM.L[l + 1];
+    TX *cb = lv<TX>(c.b, c.b32), *cx = lv<TX>(c.x, c.x32);
+    launch_mv<TX, 0>(v.R, r, cb, nullptr, done);
+    vcycle<TX>(l + 1, cb, cx, done);
+    launch_mv<TX, 2>(v.P, cx, x, nullptr, done);
+    smooth<TX, false>(l, f, x, r, dd, done);
+}
+
+// z = B r with the V-cycle (r, z: fine DOF vectors); kernels return at once while
+// *done is set.
+int lpmx_amg_apply_f64(const double *r, double *z, const int *done)
+{
+    if (!AM.ready)
+        return lpmx_fail(LPMX_ERR_STATE, "AMG hierarchy not built");
+    vcycle<double>(0, r, z, done);
+    LAUNCHCHK();
+    return LPMX_OK;
+}
+
+int lpmx_amg_apply_f32(const float *r, float *z, const int *done)
+{
+    if (!AM.ready)
+        return lpmx_fail(LPMX_ERR_STATE, "AMG hierarchy not built");
+    vcycle<float>(0, r, z, done);
+    LAUNCHCHK();
+    return LPMX_OK;
+}

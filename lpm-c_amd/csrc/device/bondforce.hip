@@ -513,17 +513,28 @@ __global__ void k_count_bc(const int *dispBC, int m, int *cnt)
         atomicAdd(cnt, sh);
 }
 
+// per-particle squares of the residual and reaction norms (2n), sized for the model
 static double *u_sq = nullptr;
 static int *u_cnt = nullptr;
+static int64_t u_cap = 0;
+
+static void update_rr_free(void)
+{
+    if (u_sq)
+        hipFree(u_sq), hipFree(u_cnt);
+    u_sq = nullptr, u_cnt = nullptr, u_cap = 0;
+}
 
 extern "C" int lpmx_update_rr(double *nres, double *nrea, int *nreaction)
 {
     RETCHK(lpmx_require_model());
     const int n = g.n;
-    if (!u_sq)
+    if (u_cap < n)
     {
+        update_rr_free();
         HIPCHK(hipMalloc(&u_sq, sizeof(double) * 2 * (int64_t)n));
         HIPCHK(hipMalloc(&u_cnt, sizeof(int)));
+        u_cap = n;
     }
     hipLaunchKernelGGL(k_update_rr, dim3(nblocks(n)), dim3(LPMX_BLOCK), 0, g_stream, g, u_sq, u_sq + n);
     LAUNCHCHK();
@@ -795,6 +806,7 @@ __global__ void k_dnl_list(DV v, Grid3 gr, const int *start, const int *list, do
 
 void lpmx_damage_free(void)
 {
+    update_rr_free();
     if (g.dptr)
         hipFree(g.dptr), hipFree(g.didx), hipFree(g.dwt);
     if (d_cnt)

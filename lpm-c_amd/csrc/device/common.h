@@ -107,6 +107,18 @@ __device__ __forceinline__ T wave_sum(T v)
     return v;
 }
 
+// Work-groups are dealt to the 8 XCDs round robin; remap so that each XCD walks a
+// contiguous range of rows and its L2 keeps the gathered neighbour columns.
+__device__ __forceinline__ int xcd_block(void)
+{
+    const int G = gridDim.x, b = blockIdx.x;
+    const int per = G / 8, rem = G % 8;
+    const int x = b % 8, k = b / 8;
+    // XCDs 0..rem-1 own per+1 groups, the others per
+    const int base = x < rem ? x * (per + 1) : rem * (per + 1) + (x - rem) * per;
+    return base + k;
+}
+
 // Work-group sum (blockDim.x multiple of 64, at most 1024): wave sums, then the
 // wave totals in wave order.  All threads get the result.
 template <typename T>
@@ -196,6 +208,23 @@ AaHist *lpmx_aa_create(void);
 void lpmx_aa_destroy(AaHist *a);
 int lpmx_aa_begin(AaHist *a, int depth);
 int lpmx_aa_step(AaHist *a, double *x, const double *d, int *mixed);
+
+// smoothed-aggregation AMG (amg.hip): hierarchy from the current eliminated tangent,
+// V-cycle z = B r on fine DOF vectors (kernels return at once while *done is set)
+int lpmx_amg_build(void);
+bool lpmx_amg_current(void);
+int lpmx_amg_apply_f64(const double *r, double *z, const int *done);
+int lpmx_amg_apply_f32(const float *r, float *z, const int *done);
+int lpmx_amg_levels(void);
+double lpmx_amg_setup_seconds(void);
+void lpmx_amg_free(void);
+int lpmx_tangent_epoch(void); // increments with every tangent assembly (tangent.hip)
+// HIP-event timing of the finest level's smoother SpMV launches: the CG tags each
+// V-cycle with its iteration in the batch and, after the batch, keeps the launches
+// of the iterations that ran
+void lpmx_amg_prof_enable(bool on);
+void lpmx_amg_prof_tag(int k);
+int lpmx_amg_prof_flush(int nreal);
 
 // K x = b on the current constraint-eliminated tangent (solver.hip)
 int lpmx_solve_into(int precision, double rtol2, double atol2, int maxit, double inner_rtol2, int ir_maxit,

@@ -39,6 +39,7 @@ int lpmx_run_steps_impl(const lpmx_solver *s, int nsteps, const int *nbd, const 
     const int n = g.n, d = g.dim;
     const int64_t nbn = (int64_t)n * g.nb;
     int rc_all = LPMX_OK;
+    RETCHK(lpmx_set_preconditioner(s->precond));
     for (int st = 0; st < nsteps; st++)
     {
         lpmx_step_stats o;

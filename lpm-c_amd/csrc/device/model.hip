@@ -148,6 +148,7 @@ static void free_all(void)
     lpmx_solver_free();
     lpmx_damage_free();
     lpmx_accel_free();
+    lpmx_amg_free();
     lpmx_cp_free();
     lpmx_adjoint_free();
     memset(&g, 0, sizeof(g));

@@ -14,6 +14,8 @@
 // once when the device flag `done` is set, so the host launches batches of
 // iterations and reads the flag once per batch.
 #include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <vector>
@@ -103,18 +105,6 @@ struct Acc<float>
     __device__ void add(float a, float b) { v = f2_dot_acc(v, a, b); }
     __device__ double block(double *sh) { return block_sum_f2(v, sh); }
 };
-
-// Work-groups are dealt to the 8 XCDs round robin; remap so that each XCD walks a
-// contiguous range of rows and its L2 keeps the gathered neighbour columns.
-__device__ __forceinline__ int xcd_block(void)
-{
-    const int G = gridDim.x, b = blockIdx.x;
-    const int per = G / 8, rem = G % 8;
-    const int x = b % 8, k = b / 8;
-    // XCDs 0..rem-1 own per+1 groups, the others per
-    const int base = x < rem ? x * (per + 1) : rem * (per + 1) + (x - rem) * per;
-    return base + k;
-}
 
 // Sum of parts[stride*k + off], k < np, by one work-group in a fixed order (the
 // partials were written by an earlier kernel on the same stream).
@@ -327,6 +317,69 @@ __global__ __launch_bounds__(256) void k_cg_init(int n, const double *__restrict
         parts[2 * blockIdx.x] = prz, parts[2 * blockIdx.x + 1] = prr;
 }
 
+// With an external preconditioner (the AMG V-cycle): the update without z, ...
+template <int D, typename TX>
+__global__ __launch_bounds__(256) void k_cg_update_np(int n, TX *__restrict__ x, const TX *__restrict__ p,
+                                                      TX *__restrict__ r, const TX *__restrict__ q, const CgS *sc)
+{
+    if (sc->done)
+        return;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n)
+        return;
+    const TX alpha = (TX)sc->alpha;
+#pragma unroll
+    for (int c = 0; c < D; c++)
+    {
+        const int64_t o = (int64_t)i * D + c;
+        x[o] += alpha * p[o];
+        r[o] -= alpha * q[o];
+    }
+}
+
+// ... the start without z ...
+template <int D, typename TX, bool RESTART>
+__global__ __launch_bounds__(256) void k_cg_init_np(int n, const double *__restrict__ b, TX *__restrict__ r,
+                                                    TX *__restrict__ x, TX *__restrict__ p0, TX *__restrict__ q0)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n)
+        return;
+#pragma unroll
+    for (int c = 0; c < D; c++)
+    {
+        const int64_t o = (int64_t)i * D + c;
+        r[o] = (TX)b[o];
+        x[o] = 0;
+        if (RESTART)
+            p0[o] = 0, q0[o] = 0;
+    }
+}
+
+// ... and the partials of r.z and r.r once z = B r is there (same layout as the
+// block-Jacobi update's).
+template <int D, typename TX>
+__global__ __launch_bounds__(256) void k_cg_dots(int n, const TX *__restrict__ r, const TX *__restrict__ z,
+                                                 double *parts, const CgS *sc)
+{
+    __shared__ double sh[4];
+    if (sc && sc->done)
+        return;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    Acc<TX> rz, rr;
+    if (i < n)
+#pragma unroll
+        for (int c = 0; c < D; c++)
+        {
+            const int64_t o = (int64_t)i * D + c;
+            rz.add(r[o], z[o]);
+            rr.add(r[o], r[o]);
+        }
+    const double prz = rz.block(sh), prr = rr.block(sh);
+    if (threadIdx.x == 0)
+        parts[2 * blockIdx.x] = prz, parts[2 * blockIdx.x + 1] = prr;
+}
+
 // Solver scalars of a new solve from the init partials (one work-group).
 __global__ __launch_bounds__(256) void k_cg_start(const double *parts, int np, CgS *sc, double rtol2, double atol2,
                                                   int maxit)
@@ -468,6 +521,22 @@ struct Work
     bool have_ev = false;
 } W;
 
+// preconditioner of the solves: 0 block-Jacobi, 1 smoothed-aggregation AMG
+static int g_precond = 0;
+static int g_amg_epoch = -1;
+static int g_amg_last_iters = 8; // iterations of the last AMG-preconditioned CG segment
+
+extern "C" int lpmx_set_preconditioner(int kind)
+{
+    if (kind != 0 && kind != 1)
+        return lpmx_fail(LPMX_ERR_ARG, "preconditioner %d: 0 (block-Jacobi) or 1 (AMG)", kind);
+    g_precond = kind;
+    return LPMX_OK;
+}
+
+static int amg_apply(const double *r, double *z, const int *done) { return lpmx_amg_apply_f64(r, z, done); }
+static int amg_apply(const float *r, float *z, const int *done) { return lpmx_amg_apply_f32(r, z, done); }
+
 static int g_prof = 0;
 static double g_prof_ms = 0.0;
 static int64_t g_prof_n = 0;
@@ -520,8 +589,16 @@ static int cg_begin(const TM *Minv, TX *x, TX *r, TX *z, TX *q, TX *p, const dou
 {
     const int n = g.n;
     const int G = nblocks(n, 256);
-    hipLaunchKernelGGL((k_cg_init<D, TX, TM, RESTART>), dim3(G), dim3(256), 0, g_stream, n, b, r, x, p, q, z, Minv,
-                       W.parts);
+    if (g_precond == 1)
+    {
+        hipLaunchKernelGGL((k_cg_init_np<D, TX, RESTART>), dim3(G), dim3(256), 0, g_stream, n, b, r, x, p, q);
+        LAUNCHCHK();
+        RETCHK(amg_apply(r, z, nullptr));
+        hipLaunchKernelGGL((k_cg_dots<D, TX>), dim3(G), dim3(256), 0, g_stream, n, r, z, W.parts, nullptr);
+    }
+    else
+        hipLaunchKernelGGL((k_cg_init<D, TX, TM, RESTART>), dim3(G), dim3(256), 0, g_stream, n, b, r, x, p, q, z,
+                           Minv, W.parts);
     if (RESTART)
         hipLaunchKernelGGL(k_cg_start, dim3(1), dim3(256), 0, g_stream, W.parts, G, W.sc, rtol2, atol2, maxit);
     else
@@ -542,10 +619,14 @@ static int cg_iterate(const TV *Kv, const TM *Minv, TX *x, TX *r, TX *z, TX *q, 
     const int it0 = W.hsc->it;
     if (W.hsc->done)
         return LPMX_OK;
-    const int batch = n > 200000 ? 16 : 48;
+    // Iterations queued per host check.  An AMG-preconditioned solve takes tens of
+    // iterations of several milliseconds each: its first batch is about as long as the
+    // last solve, then two at a time, so hardly any V-cycle is queued after the stop.
+    int batch = g_precond == 1 ? std::max(1, g_amg_last_iters - 1) : n > 200000 ? 16 : 48;
     int t = 0, ev_used = 0;
     for (;;)
     {
+        const int it_before = W.hsc->it;
         for (int k = 0; k < batch; k++, t++)
         {
             const bool timed = g_prof && ev_used < 64;
@@ -559,18 +640,36 @@ static int cg_iterate(const TV *Kv, const TM *Minv, TX *x, TX *r, TX *z, TX *q, 
                 ev_used++;
             }
             hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(256), 0, g_stream, W.parts, G, W.sc);
-            hipLaunchKernelGGL((k_cg_update<D, TX, TM>), dim3(G), dim3(256), 0, g_stream, n, x, p, r, q, z, Minv,
-                               W.parts, W.sc);
+            if (g_precond == 1)
+            {
+                hipLaunchKernelGGL((k_cg_update_np<D, TX>), dim3(G), dim3(256), 0, g_stream, n, x, p, r, q, W.sc);
+                lpmx_amg_prof_tag(k);
+                RETCHK(amg_apply(r, z, &W.sc->done));
+                hipLaunchKernelGGL((k_cg_dots<D, TX>), dim3(G), dim3(256), 0, g_stream, n, r, z, W.parts, W.sc);
+            }
+            else
+                hipLaunchKernelGGL((k_cg_update<D, TX, TM>), dim3(G), dim3(256), 0, g_stream, n, x, p, r, q, z, Minv,
+                                   W.parts, W.sc);
             hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(256), 0, g_stream, W.parts, G, W.sc);
         }
         LAUNCHCHK();
         HIPCHK(hipMemcpyAsync(W.hsc, W.sc, sizeof(CgS), hipMemcpyDeviceToHost, g_stream));
         HIPCHK(hipStreamSynchronize(g_stream));
+        if (g_precond == 1)
+        {
+            RETCHK(lpmx_amg_prof_flush(W.hsc->it - it_before)); // V-cycles of iterations that ran
+            batch = 2;
+            if (getenv("LPMX_CG_TRACE"))
+                fprintf(stderr, "[lpmx] AMG-PCG it %d rr %.6e rz %.6e pq %.6e alpha %.6e beta %.6e done %d\n",
+                        W.hsc->it, W.hsc->rr, W.hsc->rz, W.hsc->pq, W.hsc->alpha, W.hsc->beta, W.hsc->done);
+        }
         if (W.hsc->done)
             break;
         if (t > (int64_t)maxit + 2 * batch) // the device flag must have been set by now
             return lpmx_fail(LPMX_ERR_STATE, "CG loop ran past its cap without a stop flag");
     }
+    if (g_precond == 1)
+        g_amg_last_iters = std::max(1, W.hsc->it - it0);
     if (g_prof)
         for (int e = 0; e < ev_used; e++)
         {
@@ -674,6 +773,14 @@ int lpmx_solve_into(int precision, double rtol2, double atol2, int maxit, double
     RETCHK(lpmx_require_pattern());
     RETCHK(ensure_work());
     RETCHK(lpmx_bj_inverse());
+    if (g_precond == 1 && g_amg_epoch != lpmx_tangent_epoch())
+    {
+        // a new tangent: keep the hierarchy if the pattern and constraints are those
+        // it was built for
+        if (!lpmx_amg_current())
+            RETCHK(lpmx_amg_build());
+        g_amg_epoch = lpmx_tangent_epoch();
+    }
     int it = 0, st = 0, passes = 1;
     double r2 = 0;
     if (precision == LPMX_MIXED)
@@ -736,6 +843,7 @@ extern "C" void lpmx_profile_spmv(int on)
     g_prof = on;
     g_prof_ms = 0.0;
     g_prof_n = 0;
+    lpmx_amg_prof_enable(on != 0);
 }
 
 extern "C" void lpmx_profile_spmv_read(double *total_ms, int64_t *launches)

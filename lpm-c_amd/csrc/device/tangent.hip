@@ -240,11 +240,15 @@ void lpmx_tangent_free(void)
     t_kstart = nullptr;
 }
 
+static int t_epoch = 0;
+int lpmx_tangent_epoch(void) { return t_epoch; }
+
 extern "C" int lpmx_tangent(void)
 {
     RETCHK(lpmx_require_pattern());
     RETCHK(ensure_tangent_buffers());
     t_nd_valid = false;
+    t_epoch++;
     hipLaunchKernelGGL(k_elastic_sums, dim3(nblocks(g.n)), dim3(LPMX_BLOCK), 0, g_stream, g, t_A, t_B);
     LAUNCHCHK();
     if (g.dim == 3)
@@ -602,6 +606,7 @@ extern "C" int lpmx_export_upper_csr(int *IK, int *JK, double *K)
 
 extern "C" int lpmx_import_upper_csr(const double *K)
 {
+    t_epoch++;
     RETCHK(lpmx_require_pattern());
     if (!K)
         return lpmx_fail(LPMX_ERR_ARG, "null K");

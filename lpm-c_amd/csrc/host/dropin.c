@@ -21,8 +21,10 @@
 #include "lpm_ext.h"
 
 static int d_have_model = 0, d_n = -1;
-static const double *ext_modulus = NULL;
-static const double *ext_crit = NULL;
+/* extension data of the current lattice, copied when set and dropped when a new
+ * lattice is created (searchNormalNeighbor) */
+static double *ext_modulus = NULL; /* [nparticle] or NULL */
+static double *ext_crit = NULL;    /* [nparticle * nneighbors] or NULL */
 static lpmx_solver ext_solver;
 static int ext_solver_set = 0;
 
@@ -275,6 +277,9 @@ static void create_model(void)
         lpm_dropin_push(LPMX_F_XYZ_INITIAL);
     d_have_model = 1;
     d_n = nparticle;
+    /* per-particle and per-bond extension data belonged to the previous lattice */
+    free(ext_modulus), free(ext_crit);
+    ext_modulus = ext_crit = NULL;
     push_params();
 }
 
@@ -462,6 +467,7 @@ static void solve(double rtol2, double atol2, int announce)
     int irmax = ext_solver_set ? ext_solver.ir_maxit : 100;
     int it = 0, st = 0;
     double rr = 0;
+    CHK("preconditioner", lpmx_set_preconditioner(ext_solver_set ? ext_solver.precond : 0));
     CHK("solverCG", lpmx_solve_ex(prec, rtol2, atol2, dim * nparticle, inner, irmax, &it, &rr, &st));
     if (announce)
     {
@@ -790,18 +796,36 @@ void updateCrack()
 }
 
 /* ------------------------------------------------------------------ extensions --- */
+/* A copy of count doubles, or NULL for a NULL source. */
+static int ext_copy(double **dst, const double *src, long count, const char *what)
+{
+    free(*dst);
+    *dst = NULL;
+    if (!src)
+        return LPMX_OK;
+    if (!d_have_model || d_n != nparticle || count <= 0)
+    {
+        fprintf(stderr, "%s: no lattice yet; call searchNormalNeighbor() first\n", what);
+        return LPMX_ERR_STATE;
+    }
+    *dst = malloc(sizeof(double) * count);
+    if (!*dst)
+        return LPMX_ERR_ARG;
+    memcpy(*dst, src, sizeof(double) * count);
+    return LPMX_OK;
+}
+
 int lpm_ext_set_modulus(const double *factor)
 {
-    ext_modulus = factor;
-    return LPMX_OK;
+    return ext_copy(&ext_modulus, factor, nparticle, "lpm_ext_set_modulus");
 }
 
 int lpm_ext_set_bond_critical(const double *crit)
 {
-    ext_crit = crit;
-    if (d_have_model)
-        return lpmx_set_bond_critical(crit);
-    return LPMX_OK;
+    int rc = ext_copy(&ext_crit, crit, (long)nparticle * nneighbors, "lpm_ext_set_bond_critical");
+    if (rc != LPMX_OK)
+        return rc;
+    return lpmx_set_bond_critical(ext_crit);
 }
 
 int lpm_ext_set_solver(const lpmx_solver *s)

@@ -50,6 +50,31 @@ def test_neighbor_lists_and_pattern_bit_exact(oracle, kind):
     np.testing.assert_array_equal(host2d("neighbors1", n, nb1, np.int32), m.nbr1)
 
 
+def test_modulus_field_belongs_to_its_lattice():
+    """lpm_ext_set_modulus copies the field and a new lattice drops it: a model built
+    after one with a modulus field has the plain constants (the harness's earlier
+    builds once left the library a pointer into freed memory)."""
+    L = P.lib()
+    box = P.sc_box_for(5)
+    emod = 1.0 + 0.05 * np.random.default_rng(3).uniform(-1, 1, 125)
+    n, _, _, _ = P.build_sc_model(box, emod=emod)
+    kn_field = host2d("Kn", n, 18)
+    del emod  # the library holds its own copy
+    L.calcKnTv()
+    np.testing.assert_array_equal(host2d("Kn", n, 18), kn_field)
+    # a new lattice without a field, against one built with an all-ones field
+    P.make_lattice(2, 0.5, [0.0, box, 0.0, box, 0.0, box])
+    L.searchNormalNeighbor()
+    L.searchAFEMNeighbor()
+    P.set_types(P.sc_types(P.positions()), 5)
+    P.set_materials([P.elastic_constants(200e3, 0.3)] * 5, 200.0)
+    L.calcKnTv()
+    kn_plain = host2d("Kn", n, 18)
+    assert not np.array_equal(kn_plain, kn_field)
+    P.build_sc_model(box, emod=np.ones(n))
+    np.testing.assert_array_equal(host2d("Kn", n, 18), kn_plain)
+
+
 def test_bond_constants_and_tangent(oracle):
     L = P.lib()
     _, n, x, t, m = sc_pair(oracle, 6)
@@ -608,3 +633,61 @@ def test_history_adjoint_central_differences_and_autograd():
         print(f"{name}: FD {f:.10e} adjoint {a:.10e} rel {abs(f - a) / abs(a):.2e}")
     for name, f, a in checks:
         assert abs(f - a) <= 2e-5 * abs(a), (name, f, a)
+
+
+# ------------------------------------------------------------------ AMG ------------
+@pytest.mark.parametrize("prec", [P.LPMX_FP64, P.LPMX_MIXED])
+def test_amg_solve_matches_block_jacobi(capfd, prec):
+    """The smoothed-aggregation AMG preconditioner (20^3 lattice: a hierarchy of three
+    levels) gives the block-Jacobi solution of the same constrained system, in far
+    fewer CG iterations; FP64 and the mixed form."""
+    import re
+
+    L = P.lib()
+    k = 20
+    n, x, t, c = P.build_sc_model(P.sc_box_for(k))
+    L.calcStiffness3DFiniteDifference(6)
+    dbc = np.ones(3 * n, np.int32)
+    dbc.reshape(n, 3)[t == 2, :] = 0
+    dbc.reshape(n, 3)[t == 3, :] = 0
+    dbc.reshape(n, 3)[t == 1, 2] = 0
+    P.write1d("dispBC_index", dbc)
+    r = np.random.default_rng(12).standard_normal(3 * n) * dbc
+    sols, iters = {}, {}
+    for pc in (0, 1):
+        s = P.solver(precision=prec, rtol2=1e-26, atol2=0.0, maxit=50 * 3 * n, precond=pc)
+        L.lpm_ext_set_solver(C.byref(s))
+        P.write1d("residual", r)
+        L.setDispBC_stiffnessUpdate3D()
+        capfd.readouterr()
+        L.solverCG()
+        out, err = capfd.readouterr()
+        m = re.search(r"solved after (\d+) iterations", out)
+        assert m, out + err
+        iters[pc] = int(m.group(1))
+        sols[pc] = P.read1d("disp", 3 * n)
+    L.lpm_ext_set_solver(None)
+    print("iterations block-Jacobi", iters[0], "AMG", iters[1])
+    assert np.abs(sols[1] - sols[0]).max() <= 1e-9 * np.abs(sols[0]).max()
+    assert iters[1] * 4 < iters[0]
+
+
+@pytest.mark.parametrize("prec", [P.LPMX_FP64, P.LPMX_MIXED])
+def test_load_steps_amg_match_oracle(oracle, prec):
+    """The fused loop with the AMG preconditioner (12^3: two levels) reaches the
+    oracle's converged states over an elastic and three plastic steps."""
+    k = 12
+    _, n, x, t, m = sc_pair(oracle, k)
+    steps = P.sc_uniaxial_steps(k - 1.0, np.array([0.5e-3, 1e-3, 1.5e-3, 2e-3]))
+    s = P.solver(precision=prec, rtol2=1e-24 if prec == P.LPMX_FP64 else 1e-16, atol2=0.0, maxit=100 * 3 * n,
+                 newton_tol=1e-11, newton_maxit=600, accel_depth=8, precond=1)
+    rc, out = P.run_steps(steps, s, out_axis="z", out_type=1)
+    assert rc == 0, (P.lib().lpmx_last_error(), [(o.status, o.newton_iters, o.cg_iters) for o in out])
+    print("AMG: Newton", [o.newton_iters for o in out], "CG", [o.cg_iters for o in out])
+    so = oracle.solver(rtol2=1e-24, atol2=0.0, maxit=100 * 3 * n, newton_tol=1e-11, newton_maxit=600, tangent=1,
+                       fixed_diag=1, accel=8, nthreads=8)
+    rco, outo = oracle.run_steps(m, steps, so, out_axis="z", out_type=1)
+    assert rco == 0
+    u, uo = get("XYZ", (n, 3)) - x, m.x - x
+    assert np.abs(u - uo).max() <= 1e-9 * np.abs(uo).max()
+    np.testing.assert_allclose([o.reaction for o in out], [o.reaction for o in outo], rtol=1e-8)
