@@ -6,17 +6,23 @@ per-particle modulus 1 + 0.05 U(-1, 1) (splitmix64, seed 0), uniaxial tension to
 in 50 load steps (bottom face held in z, 3-2-1 corner constraints, top face moved).
 
 One bench step is one load step of that schedule: `--warmup W` load steps run
-untimed, the next `--steps K` are timed (defaults: steps 1-2 untimed, 3-7 timed; step
+untimed, the next `--steps K` are timed (defaults: steps 1-2 untimed, 3-12 timed; step
 3 is the first plastic one).  Both precision forms run the same steps from a fresh
-model: the FP64 form (PCG in FP64) and the mixed form (FP32 matrix values, vectors
-and SpMV, compensated FP32 dots, FP64 iterative refinement); both use the
-reference's CG stopping rule |r|^2 <= 1e-8 |r0|^2 + 1e-12 and Anderson-accelerated
-modified Newton (depth 16) to 1e-11 relative residual, tight enough that the mixed
-form's displacements can be compared with the FP64 form's at the 1e-8 level.  The CG
-is preconditioned by a smoothed-aggregation AMG V-cycle (`--precond bj`: 3x3
-block-Jacobi); its hierarchy is built once per model and constraint set, on the first
-solve of the untimed steps, and its set-up time is reported.  `value` is the mixed
-form's load steps/s.
+model: the FP64 form (FP64 throughout) and the mixed form (FP32 matrix values,
+vectors, SpMV and V-cycle, compensated FP32 dots, FP64 residual); the Newton test
+(1e-11 relative residual, tight enough that the mixed form's displacements can be
+compared with the FP64 form's at the 1e-8 level), the return map and the state stay
+FP64 in both.  `--solver` picks the Newton loop:
+  vcycle (default)  one smoothed-aggregation AMG V-cycle per Newton iteration in
+                    place of a linear solve, the fixed-point map u + B R(u) mixed by
+                    Anderson(32);
+  amg-cg, bj-cg     the reference's structure: a PCG solve of every Newton correction
+                    to the reference's CG rule |r|^2 <= 1e-8 |r0|^2 + 1e-12, AMG- or
+                    3x3 block-Jacobi-preconditioned, Anderson(16) mixing.
+The AMG hierarchy is built once per model and constraint set, on the first solve of
+the untimed steps; its set-up time is reported.  `value` is the mixed form's load
+steps/s with `--solver`; `--compare` (default amg-cg) runs both forms again with a
+second configuration on the same steps and reports them under `comparison`.
 
 `--workload adjoint` measures the gradient instead: SC 60 x 60 x 60 = 216,000
 particles, the same material, 30 monotonic tension steps to 1.5% (damage off),
@@ -29,13 +35,14 @@ ratio, and, with --adjoint-fd, central differences (h = 1e-6 relative) of J alon
 any of E, H, the yield stress and a random direction of the modulus field (each the
 logarithmic derivative d J / d log theta, compared with the adjoint's).
 
-roofline: the dominant kernel, timed with HIP events on the library's stream during
-the timed steps; achieved = algorithmic bytes per launch / mean launch time.  With
-AMG that is the finest level's smoother SpMV (k_slice_mv, two launches per CG
-iteration; the CG SpMV k_cg_spmv is reported beside it), with block-Jacobi the CG
-SpMV.  cpu_baseline: the oracle's threaded CG (reference solverCG algorithm)
-on the same 1M matrix, a bounded number of iterations on the host cores, in
-DOF-iters/s.
+roofline: the dominant kernel of the main form (largest total time over the timed
+steps among the profiled ones: the J2 return map k_j2_particle, the finest level's
+smoother SpMV k_slice_mv, the CG SpMV k_cg_spmv), timed with HIP events on the
+library's stream; achieved = algorithmic bytes per launch / mean launch time.
+`rooflines` lists all profiled kernels.
+
+cpu_baseline: the oracle's threaded CG (reference solverCG algorithm) on the same 1M
+matrix, a bounded number of iterations on the host cores, in DOF-iters/s.
 """
 import argparse
 import ctypes as C
@@ -57,10 +64,18 @@ EDGE = 100
 BOX = 99.195  # SC box edge giving 100 particles per edge (radius 0.5)
 # linear solves: the reference's CG stopping rule |r|^2 <= 1e-8 |r0|^2 + 1e-12
 RTOL2, ATOL2 = 1e-8, 1e-12
-PRECOND_TEXT = {
-    "amg": "smoothed-aggregation AMG V-cycle (rigid-body modes, degree-1 Chebyshev block-Jacobi smoothing; "
-           "hierarchy built once per model and constraint set, before the timed steps)",
-    "bj": "block-Jacobi 3x3",
+AMG_TEXT = ("smoothed-aggregation AMG V-cycle (rigid-body modes, degree-1 Chebyshev block-Jacobi smoothing; "
+            "hierarchy built once per model and constraint set, before the timed steps)")
+# Newton-loop configurations: preconditioner 2 applies one V-cycle per Newton iteration
+# (no inner CG; Anderson mixing of the map u + B R(u)), 1 and 0 solve each Newton
+# correction by PCG to the reference's CG rule.
+SOLVERS = {
+    "vcycle": dict(precond=2, accel_depth=32, newton_maxit=1000,
+                   text="one AMG V-cycle per Newton iteration, Anderson(32) mixing; " + AMG_TEXT),
+    "amg-cg": dict(precond=1, accel_depth=16, newton_maxit=100,
+                   text="AMG-preconditioned CG per Newton correction, Anderson(16) mixing; " + AMG_TEXT),
+    "bj-cg": dict(precond=0, accel_depth=16, newton_maxit=100,
+                  text="block-Jacobi (3x3) preconditioned CG per Newton correction, Anderson(16) mixing"),
 }
 
 
@@ -79,19 +94,22 @@ def schedule(P):
     return P.sc_uniaxial_steps(EDGE - 1.0, strains)
 
 
-def run_form(P, prec, warmup, steps, profile, accel_depth, newton_tol, newton_maxit, precond):
+def run_form(P, prec, warmup, steps, profile, accel_depth, newton_tol, newton_maxit, solver):
     L = P.lib()
+    cfg = SOLVERS[solver]
+    accel_depth = cfg["accel_depth"] if accel_depth is None else accel_depth
+    newton_maxit = cfg["newton_maxit"] if newton_maxit is None else newton_maxit
     t0 = time.time()
     emod = P.uniform_modulus(EDGE ** 3, 0)
     n, x, _, _ = P.build_sc_model(BOX, emod=emod)
     log(f"form {prec}: model of {n} particles built in {time.time() - t0:.1f} s")
     sched = schedule(P)
-    amg = precond == "amg"
+    amg = cfg["precond"] > 0
     # mixed form: with block-Jacobi the FP32 residual is replaced by the FP64 one after
     # every 10x drop (long solves accumulate FP32 drift); an AMG solve is a few tens of
     # iterations, so its FP32 CG runs to the outer threshold in one segment
     s = P.solver(precision=prec, rtol2=RTOL2, atol2=ATOL2, maxit=20000, newton_tol=newton_tol, newton_maxit=newton_maxit,
-                 inner_rtol2=RTOL2 if amg else 1e-2, ir_maxit=100, accel_depth=accel_depth, precond=1 if amg else 0)
+                 inner_rtol2=RTOL2 if amg else 1e-2, ir_maxit=100, accel_depth=accel_depth, precond=cfg["precond"])
     if warmup:
         rc, wout = P.run_steps(sched[:warmup], s)
         P.check(rc, "warmup steps")
@@ -99,6 +117,7 @@ def run_form(P, prec, warmup, steps, profile, accel_depth, newton_tol, newton_ma
     sync()
     if profile:
         L.lpmx_profile_spmv(1)
+        L.lpmx_profile_force(1)
     t1 = time.perf_counter()
     out, rc = [], 0
     for k in range(warmup, warmup + steps):  # one call per load step, for progress lines
@@ -110,18 +129,22 @@ def run_form(P, prec, warmup, steps, profile, accel_depth, newton_tol, newton_ma
             f"reverted {o[0].accel_reverts}, residual {o[0].res_norm / max(o[0].tol_mult, 1e-300):.2e}")
     sync()
     wall = time.perf_counter() - t1
-    prof = prof_sm = None
+    prof = {}
     if profile:
-        ms, cnt = C.c_double(), C.c_int64()
+        ms, cnt, by = C.c_double(), C.c_int64(), C.c_double()
         L.lpmx_profile_spmv_read(C.byref(ms), C.byref(cnt))
-        by = C.c_double()
         P.check(L.lpmx_spmv_bytes(prec, C.byref(by)), "spmv bytes")
-        prof = (ms.value, cnt.value, by.value)
+        prof["k_cg_spmv"] = (ms.value, cnt.value, by.value)
         if amg:
             L.lpmx_profile_amg_read(C.byref(ms), C.byref(cnt))
             P.check(L.lpmx_amg_smoother_bytes(prec, C.byref(by)), "smoother bytes")
-            prof_sm = (ms.value, cnt.value, by.value)
+            prof["k_slice_mv"] = (ms.value, cnt.value, by.value)
+        P.check(L.lpmx_profile_force_read(C.byref(ms), C.byref(cnt)), "force profile")
+        P.check(L.lpmx_j2_particle_bytes(C.byref(by)), "j2 bytes")
+        prof["k_j2_particle"] = (ms.value, cnt.value, by.value)
         L.lpmx_profile_spmv(0)
+        L.lpmx_profile_force(0)
+        prof = {k: v for k, v in prof.items() if v[1] > 0}
     amg_info = None
     if amg:
         lev, t_setup, dev = C.c_int(), C.c_double(), C.c_int64()
@@ -131,7 +154,8 @@ def run_form(P, prec, warmup, steps, profile, accel_depth, newton_tol, newton_ma
         log(f"form {prec}: {L.lpmx_last_error().decode()} "
             f"{[(o.status, o.newton_iters, o.cg_iters) for o in out]}")
     u = P.field("XYZ", 3 * n).reshape(n, 3) - x
-    return dict(n=n, wall=wall, out=out, rc=rc, prof=prof, prof_sm=prof_sm, amg=amg_info, u=u)
+    return dict(n=n, wall=wall, out=out, rc=rc, prof=prof, amg=amg_info, u=u, solver=solver,
+                accel_depth=accel_depth, newton_maxit=newton_maxit)
 
 
 def sync():
@@ -142,6 +166,18 @@ def sync():
             torch.cuda.synchronize()
     except ImportError:
         pass
+
+
+def compare_forms(res, sm):
+    """Mixed against FP64 on the same steps: speed-up, displacement and reaction
+    differences."""
+    if not ("fp64" in res and "mixed" in res):
+        return {}
+    u64, um = res["fp64"]["u"], res["mixed"]["u"]
+    r64, rm = np.array(sm["fp64"]["reaction"]), np.array(sm["mixed"]["reaction"])
+    return {"mixed_speedup_vs_fp64": sm["mixed"]["load_steps_per_s"] / sm["fp64"]["load_steps_per_s"],
+            "mixed_disp_rel_err": float(np.abs(um - u64).max() / np.abs(u64).max()),
+            "mixed_reaction_rel_err": float(np.abs(rm - r64).max() / np.abs(r64).max())}
 
 
 def summarize(r, n):
@@ -164,7 +200,8 @@ def summarize(r, n):
     }
 
 
-TRAFFIC_FILES = {"k_cg_spmv": "k_cg_spmv_sc1m_hbm_traffic.md", "k_slice_mv": "k_slice_mv_sc1m_hbm_traffic.md"}
+TRAFFIC_FILES = {"k_cg_spmv": "k_cg_spmv_sc1m_hbm_traffic.md", "k_slice_mv": "k_slice_mv_sc1m_hbm_traffic.md",
+                 "k_j2_particle": "k_j2_particle_sc1m_hbm_traffic.md"}
 
 
 def measured_traffic(kernel, form):
@@ -175,7 +212,7 @@ def measured_traffic(kernel, form):
             f = line.split()
             if len(f) == 3 and f[0] == "traffic_bytes" and f[1] == form:
                 return float(f[2])
-    except OSError:
+    except (OSError, KeyError):
         pass
     return None
 
@@ -192,12 +229,11 @@ def roofline(prof, form, kernel="k_cg_spmv"):
             "traffic_source": f"profiles/{TRAFFIC_FILES[kernel]} (bytes per launch)"}
 
 
-def form_roofline(r, form):
-    """The dominant kernel of the form: with AMG the finest level's smoother SpMV
-    (k_slice_mv, two launches per CG iteration), else the CG SpMV (k_cg_spmv)."""
-    if r.get("prof_sm"):
-        return roofline(r["prof_sm"], form, "k_slice_mv")
-    return roofline(r["prof"], form) if r["prof"] else None
+def form_rooflines(r, form):
+    """Rooflines of the profiled kernels of a form, the dominant one (largest total
+    time over the timed steps) first."""
+    ks = sorted(r["prof"], key=lambda k: -r["prof"][k][0])
+    return [dict(roofline(r["prof"][k], form, k), total_ms=r["prof"][k][0]) for k in ks]
 
 
 def cpu_baseline(P, budget_s):
@@ -257,9 +293,19 @@ def adjoint_workload(P, a):
     steps = P.sc_uniaxial_steps(ADJ_EDGE - 1.0, strains)
     n0 = ADJ_EDGE ** 3
     prec = P.LPMX_FP64 if a.forms.split(",")[0].strip() == "fp64" else P.LPMX_MIXED
-    s = P.solver(precision=prec, rtol2=RTOL2, atol2=ATOL2, maxit=20000, newton_tol=a.newton_tol,
-                 newton_maxit=max(a.newton_maxit, 400), inner_rtol2=1e-2, ir_maxit=100, accel_depth=a.accel_depth)
-    opts = P.adjoint_opts(tol=a.adjoint_tol, maxit=400, depth=16, precision=prec, rtol2=RTOL2, cg_maxit=20000)
+    cfg = SOLVERS[a.solver]
+    depth = cfg["accel_depth"] if a.accel_depth is None else a.accel_depth
+    nmax = max(cfg["newton_maxit"] if a.newton_maxit is None else a.newton_maxit, 400)
+    amg = cfg["precond"] > 0
+    # no absolute floor on the CG test here: with it (|r|^2 <= 1e-12) a Newton test much
+    # below 1e-6 / |R0| would stall, every solve returning a zero correction
+    s = P.solver(precision=prec, rtol2=RTOL2, atol2=0.0, maxit=20000, newton_tol=a.newton_tol,
+                 newton_maxit=nmax, inner_rtol2=RTOL2 if amg else 1e-2, ir_maxit=100,
+                 accel_depth=depth, precond=cfg["precond"])
+    # the adjoint solves use the forward's preconditioner: with one V-cycle per
+    # iteration they take more (cheaper) iterations
+    opts = P.adjoint_opts(tol=a.adjoint_tol, maxit=4000 if cfg["precond"] == 2 else 400, depth=depth,
+                          precision=prec, rtol2=RTOL2, cg_maxit=20000)
     em0 = P.uniform_modulus(n0, 0)
     sy0 = np.full(n0, SY0)
 
@@ -300,8 +346,9 @@ def adjoint_workload(P, a):
         "config": {"workload": f"SC lattice {ADJ_EDGE}^3 = {n0} particles, J2, uniaxial tension to "
                                f"{100 * ADJ_STRAIN:.1f}% in {ADJ_STEPS} steps, damage off; J = sum (R - T)^2, "
                                f"T from modulus seed 2, E +2%, yield +5%, H -10%",
-                   "newton_tol": a.newton_tol, "newton_accel_depth": a.accel_depth, "ckpt_every": a.ckpt_every,
-                   "adjoint_tol": a.adjoint_tol, "adjoint_depth": 16, "linear_rtol2": RTOL2},
+                   "newton_tol": a.newton_tol, "newton_accel_depth": depth, "ckpt_every": a.ckpt_every,
+                   "adjoint_tol": a.adjoint_tol, "adjoint_depth": depth, "linear_rtol2": RTOL2, "linear_atol2": 0.0,
+                   "solver": a.solver, "solver_text": cfg["text"], "newton_maxit": nmax},
         "t_forward_s": t_fwd, "t_backward_s": st.t_backward, "t_recompute_s": st.t_recompute,
         "t_gradient_s": t_fwd + t_bwd,
         "forward_newton_iters": int(sum(o.newton_iters for o in out)),
@@ -347,7 +394,7 @@ def adjoint_workload(P, a):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -355,12 +402,17 @@ def main():
     ap.add_argument("--copy-probe", action="store_true")
     ap.add_argument("--newton-tol", type=float, default=1e-11,
                     help="Newton test |R| <= tol * max(|R0|, |reaction|) (reference TOLITER semantics)")
-    ap.add_argument("--newton-maxit", type=int, default=100,
-                    help="Newton cap (reference MAXITER = 100; the adjoint workload uses at least 400)")
-    ap.add_argument("--accel-depth", type=int, default=16,
-                    help="Anderson acceleration depth of the Newton loop (0: modified Newton)")
-    ap.add_argument("--precond", default="amg", choices=["amg", "bj"],
-                    help="CG preconditioner: smoothed-aggregation AMG or block-Jacobi (3x3 blocks)")
+    ap.add_argument("--newton-maxit", type=int, default=None,
+                    help="Newton cap (default per solver: 1000 for vcycle, the reference's MAXITER = 100 for the "
+                         "CG solvers; the adjoint workload uses at least 400)")
+    ap.add_argument("--accel-depth", type=int, default=None,
+                    help="Anderson acceleration depth of the Newton loop (0: modified Newton; default per solver)")
+    ap.add_argument("--solver", default="vcycle", choices=sorted(SOLVERS),
+                    help="Newton-loop configuration: vcycle (one AMG V-cycle per iteration), amg-cg or bj-cg "
+                         "(a PCG solve per Newton correction)")
+    ap.add_argument("--compare", default="amg-cg", choices=sorted(SOLVERS) + ["none"],
+                    help="a second configuration run on the same steps, both forms (the CG form of the "
+                         "mixed-vs-FP64 comparison); none to skip")
     ap.add_argument("--workload", default="loadsteps", choices=["loadsteps", "adjoint"])
     ap.add_argument("--ckpt-every", type=int, default=5, help="adjoint workload: checkpoint spacing")
     ap.add_argument("--adjoint-tol", type=float, default=1e-8, help="adjoint workload: per-step solve tolerance")
@@ -391,12 +443,13 @@ def main():
     for f in forms:
         prec = P.LPMX_FP64 if f == "fp64" else P.LPMX_MIXED
         res[f] = run_form(P, prec, a.warmup, a.steps, profile=True, accel_depth=a.accel_depth,
-                          newton_tol=a.newton_tol, newton_maxit=a.newton_maxit, precond=a.precond)
+                          newton_tol=a.newton_tol, newton_maxit=a.newton_maxit, solver=a.solver)
         log(f"{f}: {a.steps} steps in {res[f]['wall']:.2f} s")
     main_form = "mixed" if "mixed" in res else forms[0]
     r = res[main_form]
     n = r["n"]
     sm = {f: summarize(res[f], n) for f in res}
+    cfg = SOLVERS[a.solver]
     line = {
         "metric": METRIC,
         "value": sm[main_form]["load_steps_per_s"],
@@ -413,27 +466,33 @@ def main():
         "config": {"workload": f"SC lattice {EDGE}^3 = {n} particles ({3 * n} DOF), J2 elastoplastic, "
                                f"uniaxial tension to 2% in {NSTEPS_SCHEDULE} load steps, E +/-5% (seed 0); "
                                f"timed load steps {a.warmup + 1}..{a.warmup + a.steps}",
-                   "form": main_form, "newton_tol": a.newton_tol, "newton_maxit": a.newton_maxit, "linear_rtol2": RTOL2,
-                   "preconditioner": PRECOND_TEXT[a.precond], "newton_accel_depth": a.accel_depth},
+                   "form": main_form, "solver": a.solver, "solver_text": cfg["text"], "newton_tol": a.newton_tol,
+                   "newton_maxit": r["newton_maxit"], "newton_accel_depth": r["accel_depth"],
+                   "linear_rtol2": RTOL2},
         "dof_iters_per_s": sm[main_form]["dof_iters_per_s"],
         "forms": sm,
     }
-    if "fp64" in res and "mixed" in res:
-        u64, um = res["fp64"]["u"], res["mixed"]["u"]
-        line["mixed_speedup_vs_fp64"] = sm["mixed"]["load_steps_per_s"] / sm["fp64"]["load_steps_per_s"]
-        line["mixed_disp_rel_err"] = float(np.abs(um - u64).max() / np.abs(u64).max())
-        r64 = np.array(sm["fp64"]["reaction"])
-        rm = np.array(sm["mixed"]["reaction"])
-        line["mixed_reaction_rel_err"] = float(np.abs(rm - r64).max() / np.abs(r64).max())
-    line["roofline"] = form_roofline(r, main_form)
-    if r.get("prof_sm") and r["prof"]:
-        line["roofline_cg_spmv"] = roofline(r["prof"], main_form)
-    if "fp64" in res and main_form != "fp64":
-        line["roofline_fp64"] = form_roofline(res["fp64"], "fp64")
-        if res["fp64"].get("prof_sm") and res["fp64"]["prof"]:
-            line["roofline_cg_spmv_fp64"] = roofline(res["fp64"]["prof"], "fp64")
+    line.update(compare_forms(res, sm))
+    rl = form_rooflines(r, main_form) if r["prof"] else []
+    line["roofline"] = rl[0] if rl else None
+    line["rooflines"] = rl
+    if "fp64" in res and main_form != "fp64" and res["fp64"]["prof"]:
+        line["rooflines_fp64"] = form_rooflines(res["fp64"], "fp64")
     if r.get("amg"):
         line["amg"] = {f: res[f]["amg"] for f in res}
+    if a.compare != "none" and a.compare != a.solver and len(forms) == 2:
+        cres = {}
+        for f in forms:
+            prec = P.LPMX_FP64 if f == "fp64" else P.LPMX_MIXED
+            cres[f] = run_form(P, prec, a.warmup, a.steps, profile=True, accel_depth=None, newton_tol=a.newton_tol,
+                               newton_maxit=None, solver=a.compare)
+            log(f"{a.compare} {f}: {a.steps} steps in {cres[f]['wall']:.2f} s")
+        csm = {f: summarize(cres[f], n) for f in cres}
+        cmp_ = {"solver": a.compare, "solver_text": SOLVERS[a.compare]["text"], "forms": csm}
+        cmp_.update(compare_forms(cres, csm))
+        cmp_["rooflines"] = {f: form_rooflines(cres[f], f) for f in cres if cres[f]["prof"]}
+        cmp_["disp_rel_diff_vs_main"] = float(np.abs(cres[main_form]["u"] - r["u"]).max() / np.abs(r["u"]).max())
+        line["comparison"] = cmp_
     if not a.no_cpu:
         t0 = time.time()
         line["cpu_baseline"] = cpu_baseline(P, a.cpu_seconds)

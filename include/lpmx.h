@@ -201,13 +201,16 @@ typedef struct lpmx_solver
     double inner_rtol2;     /* mixed form: squared reduction of an FP32 segment between
                              * FP64 residual replacements (1e-2) */
     int ir_maxit;           /* mixed form: cap on residual replacements per solve */
-    int accel_depth;        /* 0: the reference's modified Newton; 1..16: Anderson
+    int accel_depth;        /* 0: the reference's modified Newton; 1..32: Anderson
                              * acceleration of it with that many differences (same
                              * fixed points, far fewer iterations in plastic flow) */
     int precond;            /* CG preconditioner: 0 block-Jacobi, 1 smoothed-aggregation
                              * AMG (V-cycle, block-Jacobi Chebyshev smoothing; hierarchy
                              * built on the first solve, kept while the pattern and the
-                             * constraint set stay) */
+                             * constraint set stay); 2 (fused loop only) one AMG V-cycle
+                             * per Newton iteration in place of the CG solve, the
+                             * fixed-point map u + B R(u) accelerated by Anderson mixing
+                             * (solverCG treats 2 as 1) */
 } lpmx_solver;
 
 typedef struct lpmx_step_stats
@@ -222,7 +225,8 @@ typedef struct lpmx_step_stats
 } lpmx_step_stats;
 
 /* Preconditioner of the following solves (lpmx_solve*, solverCG): 0 block-Jacobi
- * (default), 1 smoothed-aggregation AMG.  lpmx_run_steps sets it from its solver. */
+ * (default), 1 smoothed-aggregation AMG, 2 one AMG V-cycle instead of a solve.
+ * lpmx_run_steps sets it from its solver. */
 int lpmx_set_preconditioner(int kind);
 
 int lpmx_run_steps(const lpmx_solver *s, int nsteps, const int *nbd, const lpmx_dispbc *dbc,
@@ -248,8 +252,9 @@ typedef struct lpmx_adjoint_opts
                              * adjoint solve (J: consistent Jacobian of the free DOFs) */
     int maxit;              /* cap on the adjoint iterations of one step */
     int depth;              /* Anderson depth of the K-preconditioned fixed-point
-                             * iteration lambda <- lambda + K^-1 (b - J^T lambda), 0..16 */
-    int precision;          /* LPMX_FP64 or LPMX_MIXED for the K^-1 solves */
+                             * iteration lambda <- lambda + K^-1 (b - J^T lambda), 0..32 */
+    int precision;          /* LPMX_FP64 or LPMX_MIXED for the K^-1 solves (which use the
+                             * forward solver's preconditioner and mixed-form segments) */
     double rtol2;           /* CG test of the K^-1 solves (squared, relative) */
     int cg_maxit;           /* their cap */
 } lpmx_adjoint_opts;
@@ -306,6 +311,14 @@ int lpmx_amg_smoother_bytes(int precision, double *bytes);
 /* The current AMG hierarchy: levels (0 when none), host set-up time of its last
  * build, device memory it holds. */
 int lpmx_amg_info(int *levels, double *setup_seconds, int64_t *device_bytes);
+/* HIP-event timing of the J2 return-map kernel (k_j2_particle, one launch per force
+ * evaluation) while on; switching on resets the totals. */
+void lpmx_profile_force(int on);
+int lpmx_profile_force_read(double *total_ms, int64_t *launches);
+/* Algorithmic HBM bytes of one such launch: n*(4+4+24+24+48) read and n*(32+48+16+4)
+ * written per particle, nbond*(4+4+48) read and nbond*40 written per bond, and the
+ * slot-2 plastic stretch of all n*nneighbors list slots (8 B each). */
+int lpmx_j2_particle_bytes(double *bytes);
 
 #ifdef __cplusplus
 }

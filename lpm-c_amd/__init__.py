@@ -144,6 +144,8 @@ def lib():
         "lpmx_profile_amg_read": [C.POINTER(C.c_double), C.POINTER(C.c_int64)],
         "lpmx_amg_smoother_bytes": [C.c_int, C.POINTER(C.c_double)],
         "lpmx_amg_info": [_ip, C.POINTER(C.c_double), C.POINTER(C.c_int64)],
+        "lpmx_profile_force_read": [C.POINTER(C.c_double), C.POINTER(C.c_int64)],
+        "lpmx_j2_particle_bytes": [C.POINTER(C.c_double)],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -156,6 +158,8 @@ def lib():
     L.lpmx_profile_spmv_read.argtypes = [_dp, C.POINTER(C.c_int64)]
     L.lpmx_profile_spmv_read.restype = None
     L.lpmx_profile_amg_read.restype = None
+    L.lpmx_profile_force.argtypes = [C.c_int]
+    L.lpmx_profile_force.restype = None
     L.lpm_ext_set_modulus.argtypes = [_dp]
     L.lpm_ext_set_bond_critical.argtypes = [_dp]
     L.lpm_ext_set_solver.argtypes = [C.POINTER(Solver)]

@@ -24,7 +24,7 @@
 
 #include "common.h"
 
-#define AA_MAX 16
+#define AA_MAX 32
 
 struct AaPtrs
 {

@@ -422,6 +422,66 @@ extern "C" int lpmx_computedL(void)
     return LPMX_OK;
 }
 
+// HIP-event timing of the k_j2_particle launches while on: each launch's pair is
+// read at the next launch (or at the read), after its end event has completed.
+static struct ForceProf
+{
+    bool on = false, have_ev = false, pending = false;
+    hipEvent_t e0, e1;
+    double ms = 0.0;
+    int64_t n = 0;
+} FP;
+
+static int force_prof_harvest(void)
+{
+    if (!FP.pending)
+        return LPMX_OK;
+    HIPCHK(hipEventSynchronize(FP.e1));
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, FP.e0, FP.e1));
+    FP.ms += ms, FP.n++;
+    FP.pending = false;
+    return LPMX_OK;
+}
+
+extern "C" void lpmx_profile_force(int on)
+{
+    FP.on = on != 0, FP.pending = false, FP.ms = 0.0, FP.n = 0;
+    if (FP.on && !FP.have_ev)
+    {
+        hipEventCreate(&FP.e0), hipEventCreate(&FP.e1);
+        FP.have_ev = true;
+    }
+}
+
+extern "C" int lpmx_profile_force_read(double *total_ms, int64_t *launches)
+{
+    RETCHK(force_prof_harvest());
+    *total_ms = FP.ms;
+    *launches = FP.n;
+    return LPMX_OK;
+}
+
+// Algorithmic HBM bytes of one k_j2_particle launch: per particle its own position,
+// counts, yield stress, hardening state and slot-0 back stress read, its shell sums,
+// slot-2 state and flag written; per bond the partner index and sign, L0, slot-0
+// plastic stretch, both damage factors, Kn, Tv read and dL, the direction cosines,
+// ddLp and the slot-2 plastic stretch written (gathered partner positions are
+// re-reads from cache); the slot-2 plastic stretch of every list slot.
+extern "C" int lpmx_j2_particle_bytes(double *bytes)
+{
+    RETCHK(lpmx_require_model());
+    std::vector<int> nb0(g.n);
+    HIPCHK(hipMemcpyAsync(nb0.data(), g.nb0, sizeof(int) * (size_t)g.n, hipMemcpyDeviceToHost, g_stream));
+    HIPCHK(hipStreamSynchronize(g_stream));
+    int64_t nbond = 0;
+    for (int c : nb0)
+        nbond += c;
+    *bytes = (double)g.n * (4 + 4 + 24 + 24 + 48) + (double)g.n * (32 + 48 + 16 + 4) + (double)nbond * (4 + 4 + 6 * 8) +
+             (double)nbond * (5 * 8) + (double)g.n * g.nb * 8;
+    return LPMX_OK;
+}
+
 extern "C" int lpmx_bond_force_range(int mode, int first, int count)
 {
     RETCHK(lpmx_require_model());
@@ -433,8 +493,18 @@ extern "C" int lpmx_bond_force_range(int mode, int first, int count)
     if (mode == 0)
     {
         // the final force of a particle needs its partners' return maps too
+        if (FP.on)
+        {
+            RETCHK(force_prof_harvest());
+            HIPCHK(hipEventRecord(FP.e0, g_stream));
+        }
         hipLaunchKernelGGL(k_j2_particle, dim3(nblocks(g.n)), B, 0, g_stream, g, 0, g.n);
         LAUNCHCHK();
+        if (FP.on)
+        {
+            HIPCHK(hipEventRecord(FP.e1, g_stream));
+            FP.pending = true;
+        }
         hipLaunchKernelGGL(k_j2_force, G, B, 0, g_stream, g, first, count);
     }
     else if (mode == 1)

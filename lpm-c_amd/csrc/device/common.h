@@ -219,6 +219,7 @@ int lpmx_amg_levels(void);
 double lpmx_amg_setup_seconds(void);
 void lpmx_amg_free(void);
 int lpmx_tangent_epoch(void); // increments with every tangent assembly (tangent.hip)
+int lpmx_bc_eliminate_repeat(void); // residual-only repeat of lpmx_bc_eliminate_ex(1) (tangent.hip)
 // HIP-event timing of the finest level's smoother SpMV launches: the CG tags each
 // V-cycle with its iteration in the batch and, after the batch, keeps the launches
 // of the iterations that ran

@@ -414,7 +414,8 @@ static int reverse_step(Rev &R, int cur, const lpmx_adjoint_opts *o, double Rbar
             return lpmx_fail(LPMX_ERR_NOCONV, "adjoint solve: %d iterations left |r|/|b| = %.3e", it, sqrt(rr / bb));
         int cg = 0, stt = 0;
         double r2 = 0.0;
-        RETCHK(lpmx_solve_into(o->precision, o->rtol2, 0.0, o->cg_maxit, 1e-2, 100, R.r, R.z, &cg, &r2, &stt));
+        RETCHK(lpmx_solve_into(o->precision, o->rtol2, 0.0, o->cg_maxit, H.s.inner_rtol2, H.s.ir_maxit, R.r, R.z,
+                               &cg, &r2, &stt));
         if (stt == 3)
             return lpmx_fail(LPMX_ERR_NOCONV, "adjoint preconditioner solve broke down");
         st->adj_cg_iters += cg;
@@ -457,9 +458,11 @@ extern "C" int lpmx_history_backward(const lpmx_adjoint_opts *o, const double *R
     RETCHK(lpmx_require_pattern());
     if (!H.valid || H.n != g.n || H.nb != g.nb)
         return lpmx_fail(LPMX_ERR_STATE, "no recorded history for this model (lpmx_history_forward first)");
-    if (!o || !Rbar || !g_emod || !g_sigmay || !g_H || o->depth < 0 || o->depth > 16 || o->maxit < 0 ||
+    if (!o || !Rbar || !g_emod || !g_sigmay || !g_H || o->depth < 0 || o->depth > 32 || o->maxit < 0 ||
         !(o->tol > 0.0))
         return lpmx_fail(LPMX_ERR_ARG, "invalid arguments to lpmx_history_backward");
+    // the K^-1 solves use the forward's preconditioner and mixed-form segments
+    RETCHK(lpmx_set_preconditioner(H.s.precond));
     lpmx_adjoint_stats S;
     memset(&S, 0, sizeof(S));
     S.t_forward = H.t_forward;

@@ -71,7 +71,9 @@ int lpmx_run_steps_impl(const lpmx_solver *s, int nsteps, const int *nbd, const 
             while (nres > s->newton_tol * tolm && ni < s->newton_maxit)
             {
                 RETCHK(lpmx_switch_state(0));
-                RETCHK(lpmx_bc_eliminate_ex(1));
+                // K and the constraints stay within the loop: the first pass eliminates,
+                // the later ones only zero the constrained residual
+                RETCHK(ni == 0 ? lpmx_bc_eliminate_ex(1) : lpmx_bc_eliminate_repeat());
                 double ts = now_s();
                 int it = 0, stt = 0;
                 double rr = 0;

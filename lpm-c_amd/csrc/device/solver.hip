@@ -528,8 +528,8 @@ static int g_amg_last_iters = 8; // iterations of the last AMG-preconditioned CG
 
 extern "C" int lpmx_set_preconditioner(int kind)
 {
-    if (kind != 0 && kind != 1)
-        return lpmx_fail(LPMX_ERR_ARG, "preconditioner %d: 0 (block-Jacobi) or 1 (AMG)", kind);
+    if (kind < 0 || kind > 2)
+        return lpmx_fail(LPMX_ERR_ARG, "preconditioner %d: 0 (block-Jacobi), 1 (AMG) or 2 (one AMG V-cycle)", kind);
     g_precond = kind;
     return LPMX_OK;
 }
@@ -764,6 +764,38 @@ static int solve_mixed(const double *b, double *x, double rtol2, double atol2, d
 static int g_last_passes = 0;
 int lpmx_last_ir_passes(void) { return g_last_passes; }
 
+__global__ void k_to_f32(int64_t m, const double *__restrict__ a, float *__restrict__ b)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < m)
+        b[i] = (float)a[i];
+}
+
+__global__ void k_to_f64(int64_t m, const float *__restrict__ a, double *__restrict__ b)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < m)
+        b[i] = (double)a[i];
+}
+
+// x = B b with one AMG V-cycle and no Krylov iteration (preconditioner 2: the Newton
+// loop's fixed-point map becomes u + B R(u), accelerated by Anderson mixing, instead
+// of u + K^-1 R(u) with an inner CG per iteration).  FP32 cycle in the mixed form.
+static int vcycle_only(int precision, const double *b, double *x)
+{
+    const int64_t m = (int64_t)g.n * g.dim;
+    if (precision == LPMX_MIXED)
+    {
+        hipLaunchKernelGGL(k_to_f32, dim3(nblocks(m)), dim3(LPMX_BLOCK), 0, g_stream, m, b, W.r32);
+        LAUNCHCHK();
+        RETCHK(lpmx_amg_apply_f32(W.r32, W.z32, nullptr));
+        hipLaunchKernelGGL(k_to_f64, dim3(nblocks(m)), dim3(LPMX_BLOCK), 0, g_stream, m, W.z32, x);
+        LAUNCHCHK();
+        return LPMX_OK;
+    }
+    return lpmx_amg_apply_f64(b, x, nullptr);
+}
+
 // K x = b on the current (constraint-eliminated) matrix: block-Jacobi inverse, then
 // PCG (FP64 or mixed).  b and x are DOF vectors; status: 1 converged, 2 cap,
 // 3 breakdown.
@@ -773,7 +805,7 @@ int lpmx_solve_into(int precision, double rtol2, double atol2, int maxit, double
     RETCHK(lpmx_require_pattern());
     RETCHK(ensure_work());
     RETCHK(lpmx_bj_inverse());
-    if (g_precond == 1 && g_amg_epoch != lpmx_tangent_epoch())
+    if (g_precond >= 1 && g_amg_epoch != lpmx_tangent_epoch())
     {
         // a new tangent: keep the hierarchy if the pattern and constraints are those
         // it was built for
@@ -783,7 +815,12 @@ int lpmx_solve_into(int precision, double rtol2, double atol2, int maxit, double
     }
     int it = 0, st = 0, passes = 1;
     double r2 = 0;
-    if (precision == LPMX_MIXED)
+    if (g_precond == 2)
+    {
+        RETCHK(vcycle_only(precision, b, x));
+        it = 1, st = 1;
+    }
+    else if (precision == LPMX_MIXED)
     {
         if (g.dim == 3)
             RETCHK(solve_mixed<3>(b, x, rtol2, atol2, inner_rtol2, ir_maxit, maxit, &it, &r2, &st, &passes));

@@ -238,6 +238,7 @@ void lpmx_tangent_free(void)
         hipFree(t_kstart);
     t_A = t_B = nullptr;
     t_kstart = nullptr;
+    t_nd_valid = false;
 }
 
 static int t_epoch = 0;
@@ -387,6 +388,33 @@ extern "C" int lpmx_bc_eliminate_ex(int fixed_diag)
 }
 
 extern "C" int lpmx_bc_eliminate(void) { return lpmx_bc_eliminate_ex(0); }
+
+template <int D>
+__global__ void k_bc_res(DV v)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= v.n)
+        return;
+    for (int c = 0; c < D; c++)
+        if (dof_fixed(v, i, c))
+            v.res[(int64_t)i * D + c] = 0.0;
+}
+
+// A repeated elimination with the fixed diagonal on an unchanged tangent and
+// constraint set leaves K as it is: only the residual of the constrained DOFs is
+// zeroed (the fused loop's Newton iterations after the first of a tangent).
+int lpmx_bc_eliminate_repeat(void)
+{
+    RETCHK(lpmx_require_pattern());
+    if (!t_nd_valid)
+        return lpmx_bc_eliminate_ex(1);
+    if (g.dim == 3)
+        hipLaunchKernelGGL(k_bc_res<3>, dim3(nblocks(g.n)), dim3(LPMX_BLOCK), 0, g_stream, g);
+    else
+        hipLaunchKernelGGL(k_bc_res<2>, dim3(nblocks(g.n)), dim3(LPMX_BLOCK), 0, g_stream, g);
+    LAUNCHCHK();
+    return LPMX_OK;
+}
 
 // ------------------------------------------------------ block-Jacobi inverse -----
 template <int D>

@@ -467,7 +467,8 @@ static void solve(double rtol2, double atol2, int announce)
     int irmax = ext_solver_set ? ext_solver.ir_maxit : 100;
     int it = 0, st = 0;
     double rr = 0;
-    CHK("preconditioner", lpmx_set_preconditioner(ext_solver_set ? ext_solver.precond : 0));
+    /* a solve to tolerance: the single-V-cycle mode (2) of the fused loop means AMG here */
+    CHK("preconditioner", lpmx_set_preconditioner(ext_solver_set ? (ext_solver.precond ? 1 : 0) : 0));
     CHK("solverCG", lpmx_solve_ex(prec, rtol2, atol2, dim * nparticle, inner, irmax, &it, &rr, &st));
     if (announce)
     {

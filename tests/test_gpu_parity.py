@@ -691,3 +691,72 @@ def test_load_steps_amg_match_oracle(oracle, prec):
     u, uo = get("XYZ", (n, 3)) - x, m.x - x
     assert np.abs(u - uo).max() <= 1e-9 * np.abs(uo).max()
     np.testing.assert_allclose([o.reaction for o in out], [o.reaction for o in outo], rtol=1e-8)
+
+
+# ----------------------------------------------- one V-cycle per Newton iteration --
+@pytest.mark.parametrize("prec", [P.LPMX_FP64, P.LPMX_MIXED])
+@pytest.mark.parametrize("case", ["j2", "hex", "brittle", "cp"])
+def test_vcycle_newton_matches_oracle(oracle, case, prec):
+    """Preconditioner 2 of the fused loop: one AMG V-cycle per Newton iteration in
+    place of the linear solve, the map u + B R(u) mixed by Anderson(32).  It reaches
+    the oracle's equilibria: J2 tension (12^3, one elastic and three plastic steps),
+    2D elastic hexagonal shear (900 particles), the brittle hexagonal shear with
+    Weibull bonds (broken-bond sets bit-exact) and FCC crystal plasticity through first
+    slip (active-slip sets bit-exact).  All but the brittle case (196 particles) have
+    hierarchies of at least two levels."""
+    tol_u = 1e-9 if prec == P.LPMX_FP64 else 1e-8
+    s = None
+    if case == "j2":
+        k = 12
+        _, n, x, t, m = sc_pair(oracle, k)
+        steps = P.sc_uniaxial_steps(k - 1.0, np.array([0.5e-3, 1e-3, 1.5e-3, 2e-3]))
+        axis, dim, accel = "z", 3, 8
+    elif case == "hex":
+        box = [0.0, 30.0, 0.0, 26.0, 0.0, 1.0]  # 900 particles: a two-level 2D hierarchy
+        n, x, t, c = P.build_hex_model(box, critical=1.0, nbreak=20)  # no bond reaches 1.0
+        m = oracle_model(oracle, x, 1, t, [c] * 3, plmode=6, sigmay=1e30, critical_bstrain=1.0, nbreak=20)
+        height = float(x[:, 1].max() - x[:, 1].min())
+        steps = P.hex_shear_steps(height, np.linspace(4e-4, 1.2e-3, 3))
+        axis, dim, accel = "x", 2, 0
+    elif case == "brittle":
+        box = [0.0, 14.0, 0.0, 12.0, 0.0, 1.0]  # 196 particles (the hierarchy is its dense level)
+        n, x, t, c = P.build_hex_model(box, critical=1e-3, nbreak=20)
+        nb = P.gint("nneighbors").value
+        crit = P.weibull_bond_critical(get("NEIGHBORS", (n, nb)), 1, shape=10.0, scale=1e-3)
+        P.build_hex_model(box, critical=1e-3, nbreak=20, crit=crit)
+        m = oracle_model(oracle, x, 1, t, [c] * 3, plmode=6, sigmay=1e30, critical_bstrain=1e-3, nbreak=20,
+                         crit=crit)
+        height = float(x[:, 1].max() - x[:, 1].min())
+        steps = P.hex_shear_steps(height, np.linspace(4e-4, 4e-3, 10))
+        axis, dim, accel = "x", 2, 0
+    else:
+        n, x, t, c, m = cp_pair(oracle, 6.5)  # 500 particles: a two-level hierarchy
+        height = float(x[:, 2].max() - x[:, 2].min())
+        steps = P.sc_uniaxial_steps(height, np.array([0.5, 1.0, 1.3]) * (CP["tau0"] / (c[2] * 0.8)))
+        axis, dim, accel = "z", 3, 5
+        # past first slip the stiffness drops (slip hardening h0 = 100 MPa): both loops
+        # stop at 1e-11 in the residual, which leaves ~3e-9 in the displacements
+        tol_u = 1e-8
+    s = P.solver(precision=prec, rtol2=1e-8, atol2=0.0, maxit=100 * dim * n, newton_tol=1e-11, newton_maxit=4000,
+                 accel_depth=32, precond=2)
+    rc, out = P.run_steps(steps, s, out_axis=axis, out_type=1)
+    assert rc == 0, (P.lib().lpmx_last_error(), [(o.status, o.newton_iters) for o in out])
+    print(case, "V-cycle Newton iterations", [o.newton_iters for o in out])
+    so = oracle.solver(rtol2=1e-24, atol2=0.0, maxit=100 * dim * n, newton_tol=1e-11, newton_maxit=800, tangent=1,
+                       fixed_diag=1, accel=accel)
+    rco, outo = oracle.run_steps(m, steps, so, out_axis=axis, out_type=1)
+    assert rco == 0
+    if case == "brittle":
+        nb = P.gint("nneighbors").value
+        assert [o.broken for o in out] == [o.broken for o in outo]
+        assert sum(o.broken for o in outo) > 0
+        np.testing.assert_array_equal(get("DAMAGE_BROKEN", (n, nb)), m.dmg_broken)
+    if case == "cp":
+        assert m.s.cp_fail == 0
+        jact = get("CP_JACT", (n, 24))
+        assert jact.any()
+        np.testing.assert_array_equal(jact, m.cp_Jact)
+    r, ro = np.array([o.reaction for o in out]), np.array([o.reaction for o in outo])
+    assert np.abs(r - ro).max() <= 1e-8 * np.abs(ro).max()
+    u, uo = get("XYZ", (n, 3)) - x, m.x - x
+    assert np.abs(u - uo).max() <= tol_u * np.abs(uo).max()
