@@ -94,7 +94,6 @@ __global__ void k_aa_push(int n, int dim, int xs, const double *__restrict__ x, 
 __global__ __launch_bounds__(256) void k_aa_dots(int64_t m, AaPtrs P, int cnt, int s, const double *__restrict__ d,
                                                  double *parts)
 {
-    __shared__ double sh[4];
     double g[AA_MAX], h[AA_MAX];
 #pragma unroll
     for (int j = 0; j < AA_MAX; j++)
@@ -111,12 +110,26 @@ __global__ __launch_bounds__(256) void k_aa_dots(int64_t m, AaPtrs P, int cnt, i
                 h[j] += fj * dt;
             }
     }
-    for (int j = 0; j < cnt; j++)
+    // wave sums of all 2 cnt values first, then one exchange through LDS: the
+    // work-group totals in wave order (fixed, so deterministic)
+    __shared__ double wsum[4][2 * AA_MAX];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int j = 0; j < AA_MAX; j++)
+        if (j < cnt)
+        {
+            const double a = wave_sum(g[j]), b = wave_sum(h[j]);
+            if (lane == 0)
+                wsum[w][j] = a, wsum[w][AA_MAX + j] = b;
+        }
+    __syncthreads();
+    const int q = threadIdx.x;
+    if (q < 2 * AA_MAX && (q % AA_MAX) < cnt)
     {
-        const double a = block_sum(g[j], sh);
-        const double b = block_sum(h[j], sh);
-        if (threadIdx.x == 0)
-            parts[(int64_t)blockIdx.x * 2 * AA_MAX + j] = a, parts[(int64_t)blockIdx.x * 2 * AA_MAX + AA_MAX + j] = b;
+        double t = 0.0;
+        for (int k = 0; k < (int)(blockDim.x >> 6); k++)
+            t += wsum[k][q];
+        parts[(int64_t)blockIdx.x * 2 * AA_MAX + q] = t;
     }
 }
 

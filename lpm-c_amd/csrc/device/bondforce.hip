@@ -215,6 +215,175 @@ __global__ void k_j2_particle(DV v, int first, int count)
     v.dlam[i] = dlam;
 }
 
+// k_j2_particle with the bond loops unrolled to NBM list slots and every array a
+// separate restrict-qualified argument: the loads of all bonds can then be issued
+// ahead of the stores (the generic kernel's pointers may alias, which serialises the
+// dependent index and position loads of one bond after another).  Same arithmetic in
+// the same order; the second stretch pass keeps the distances of the first in
+// registers and does not rewrite the unchanged direction cosines.
+template <int NBM>
+__global__ __launch_bounds__(256) void k_j2_particle_u(
+    int n, int nbmax, double volume, double H, double xi, const int *__restrict__ nb0_,
+    const int *__restrict__ nbcur, const int *__restrict__ nbr, const int *__restrict__ nsign,
+    const int *__restrict__ opp, const double *__restrict__ X, const double *__restrict__ L0,
+    const double *__restrict__ dLp0, const double *__restrict__ db, const double *__restrict__ dw,
+    const double *__restrict__ Kn, const double *__restrict__ Tv, const double *__restrict__ beta0,
+    const double *__restrict__ alpha0, const double *__restrict__ sigmay, const double *__restrict__ c44,
+    double *__restrict__ dL, double *__restrict__ csx, double *__restrict__ csy, double *__restrict__ csz,
+    double *__restrict__ ddLp, double *__restrict__ dLp2, double *__restrict__ A, double *__restrict__ Bt,
+    double *__restrict__ beta2, double *__restrict__ alpha2, double *__restrict__ dlam_, int *__restrict__ pl_flag)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const int nb0 = nb0_[i];
+    const double xi0 = X[3 * i], xi1 = X[3 * i + 1], xi2 = X[3 * i + 2];
+    double dis[NBM];
+    // first stretch pass (constitutive.c:495-515)
+    double a0 = 0, a1 = 0, b0 = 0, b1 = 0;
+#pragma unroll
+    for (int j = 0; j < NBM; j++)
+        if (j < nb0)
+        {
+            const int64_t b = (int64_t)j * n + i;
+            const int q = nbr[b];
+            const double dx = xi0 - X[3 * q], dy = xi1 - X[3 * q + 1], dz = xi2 - X[3 * q + 2];
+            const double d = sqrt(dx * dx + dy * dy + dz * dz);
+            dis[j] = d;
+            double dl = d - L0[b];
+            dl -= dLp0[b];
+            dl *= db[b];
+            dL[b] = dl;
+            if (nsign[b] == 0)
+                a0 += dl, b0 += Tv[b] * dl;
+            else
+                a1 += dl, b1 += Tv[b] * dl;
+            csx[b] = dx / d;
+            csy[b] = dy / d;
+            csz[b] = dz / d;
+        }
+    // trial stress from the particle's own sums (constitutive.c:518-568)
+    const bool full = nbcur[i] == nbmax;
+    double sl[6] = {0, 0, 0, 0, 0, 0}, dep[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int j = 0; j < NBM; j++)
+        if (j < nb0)
+        {
+            const int64_t b = (int64_t)j * n + i;
+            const int s = nsign[b];
+            double Fij = 2.0 * Kn[b] * dL[b] + (s == 0 ? b0 : b1) + Tv[b] * (s == 0 ? a0 : a1);
+            Fij *= dw[b];
+            double of = 0.5;
+            if (!full)
+            {
+                const int o = opp[b];
+                of = o < 0 ? 1.0 : (db[(int64_t)o * n + i] <= LPMX_EPS ? 1.0 : 0.5);
+            }
+            const double cx = csx[b], cy = csy[b], cz = csz[b], l0 = L0[b];
+            sl[0] += of / volume * l0 * Fij * cx * cx;
+            sl[1] += of / volume * l0 * Fij * cy * cy;
+            sl[2] += of / volume * l0 * Fij * cz * cz;
+            sl[3] += of / volume * l0 * Fij * cy * cz;
+            sl[4] += of / volume * l0 * Fij * cx * cz;
+            sl[5] += of / volume * l0 * Fij * cx * cy;
+        }
+    // radial return with linear mixed hardening (constitutive.c:571-608)
+    const double tmp = 1.0 / 3.0 * (sl[0] + sl[1] + sl[2]);
+    for (int c = 0; c < 3; c++)
+        sl[c] -= tmp;
+    double beta[6];
+    for (int c = 0; c < 6; c++)
+    {
+        beta[c] = beta0[(int64_t)c * n + i];
+        sl[c] -= beta[c];
+    }
+    double seq = 0.0;
+    for (int c = 0; c < 6; c++)
+    {
+        if (c < 3)
+            seq += sl[c] * sl[c];
+        else
+            seq += 2.0 * sl[c] * sl[c];
+    }
+    seq = sqrt(3.0 / 2.0 * seq);
+    double alpha = alpha0[i], dlam = 0.0;
+    const double yf = seq - (sigmay[i] + (1.0 - xi) * H * alpha);
+    if (yf > 0.0)
+    {
+        pl_flag[i] = 1;
+        dlam = yf / (3 * c44[i] + H);
+    }
+    alpha += dlam;
+    for (int c = 0; c < 6; c++)
+        if (fabs(seq) > LPMX_EPS)
+        {
+            dep[c] = dlam * 1.5 * sl[c] / seq;
+            beta[c] += 2. / 3. * xi * H * dep[c];
+        }
+    // plastic bond stretch (constitutive.c:611-621) over every list slot
+    double xp[NBM];
+#pragma unroll
+    for (int j = 0; j < NBM; j++)
+        if (j < nbmax)
+        {
+            const int64_t b = (int64_t)j * n + i;
+            double x_ = dLp0[b];
+            if (j < nb0)
+            {
+                const double cx = csx[b], cy = csy[b], cz = csz[b];
+                double d = L0[b] * (dep[0] * cx * cx + dep[1] * cy * cy + dep[2] * cz * cz + 2 * dep[3] * cy * cz +
+                                    2 * dep[4] * cx * cz + 2 * dep[5] * cx * cy);
+                d *= db[b];
+                ddLp[b] = d;
+                x_ += d;
+            }
+            xp[j] = x_;
+        }
+    // second stretch pass with the updated plastic stretch (constitutive.c:625-645)
+    a0 = 0, a1 = 0, b0 = 0, b1 = 0;
+#pragma unroll
+    for (int j = 0; j < NBM; j++)
+        if (j < nb0)
+        {
+            const int64_t b = (int64_t)j * n + i;
+            double dl = dis[j] - L0[b];
+            dl -= xp[j];
+            dl *= db[b];
+            dL[b] = dl;
+            if (nsign[b] == 0)
+                a0 += dl, b0 += Tv[b] * dl;
+            else
+                a1 += dl, b1 += Tv[b] * dl;
+        }
+#pragma unroll
+    for (int j = 0; j < NBM; j++)
+        if (j < nbmax)
+        {
+            const int64_t b = (int64_t)j * n + i;
+            dLp2[b] = db[b] * xp[j];
+        }
+    A[i] = a0, A[(int64_t)n + i] = a1;
+    Bt[i] = b0, Bt[(int64_t)n + i] = b1;
+    for (int c = 0; c < 6; c++)
+        beta2[(int64_t)c * n + i] = beta[c];
+    alpha2[i] = alpha;
+    dlam_[i] = dlam;
+}
+
+// The J2 return maps of all particles: the unrolled kernel for lists of up to 18
+// slots (SC, FCC), the generic one otherwise.
+static void launch_j2_particles(void)
+{
+    const dim3 G(nblocks(g.n)), B(LPMX_BLOCK);
+    if (g.nb <= 18)
+        hipLaunchKernelGGL(k_j2_particle_u<18>, G, B, 0, g_stream, g.n, g.nb, g.volume, g.H, g.xi, g.nb0, g.nbcur,
+                           g.nbr, g.nsign, g.opp, g.x, g.L0, g.dLp0, g.db, g.dw, g.Kn, g.Tv, g.beta0, g.alpha0,
+                           g.sigmay, g.c44, g.dL, g.csx, g.csy, g.csz, g.ddLp, g.dLp2, g.A, g.Bt, g.beta2, g.alpha2,
+                           g.dlam, g.pl_flag);
+    else
+        hipLaunchKernelGGL(k_j2_particle, G, B, 0, g_stream, g, 0, g.n);
+}
+
 // Final J2 force of particle i from the half-averaged elastic stretch of the bond
 // and its reverse bond, times damage_w (constitutive.c:647-667).
 __global__ void k_j2_force(DV v, int first, int count)
@@ -498,7 +667,7 @@ extern "C" int lpmx_bond_force_range(int mode, int first, int count)
             RETCHK(force_prof_harvest());
             HIPCHK(hipEventRecord(FP.e0, g_stream));
         }
-        hipLaunchKernelGGL(k_j2_particle, dim3(nblocks(g.n)), B, 0, g_stream, g, 0, g.n);
+        launch_j2_particles();
         LAUNCHCHK();
         if (FP.on)
         {
@@ -533,7 +702,7 @@ extern "C" int lpmx_bond_force_range(int mode, int first, int count)
 int lpmx_eval_j2(void)
 {
     const dim3 G(nblocks(g.n)), B(LPMX_BLOCK);
-    hipLaunchKernelGGL(k_j2_particle, G, B, 0, g_stream, g, 0, g.n);
+    launch_j2_particles();
     LAUNCHCHK();
     hipLaunchKernelGGL(k_j2_force, G, B, 0, g_stream, g, 0, g.n);
     LAUNCHCHK();

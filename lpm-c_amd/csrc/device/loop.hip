@@ -68,9 +68,14 @@ int lpmx_run_steps_impl(const lpmx_solver *s, int nsteps, const int *nbd, const 
             double ring[RING];
             int since = 0;
             ring[0] = nres;
+            // Within the loop every force evaluation reads slot 0 (= slot 1 from the first
+            // iteration on) and writes slot 2 and the forces; the stress and the slot
+            // 2 -> 0 copy that follow each evaluation in the reference
+            // (constitutive.c:144-145) are needed only after the last one, below.
             while (nres > s->newton_tol * tolm && ni < s->newton_maxit)
             {
-                RETCHK(lpmx_switch_state(0));
+                if (ni == 0)
+                    RETCHK(lpmx_switch_state(0));
                 // K and the constraints stay within the loop: the first pass eliminates,
                 // the later ones only zero the constrained residual
                 RETCHK(ni == 0 ? lpmx_bc_eliminate_ex(1) : lpmx_bc_eliminate_repeat());
@@ -88,7 +93,7 @@ int lpmx_run_steps_impl(const lpmx_solver *s, int nsteps, const int *nbd, const 
                 o.ir_passes += lpmx_last_ir_passes();
                 if (stt != 1)
                     o.status |= stt == 3 ? 4 : 2;
-                RETCHK(lpmx_bond_force(g.plmode));
+                RETCHK(lpmx_bond_force_range(g.plmode, 0, n));
                 RETCHK(lpmx_update_rr(&nres, nullptr, nullptr));
                 if (mixed)
                 {
@@ -97,8 +102,7 @@ int lpmx_run_steps_impl(const lpmx_solver *s, int nsteps, const int *nbd, const 
                     {
                         o.accel_reverts++;
                         RETCHK(lpmx_accel_revert());
-                        RETCHK(lpmx_switch_state(0));
-                        RETCHK(lpmx_bond_force(g.plmode));
+                        RETCHK(lpmx_bond_force_range(g.plmode, 0, n));
                         RETCHK(lpmx_update_rr(&nres, nullptr, nullptr));
                     }
                 }
@@ -114,6 +118,11 @@ int lpmx_run_steps_impl(const lpmx_solver *s, int nsteps, const int *nbd, const 
                     since = 0;
                 }
                 ring[since % RING] = nres;
+            }
+            if (ni > 0)
+            {
+                RETCHK(lpmx_compute_stress());
+                RETCHK(lpmx_switch_state(2));
             }
             if (ni >= s->newton_maxit)
                 o.status |= 1;

@@ -9,6 +9,9 @@ static double *t_A = nullptr, *t_B = nullptr; // shell sums of the elastic law
 static int *t_kstart = nullptr;               // K_pointer[i][1] (0-based starts), n+1
 static double t_nd = 0.0;                     // diagonal value of the constrained DOFs
 static bool t_nd_valid = false;               // computed since the last tangent assembly
+// version of the device K, bumped by every kernel sequence that writes it (assembly,
+// import, elimination): the block-Jacobi inverse is formed once per version
+static int t_kver = 0, t_bj_kver = -1;
 
 // Shell sums A_s = sum dL, B_s = sum Tv dL of every particle under the elastic law
 // (constitutive.c:241-261): dL = (|x_i - x_j| - L0 - dLp[0]) * damage_broken.
@@ -250,6 +253,7 @@ extern "C" int lpmx_tangent(void)
     RETCHK(ensure_tangent_buffers());
     t_nd_valid = false;
     t_epoch++;
+    t_kver++;
     hipLaunchKernelGGL(k_elastic_sums, dim3(nblocks(g.n)), dim3(LPMX_BLOCK), 0, g_stream, g, t_A, t_B);
     LAUNCHCHK();
     if (g.dim == 3)
@@ -371,6 +375,7 @@ extern "C" int lpmx_bc_eliminate_ex(int fixed_diag)
         t_nd = nd;
         t_nd_valid = true;
     }
+    t_kver++;
     if (D == 3)
     {
         hipLaunchKernelGGL(k_bc_zero<3>, dim3(g.nslices), dim3(64), 0, g_stream, g);
@@ -463,7 +468,11 @@ int lpmx_bj_inverse(void)
     {
         HIPCHK(hipMalloc(&g.Minv, sizeof(double) * g.d2 * (int64_t)g.n));
         HIPCHK(hipMalloc(&g.Minv32, sizeof(float) * g.d2 * (int64_t)g.n));
+        t_bj_kver = -1;
     }
+    if (t_bj_kver == t_kver)
+        return LPMX_OK;
+    t_bj_kver = t_kver;
     if (g.dim == 3)
         hipLaunchKernelGGL(k_bj_inverse<3>, dim3(nblocks(g.n)), dim3(LPMX_BLOCK), 0, g_stream, g);
     else
@@ -635,6 +644,7 @@ extern "C" int lpmx_export_upper_csr(int *IK, int *JK, double *K)
 extern "C" int lpmx_import_upper_csr(const double *K)
 {
     t_epoch++;
+    t_kver++;
     RETCHK(lpmx_require_pattern());
     if (!K)
         return lpmx_fail(LPMX_ERR_ARG, "null K");
