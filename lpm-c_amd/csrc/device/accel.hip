@@ -26,9 +26,11 @@
 
 #define AA_MAX 32
 
+// dC_j = dU_j + dF_j (the iterate and correction differences, summed: the mix uses
+// only their sum) and dF_j (the Gram matrix and right-hand side use the corrections).
 struct AaPtrs
 {
-    double *dU[AA_MAX], *dF[AA_MAX];
+    double *dC[AA_MAX], *dF[AA_MAX];
 };
 
 struct AaGamma
@@ -55,8 +57,8 @@ static void aa_release(AaHist *a)
 {
     for (int j = 0; j < AA_MAX; j++)
     {
-        if (a->P.dU[j])
-            hipFree(a->P.dU[j]);
+        if (a->P.dC[j])
+            hipFree(a->P.dC[j]);
         if (a->P.dF[j])
             hipFree(a->P.dF[j]);
     }
@@ -72,7 +74,7 @@ void lpmx_accel_free(void) { aa_release(&A); }
 // u = x - d (the iterate the update started from); with a previous iterate the new
 // differences go to slot s; then (u, d) become the previous iterate.
 __global__ void k_aa_push(int n, int dim, int xs, const double *__restrict__ x, const double *__restrict__ d,
-                          double *__restrict__ u_prev, double *__restrict__ f_prev, double *__restrict__ dU,
+                          double *__restrict__ u_prev, double *__restrict__ f_prev, double *__restrict__ dC,
                           double *__restrict__ dF, int have_prev)
 {
     const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -83,8 +85,9 @@ __global__ void k_aa_push(int n, int dim, int xs, const double *__restrict__ x, 
     const double u = x[xs * i + c] - d[t];
     if (have_prev)
     {
-        dU[t] = u - u_prev[t];
-        dF[t] = d[t] - f_prev[t];
+        const double du = u - u_prev[t], df = d[t] - f_prev[t];
+        dC[t] = du + df;
+        dF[t] = df;
     }
     u_prev[t] = u;
     f_prev[t] = d[t];
@@ -133,23 +136,22 @@ __global__ __launch_bounds__(256) void k_aa_dots(int64_t m, AaPtrs P, int cnt, i
     }
 }
 
+// One work-group per dot product (q = blockIdx.x): the partials in a fixed order.
 __global__ __launch_bounds__(256) void k_aa_dots_total(const double *parts, int np, int cnt, double *out)
 {
     __shared__ double sh[4];
-    for (int q = 0; q < 2 * AA_MAX; q++)
-    {
-        if ((q % AA_MAX) >= cnt)
-            continue;
-        double s = 0.0;
-        for (int k = threadIdx.x; k < np; k += blockDim.x)
-            s += parts[(int64_t)k * 2 * AA_MAX + q];
-        s = block_sum(s, sh);
-        if (threadIdx.x == 0)
-            out[q] = s;
-    }
+    const int q = blockIdx.x;
+    if ((q % AA_MAX) >= cnt)
+        return;
+    double s = 0.0;
+    for (int k = threadIdx.x; k < np; k += blockDim.x)
+        s += parts[(int64_t)k * 2 * AA_MAX + q];
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0)
+        out[q] = s;
 }
 
-// x -= sum_j gamma_j (dU_j + dF_j)
+// x -= sum_j gamma_j dC_j
 __global__ void k_aa_mix(int n, int dim, int xs, double *__restrict__ x, AaPtrs P, int cnt, AaGamma gam)
 {
     const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -159,7 +161,7 @@ __global__ void k_aa_mix(int n, int dim, int xs, double *__restrict__ x, AaPtrs 
 #pragma unroll
     for (int j = 0; j < AA_MAX; j++)
         if (j < cnt)
-            s += gam.g[j] * (P.dU[j][t] + P.dF[j][t]);
+            s += gam.g[j] * P.dC[j][t];
     const int64_t i = t / dim;
     const int c = (int)(t - i * dim);
     x[xs * i + c] -= s;
@@ -188,8 +190,8 @@ static int aa_begin(AaHist *a, int depth)
     const int64_t m = (int64_t)g.n * g.dim;
     for (int j = 0; j < depth; j++)
     {
-        if (!a->P.dU[j])
-            HIPCHK(hipMalloc(&a->P.dU[j], sizeof(double) * m));
+        if (!a->P.dC[j])
+            HIPCHK(hipMalloc(&a->P.dC[j], sizeof(double) * m));
         if (!a->P.dF[j])
             HIPCHK(hipMalloc(&a->P.dF[j], sizeof(double) * m));
     }
@@ -276,7 +278,7 @@ static int aa_step(AaHist *a, double *x, int xs, const double *d, int *mixed)
     const int nb = nblocks(m);
     const int s = a->next;
     hipLaunchKernelGGL(k_aa_push, dim3(nb), dim3(LPMX_BLOCK), 0, g_stream, n, dim, xs, x, d, a->u_prev, a->f_prev,
-                       a->P.dU[s], a->P.dF[s], a->have_prev);
+                       a->P.dC[s], a->P.dF[s], a->have_prev);
     LAUNCHCHK();
     if (!a->have_prev)
     {
@@ -288,7 +290,7 @@ static int aa_step(AaHist *a, double *x, int xs, const double *d, int *mixed)
     a->next = (a->next + 1) % a->depth;
     const int cnt = a->cnt;
     hipLaunchKernelGGL(k_aa_dots, dim3(AA_GRID), dim3(256), 0, g_stream, m, a->P, cnt, s, d, a->parts);
-    hipLaunchKernelGGL(k_aa_dots_total, dim3(1), dim3(256), 0, g_stream, a->parts, AA_GRID, cnt, a->out);
+    hipLaunchKernelGGL(k_aa_dots_total, dim3(2 * AA_MAX), dim3(256), 0, g_stream, a->parts, AA_GRID, cnt, a->out);
     LAUNCHCHK();
     double o[2 * AA_MAX];
     HIPCHK(hipMemcpyAsync(o, a->out, sizeof(o), hipMemcpyDeviceToHost, g_stream));

@@ -220,7 +220,8 @@ __global__ void k_j2_particle(DV v, int first, int count)
 // ahead of the stores (the generic kernel's pointers may alias, which serialises the
 // dependent index and position loads of one bond after another).  Same arithmetic in
 // the same order; the second stretch pass keeps the distances of the first in
-// registers and does not rewrite the unchanged direction cosines.
+// registers, runs in the plastic-stretch loop and does not rewrite the unchanged
+// direction cosines.
 template <int NBM>
 __global__ __launch_bounds__(256) void k_j2_particle_u(
     int n, int nbmax, double volume, double H, double xi, const int *__restrict__ nb0_,
@@ -320,47 +321,36 @@ __global__ __launch_bounds__(256) void k_j2_particle_u(
             dep[c] = dlam * 1.5 * sl[c] / seq;
             beta[c] += 2. / 3. * xi * H * dep[c];
         }
-    // plastic bond stretch (constitutive.c:611-621) over every list slot
-    double xp[NBM];
-#pragma unroll
-    for (int j = 0; j < NBM; j++)
-        if (j < nbmax)
-        {
-            const int64_t b = (int64_t)j * n + i;
-            double x_ = dLp0[b];
-            if (j < nb0)
-            {
-                const double cx = csx[b], cy = csy[b], cz = csz[b];
-                double d = L0[b] * (dep[0] * cx * cx + dep[1] * cy * cy + dep[2] * cz * cz + 2 * dep[3] * cy * cz +
-                                    2 * dep[4] * cx * cz + 2 * dep[5] * cx * cy);
-                d *= db[b];
-                ddLp[b] = d;
-                x_ += d;
-            }
-            xp[j] = x_;
-        }
-    // second stretch pass with the updated plastic stretch (constitutive.c:625-645)
+    // plastic bond stretch over every list slot (constitutive.c:611-621) and, per
+    // bond, the second stretch pass with it (constitutive.c:625-645): each bond's
+    // plastic stretch is final before its stretch uses it, so one loop serves both
+    // (the shell sums still run in slot order)
     a0 = 0, a1 = 0, b0 = 0, b1 = 0;
 #pragma unroll
     for (int j = 0; j < NBM; j++)
-        if (j < nb0)
-        {
-            const int64_t b = (int64_t)j * n + i;
-            double dl = dis[j] - L0[b];
-            dl -= xp[j];
-            dl *= db[b];
-            dL[b] = dl;
-            if (nsign[b] == 0)
-                a0 += dl, b0 += Tv[b] * dl;
-            else
-                a1 += dl, b1 += Tv[b] * dl;
-        }
-#pragma unroll
-    for (int j = 0; j < NBM; j++)
         if (j < nbmax)
         {
             const int64_t b = (int64_t)j * n + i;
-            dLp2[b] = db[b] * xp[j];
+            const double dbb = db[b];
+            double x_ = dLp0[b];
+            if (j < nb0)
+            {
+                const double cx = csx[b], cy = csy[b], cz = csz[b], l0 = L0[b];
+                double d = l0 * (dep[0] * cx * cx + dep[1] * cy * cy + dep[2] * cz * cz + 2 * dep[3] * cy * cz +
+                                 2 * dep[4] * cx * cz + 2 * dep[5] * cx * cy);
+                d *= dbb;
+                ddLp[b] = d;
+                x_ += d;
+                double dl = dis[j] - l0;
+                dl -= x_;
+                dl *= dbb;
+                dL[b] = dl;
+                if (nsign[b] == 0)
+                    a0 += dl, b0 += Tv[b] * dl;
+                else
+                    a1 += dl, b1 += Tv[b] * dl;
+            }
+            dLp2[b] = dbb * x_;
         }
     A[i] = a0, A[(int64_t)n + i] = a1;
     Bt[i] = b0, Bt[(int64_t)n + i] = b1;

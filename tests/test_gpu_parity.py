@@ -587,8 +587,8 @@ def test_history_adjoint_matches_oracle(oracle):
 def test_history_adjoint_central_differences_and_autograd():
     """Gradients of J = sum (R_n - t_n)^2 over a six-step J2 tension history on the
     device (checkpoint every 2 steps), through the torch.autograd harness, against
-    central differences of the device forward with h = 1e-6 relative: the global
-    modulus E, a random direction of the per-particle modulus field, the yield
+    central differences of the device forward with h = 1e-6 and 1e-5 relative: the
+    global modulus E, a random direction of the per-particle modulus field, the yield
     stress field along a random direction, and H."""
     import torch
 
@@ -618,21 +618,29 @@ def test_history_adjoint_central_differences_and_autograd():
     def Jof(E, em, s_, H):
         return float(np.sum((reactions(E, em, s_, H) - target) ** 2))
 
-    h = 1e-6
     d1 = rng.standard_normal(n)
     d2 = rng.standard_normal(n)
-    checks = [
-        ("E", (Jof(E0 * (1 + h), emod, sy, H0) - Jof(E0 * (1 - h), emod, sy, H0)) / (2 * h * E0), tE.grad.item()),
-        ("modulus field", (Jof(E0, emod * (1 + h * d1), sy, H0) - Jof(E0, emod * (1 - h * d1), sy, H0)) / (2 * h),
-         float(tm.grad.numpy() @ (emod * d1))),
-        ("yield stress", (Jof(E0, emod, sy * (1 + h * d2), H0) - Jof(E0, emod, sy * (1 - h * d2), H0)) / (2 * h),
-         float(ts.grad.numpy() @ (sy * d2))),
-        ("H", (Jof(E0, emod, sy, H0 * (1 + h)) - Jof(E0, emod, sy, H0 * (1 - h))) / (2 * h * H0), tH.grad.item()),
-    ]
-    for name, f, a in checks:
-        print(f"{name}: FD {f:.10e} adjoint {a:.10e} rel {abs(f - a) / abs(a):.2e}")
-    for name, f, a in checks:
-        assert abs(f - a) <= 2e-5 * abs(a), (name, f, a)
+
+    def checks(h):
+        return [
+            ("E", (Jof(E0 * (1 + h), emod, sy, H0) - Jof(E0 * (1 - h), emod, sy, H0)) / (2 * h * E0),
+             tE.grad.item()),
+            ("modulus field", (Jof(E0, emod * (1 + h * d1), sy, H0) - Jof(E0, emod * (1 - h * d1), sy, H0)) / (2 * h),
+             float(tm.grad.numpy() @ (emod * d1))),
+            ("yield stress", (Jof(E0, emod, sy * (1 + h * d2), H0) - Jof(E0, emod, sy * (1 - h * d2), H0)) / (2 * h),
+             float(ts.grad.numpy() @ (sy * d2))),
+            ("H", (Jof(E0, emod, sy, H0 * (1 + h)) - Jof(E0, emod, sy, H0 * (1 - h))) / (2 * h * H0), tH.grad.item()),
+        ]
+
+    # h = 1e-6: the numerators are ~1e-6 of J while each history carries its Newton
+    # tolerance (1e-12), so the field and H quotients are noise-limited near 3e-6; E
+    # moves J most and agrees to 1e-7.  h = 1e-5 (truncation ~1e-10) brings the field
+    # directions under 1e-6; H, which moves J least, stays near 2e-6.
+    bounds = {1e-6: dict(E=1e-6, default=2e-5), 1e-5: dict(H=5e-6, default=1e-6)}
+    for h, bd in bounds.items():
+        for name, f, a in checks(h):
+            print(f"h {h:g} {name}: FD {f:.10e} adjoint {a:.10e} rel {abs(f - a) / abs(a):.2e}")
+            assert abs(f - a) <= bd.get(name, bd["default"]) * abs(a), (h, name, f, a)
 
 
 # ------------------------------------------------------------------ AMG ------------
