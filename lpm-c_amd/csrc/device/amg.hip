@@ -541,6 +541,7 @@ static struct Amg
     int degree = 1;         // Chebyshev degree on the finest level
     int cdegree = 1;        // ... and on the coarser levels
     double ratio = 10.0;
+    int post = 1;           // post-smoothing (0: V(1,0) cycles, not symmetric)
     int64_t bytes = 0;      // device memory of the hierarchy
 } AM;
 
@@ -578,10 +579,10 @@ void lpmx_amg_free(void)
 {
     for (void *p : AM.mem)
         hipFree(p);
-    const int deg = AM.degree, cdeg = AM.cdegree;
+    const int deg = AM.degree, cdeg = AM.cdegree, post = AM.post;
     const double ratio = AM.ratio;
     AM = Amg();
-    AM.degree = deg, AM.cdegree = cdeg, AM.ratio = ratio;
+    AM.degree = deg, AM.cdegree = cdeg, AM.ratio = ratio, AM.post = post;
 }
 
 template <typename T>
@@ -721,6 +722,8 @@ int lpmx_amg_build(void)
         AM.degree = std::max(1, std::min(6, atoi(e)));
     if (const char *e = getenv("LPMX_AMG_CDEGREE"))
         AM.cdegree = std::max(1, std::min(6, atoi(e)));
+    if (const char *e = getenv("LPMX_AMG_POST"))
+        AM.post = atoi(e) != 0;
     if (const char *e = getenv("LPMX_AMG_RATIO"))
         AM.ratio = std::max(2.0, atof(e));
     const int n = g.n, D = g.dim, D2 = D * D, nm = D == 3 ? 6 : 3;
@@ -1358,7 +1361,8 @@ static void vcycle(int l, const TX *f, TX *x, const int *done)
     launch_mv<TX, 0>(v.R, r, cb, nullptr, done);
     vcycle<TX>(l + 1, cb, cx, done);
     launch_mv<TX, 2>(v.P, cx, x, nullptr, done);
-    smooth<TX, false>(l, f, x, r, dd, done);
+    if (AM.post)
+        smooth<TX, false>(l, f, x, r, dd, done);
 }
 
 // z = B r with the V-cycle (r, z: fine DOF vectors); kernels return at once while
