@@ -6,9 +6,16 @@
 // Jacobian included, is a plane [entry][particle], so the 64 lanes of a wavefront
 // touch 64 consecutive particles' entry e at once.
 //
-// The reference factorises the slip Jacobian in place (LAPACKE_dgesv) and rewrites
-// only the entries of active pairs and inactive diagonals before the next inner
-// iteration; the entries it leaves carry the previous factors.  That is kept.
+// The reference factorises the full nslip x nslip slip Jacobian in place
+// (LAPACKE_dgesv), with zeros in every row and column of an inactive system but its
+// unit diagonal.  Partial pivoting never picks such a row (its entries in the active
+// columns are zero), its multipliers are zero and its right-hand side entry is
+// zero, so every elimination step on it subtracts zero products: the active
+// systems' solution is the LU solve of the active submatrix, rows and columns kept
+// in ascending system order, with the same operations in the same order.  This
+// kernel factorises only that submatrix (na x na, na active systems) and sums the
+// hardening matrix over the active systems only; the results are the same values,
+// and the work per inner iteration drops from nslip^3 to na^3.
 //
 // Compiled without floating-point contraction, in the reference's evaluation order.
 #include <math.h>
@@ -103,9 +110,10 @@ __device__ void cp_stress(const DV &v, int i, const double *xdl, const double *x
     }
 }
 
-// Solve D x = r in place for particle i (D in planes): Gaussian elimination with
-// partial pivoting (first largest |d_ik| on or below the diagonal), whole-row swaps,
-// multipliers stored below the diagonal.  Returns false on an exact zero pivot.
+// Solve D x = r in place for particle i (D an ns x ns matrix in planes, entry
+// (a, b) at plane a*ns + b): Gaussian elimination with partial pivoting (first
+// largest |d_ik| on or below the diagonal), whole-row swaps, multipliers stored
+// below the diagonal.  Returns false on an exact zero pivot.
 __device__ bool cp_lu_solve(const DV &v, int i, int ns, double *r)
 {
     int piv[LPMX_MAXSLIP];
@@ -219,7 +227,8 @@ __global__ void k_cp_particle(DV v, int first, int count)
     cp_stress(v, i, dl, a, bt, sl);
     double rss[LPMX_MAXSLIP], yf[LPMX_MAXSLIP], xgy[LPMX_MAXSLIP], gam[LPMX_MAXSLIP], r[LPMX_MAXSLIP],
         rr[LPMX_MAXSLIP], dgy[LPMX_MAXSLIP];
-    int Jact[LPMX_MAXSLIP];
+    int Jact[LPMX_MAXSLIP], act[LPMX_MAXSLIP];
+    double ract[LPMX_MAXSLIP];
     double tmax = 0.0;
     for (int s = 0; s < ns; s++)
     {
@@ -245,18 +254,25 @@ __global__ void k_cp_particle(DV v, int first, int count)
         {
             nouter++;
             double nr = 1.0;
+            int na = 0; // the active systems of this pass, ascending
             for (int s = 0; s < ns; s++)
+            {
                 gam[s] = 0.0, r[s] = 0.0, rr[s] = 0.0;
-            for (int e = 0; e < ns * ns; e++)
-                v.cp_D[CPI(e, i)] = 0.0;
+                if (Jact[s] == 1)
+                    act[na++] = s;
+            }
             int ninner = 0;
             do
             {
                 ninner++;
+                // sums over the active systems: an inactive one adds Jact * ... = 0
                 double dpl[6] = {0, 0, 0, 0, 0, 0};
-                for (int s = 0; s < ns; s++)
+                for (int a = 0; a < na; a++)
+                {
+                    const int s = act[a];
                     for (int c = 0; c < 6; c++)
-                        dpl[c] += Jact[s] * gam[s] * v.schmid[6 * s + c];
+                        dpl[c] += gam[s] * v.schmid[6 * s + c];
+                }
                 double xa[2] = {0, 0}, xb[2] = {0, 0};
                 for (int j = 0; j < nbi; j++)
                 {
@@ -278,16 +294,21 @@ __global__ void k_cp_particle(DV v, int first, int count)
                 const double h = v.cp_h0 / cp_pow(cosh(v.cp_h0 * xA / tsd), 2.0);
                 const double hp = -2.0 * v.cp_h0 * v.cp_h0 / tsd * tanh(v.cp_h0 * xA / tsd) * h;
                 for (int s = 0; s < ns; s++)
+                    dgy[s] = 0.0;
+                for (int a = 0; a < na; a++)
                 {
+                    const int s = act[a];
                     double t1 = 0.0;
-                    for (int u = 0; u < ns; u++)
+                    for (int c = 0; c < na; c++)
                     {
+                        const int u = act[c];
                         const double hab = s == u ? h : v.cp_q * h;
-                        t1 += Jact[u] * hab * gam[u];
+                        t1 += hab * gam[u];
                     }
-                    dgy[s] = Jact[s] * t1;
-                    xgy[s] = v.cp_gy0[CPI(s, i)] + dgy[s];
+                    dgy[s] = t1;
                 }
+                for (int s = 0; s < ns; s++)
+                    xgy[s] = v.cp_gy0[CPI(s, i)] + dgy[s];
                 for (int s = 0; s < ns; s++)
                 {
                     const double t1 = cp_pow(1. + gam[s] * v.cp_eta / v.dtime, 1. / v.cp_p);
@@ -296,39 +317,45 @@ __global__ void k_cp_particle(DV v, int first, int count)
                     r[s] = Jact[s] * (rss[s] - xgy[s] * t1);
                     rr[s] = r[s];
                 }
-                for (int m = 0; m < ns; m++)
-                    for (int u = 0; u < ns; u++)
+                // the active submatrix of the slip Jacobian; the hardening sum over
+                // the active systems (the inactive ones add Jact * hd = 0)
+                for (int a = 0; a < na; a++)
+                {
+                    const int m = act[a];
+                    const double t1 = xgy[m] * (v.cp_eta / v.cp_p / v.dtime *
+                                                cp_pow(1. + v.cp_eta * gam[m] / v.dtime, (1. - v.cp_p) / v.cp_p));
+                    const double pw = cp_pow(1. + v.cp_eta * gam[m] / v.dtime, (1. / v.cp_p));
+                    for (int b = 0; b < na; b++)
                     {
-                        if (Jact[m] == 1 && Jact[u] == 1)
+                        const int u = act[b];
+                        double hs = 0.0;
+                        for (int c = 0; c < na; c++)
                         {
-                            double hs = 0.0;
-                            for (int d = 0; d < ns; d++)
-                            {
-                                double hd;
-                                if (m == d && u == d)
-                                    hd = h + hp * gam[d];
-                                else if (m == d && u != d)
-                                    hd = hp * gam[d];
-                                else if (m != d && u == d)
-                                    hd = v.cp_q * (h + hp * gam[d]);
-                                else
-                                    hd = v.cp_q * hp * gam[d];
-                                hs += Jact[d] * hd;
-                            }
-                            const double t1 = xgy[m] * (v.cp_eta / v.cp_p / v.dtime *
-                                                        cp_pow(1. + v.cp_eta * gam[m] / v.dtime, (1. - v.cp_p) / v.cp_p));
-                            const double t2 = hs * cp_pow(1. + v.cp_eta * gam[m] / v.dtime, (1. / v.cp_p));
-                            const double cab = v.cp_Cab[CPI(m * ns + u, i)];
-                            v.cp_D[CPI(m * ns + u, i)] = m == u ? cab + t1 + t2 : cab + t2;
+                            const int d = act[c];
+                            double hd;
+                            if (m == d && u == d)
+                                hd = h + hp * gam[d];
+                            else if (m == d && u != d)
+                                hd = hp * gam[d];
+                            else if (m != d && u == d)
+                                hd = v.cp_q * (h + hp * gam[d]);
+                            else
+                                hd = v.cp_q * hp * gam[d];
+                            hs += hd;
                         }
-                        else if (m == u)
-                            v.cp_D[CPI(m * ns + u, i)] = 1.0;
+                        const double t2 = hs * pw;
+                        const double cab = v.cp_Cab[CPI(m * ns + u, i)];
+                        v.cp_D[CPI(a * na + b, i)] = m == u ? cab + t1 + t2 : cab + t2;
                     }
-                if (!cp_lu_solve(v, i, ns, rr))
+                    ract[a] = rr[m];
+                }
+                if (!cp_lu_solve(v, i, na, ract))
                 {
                     atomicAdd(v.cp_fail, 1);
                     goto done;
                 }
+                for (int a = 0; a < na; a++)
+                    rr[act[a]] = ract[a];
                 for (int s = 0; s < ns; s++)
                     gam[s] += Jact[s] * rr[s];
                 nr = cp_nrm2(ns, r);

@@ -10,6 +10,8 @@
 // halves the iteration count).
 #include <chrono>
 #include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include "common.h"
@@ -27,6 +29,15 @@ static int sync(void)
 {
     HIPCHK(hipStreamSynchronize(g_stream));
     return LPMX_OK;
+}
+
+// LPMX_PROGRESS set: a line to stderr every 50 Newton iterations of a step
+static bool loop_progress(void)
+{
+    static int on = -1;
+    if (on < 0)
+        on = getenv("LPMX_PROGRESS") != nullptr;
+    return on;
 }
 
 int lpmx_run_steps_impl(const lpmx_solver *s, int nsteps, const int *nbd, const lpmx_dispbc *dbc, const int *nbf,
@@ -119,6 +130,9 @@ int lpmx_run_steps_impl(const lpmx_solver *s, int nsteps, const int *nbd, const 
                     since = 0;
                 }
                 ring[since % RING] = nres;
+                if (ni % 50 == 0 && loop_progress())
+                    fprintf(stderr, "[lpmx] step %d: Newton %d, |R| %.3e (target %.3e), CG %d, %.2f s\n", st, ni,
+                            nres, s->newton_tol * tolm, o.cg_iters, now_s() - t0);
             }
             if (ni > 0)
             {
