@@ -221,7 +221,11 @@ __global__ void k_j2_particle(DV v, int first, int count)
 // dependent index and position loads of one bond after another).  Same arithmetic in
 // the same order; the second stretch pass keeps the distances of the first in
 // registers, runs in the plastic-stretch loop and does not rewrite the unchanged
-// direction cosines.
+// direction cosines.  The direction cosines, elastic stretches and shell bits of the
+// first pass stay in registers for the two later loops (234 VGPRs, two waves per
+// SIMD): 42% fewer bytes read than with them read back, and at 1M as fast in a
+// partly plastic step and 8% faster when every particle yields.  Keeping L0 and Tv
+// as well needs more than 256 VGPRs, one wave per SIMD, and is 28% slower.
 template <int NBM>
 __global__ __launch_bounds__(256) void k_j2_particle_u(
     int n, int nbmax, double volume, double H, double xi, const int *__restrict__ nb0_,
@@ -239,7 +243,10 @@ __global__ __launch_bounds__(256) void k_j2_particle_u(
         return;
     const int nb0 = nb0_[i];
     const double xi0 = X[3 * i], xi1 = X[3 * i + 1], xi2 = X[3 * i + 2];
-    double dis[NBM];
+    // per bond, kept in registers for the later loops: distance, direction cosines,
+    // elastic stretch and the shell bit
+    double dis[NBM], rcx[NBM], rcy[NBM], rcz[NBM], rdl[NBM];
+    unsigned sbit = 0;
     // first stretch pass (constitutive.c:495-515)
     double a0 = 0, a1 = 0, b0 = 0, b1 = 0;
 #pragma unroll
@@ -251,17 +258,23 @@ __global__ __launch_bounds__(256) void k_j2_particle_u(
             const double dx = xi0 - X[3 * q], dy = xi1 - X[3 * q + 1], dz = xi2 - X[3 * q + 2];
             const double d = sqrt(dx * dx + dy * dy + dz * dz);
             dis[j] = d;
-            double dl = d - L0[b];
+            const double l0 = L0[b], tv = Tv[b];
+            double dl = d - l0;
             dl -= dLp0[b];
             dl *= db[b];
             dL[b] = dl;
+            rdl[j] = dl;
             if (nsign[b] == 0)
-                a0 += dl, b0 += Tv[b] * dl;
+                a0 += dl, b0 += tv * dl;
             else
-                a1 += dl, b1 += Tv[b] * dl;
-            csx[b] = dx / d;
-            csy[b] = dy / d;
-            csz[b] = dz / d;
+            {
+                a1 += dl, b1 += tv * dl;
+                sbit |= 1u << j;
+            }
+            rcx[j] = dx / d, rcy[j] = dy / d, rcz[j] = dz / d;
+            csx[b] = rcx[j];
+            csy[b] = rcy[j];
+            csz[b] = rcz[j];
         }
     // trial stress from the particle's own sums (constitutive.c:518-568)
     const bool full = nbcur[i] == nbmax;
@@ -271,8 +284,8 @@ __global__ __launch_bounds__(256) void k_j2_particle_u(
         if (j < nb0)
         {
             const int64_t b = (int64_t)j * n + i;
-            const int s = nsign[b];
-            double Fij = 2.0 * Kn[b] * dL[b] + (s == 0 ? b0 : b1) + Tv[b] * (s == 0 ? a0 : a1);
+            const bool s1 = (sbit >> j) & 1u;
+            double Fij = 2.0 * Kn[b] * rdl[j] + (s1 ? b1 : b0) + Tv[b] * (s1 ? a1 : a0);
             Fij *= dw[b];
             double of = 0.5;
             if (!full)
@@ -280,7 +293,7 @@ __global__ __launch_bounds__(256) void k_j2_particle_u(
                 const int o = opp[b];
                 of = o < 0 ? 1.0 : (db[(int64_t)o * n + i] <= LPMX_EPS ? 1.0 : 0.5);
             }
-            const double cx = csx[b], cy = csy[b], cz = csz[b], l0 = L0[b];
+            const double cx = rcx[j], cy = rcy[j], cz = rcz[j], l0 = L0[b];
             sl[0] += of / volume * l0 * Fij * cx * cx;
             sl[1] += of / volume * l0 * Fij * cy * cy;
             sl[2] += of / volume * l0 * Fij * cz * cz;
@@ -335,7 +348,7 @@ __global__ __launch_bounds__(256) void k_j2_particle_u(
             double x_ = dLp0[b];
             if (j < nb0)
             {
-                const double cx = csx[b], cy = csy[b], cz = csz[b], l0 = L0[b];
+                const double cx = rcx[j], cy = rcy[j], cz = rcz[j], l0 = L0[b];
                 double d = l0 * (dep[0] * cx * cx + dep[1] * cy * cy + dep[2] * cz * cz + 2 * dep[3] * cy * cz +
                                  2 * dep[4] * cx * cz + 2 * dep[5] * cx * cy);
                 d *= dbb;
@@ -345,7 +358,7 @@ __global__ __launch_bounds__(256) void k_j2_particle_u(
                 dl -= x_;
                 dl *= dbb;
                 dL[b] = dl;
-                if (nsign[b] == 0)
+                if (((sbit >> j) & 1u) == 0)
                     a0 += dl, b0 += Tv[b] * dl;
                 else
                     a1 += dl, b1 += Tv[b] * dl;
