@@ -51,7 +51,9 @@ struct AaHist
 
 static AaHist A; // the Newton loop's history
 
-static const int AA_GRID = 1024;
+// work-groups of the dot pass: three per CU (256 CUs), one resident round of four-wave
+// groups at the kernel's 166 VGPRs (three waves per SIMD)
+static const int AA_GRID = 768;
 
 static void aa_release(AaHist *a)
 {
@@ -94,6 +96,9 @@ __global__ void k_aa_push(int n, int dim, int xs, const double *__restrict__ x, 
 }
 
 // Partials of dF_s . dF_j and dF_j . d for the cnt active slots: parts[b][2*AA_MAX].
+// Every slot is read: the host points the slots from cnt on at slot s (cached after
+// its first read), so the AA_MAX loads of an element are issued together.  With the
+// loads under `j < cnt` each one was waited for before the next was issued.
 __global__ __launch_bounds__(256) void k_aa_dots(int64_t m, AaPtrs P, int cnt, int s, const double *__restrict__ d,
                                                  double *parts)
 {
@@ -104,14 +109,16 @@ __global__ __launch_bounds__(256) void k_aa_dots(int64_t m, AaPtrs P, int cnt, i
     for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < m; t += (int64_t)gridDim.x * blockDim.x)
     {
         const double fs = P.dF[s][t], dt = d[t];
+        double f[AA_MAX];
 #pragma unroll
         for (int j = 0; j < AA_MAX; j++)
-            if (j < cnt)
-            {
-                const double fj = P.dF[j][t];
-                g[j] += fs * fj;
-                h[j] += fj * dt;
-            }
+            f[j] = P.dF[j][t];
+#pragma unroll
+        for (int j = 0; j < AA_MAX; j++)
+        {
+            g[j] += fs * f[j];
+            h[j] += f[j] * dt;
+        }
     }
     // wave sums of all 2 cnt values first, then one exchange through LDS: the
     // work-group totals in wave order (fixed, so deterministic)
@@ -289,7 +296,10 @@ static int aa_step(AaHist *a, double *x, int xs, const double *d, int *mixed)
         a->cnt++;
     a->next = (a->next + 1) % a->depth;
     const int cnt = a->cnt;
-    hipLaunchKernelGGL(k_aa_dots, dim3(AA_GRID), dim3(256), 0, g_stream, m, a->P, cnt, s, d, a->parts);
+    AaPtrs Q = a->P; // the unused slots read slot s
+    for (int j = cnt; j < AA_MAX; j++)
+        Q.dF[j] = a->P.dF[s];
+    hipLaunchKernelGGL(k_aa_dots, dim3(AA_GRID), dim3(256), 0, g_stream, m, Q, cnt, s, d, a->parts);
     hipLaunchKernelGGL(k_aa_dots_total, dim3(2 * AA_MAX), dim3(256), 0, g_stream, a->parts, AA_GRID, cnt, a->out);
     LAUNCHCHK();
     double o[2 * AA_MAX];
