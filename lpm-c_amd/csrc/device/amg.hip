@@ -678,15 +678,55 @@ static int alloc_vectors(Level &v)
     return LPMX_OK;
 }
 
-static uint64_t mask_hash(const std::vector<int> &db, const std::vector<int> &fx)
+// A fingerprint of the constraint set: the sum, mod 2^64, of the splitmix64 finaliser
+// (Steele, Lea and Flood, OOPSLA 2014) of the index of every DOF that is both
+// displacement-constrained and fixed.  Integer sums do not depend on their order, so
+// per-work-group partials summed on the host give the same value every time, and
+// only the partials leave the device.
+#define SIG_GRID 256
+__device__ __forceinline__ uint64_t sig_mix(uint64_t z)
 {
-    uint64_t h = 1469598103934665603ull;
-    for (size_t k = 0; k < db.size(); k++)
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+__global__ __launch_bounds__(256) void k_mask_sig(int64_t m, const int *__restrict__ db, const int *__restrict__ fx,
+                                                   uint64_t *__restrict__ parts)
+{
+    __shared__ uint64_t sh[256];
+    uint64_t s = 0;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < m; t += (int64_t)gridDim.x * blockDim.x)
+        if (db[t] != 0 && fx[t] != 0)
+            s += sig_mix((uint64_t)t);
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x == 0)
     {
-        h ^= (uint64_t)(db[k] != 0 && fx[k] != 0);
-        h *= 1099511628211ull;
+        uint64_t b = 0;
+        for (int k = 0; k < (int)blockDim.x; k++)
+            b += sh[k];
+        parts[blockIdx.x] = b;
     }
-    return h;
+}
+
+static uint64_t *g_sig_parts = nullptr; // SIG_GRID partials, kept until the process ends
+
+static int mask_signature(uint64_t *sig)
+{
+    if (!g_sig_parts)
+        HIPCHK(hipMalloc((void **)&g_sig_parts, sizeof(uint64_t) * SIG_GRID));
+    const int64_t m = (int64_t)g.n * g.dim;
+    hipLaunchKernelGGL(k_mask_sig, dim3(SIG_GRID), dim3(256), 0, g_stream, m, g.dispBC, g.fix, g_sig_parts);
+    LAUNCHCHK();
+    uint64_t parts[SIG_GRID];
+    RETCHK(d2h(parts, g_sig_parts, sizeof(parts)));
+    uint64_t h = 0;
+    for (int b = 0; b < SIG_GRID; b++)
+        h += parts[b];
+    *sig = h ^ (uint64_t)m;
+    return LPMX_OK;
 }
 
 static int read_masks(std::vector<int> &db, std::vector<int> &fx)
@@ -703,10 +743,10 @@ bool lpmx_amg_current(void)
 {
     if (!AM.ready || AM.n_model != g.n)
         return false;
-    std::vector<int> db, fx;
-    if (read_masks(db, fx) != LPMX_OK)
+    uint64_t sig = 0;
+    if (mask_signature(&sig) != LPMX_OK)
         return false;
-    return mask_hash(db, fx) == AM.bc_hash;
+    return sig == AM.bc_hash;
 }
 
 double lpmx_amg_setup_seconds(void) { return AM.t_setup; }
@@ -928,7 +968,7 @@ int lpmx_amg_build(void)
         RETCHK(alloc_vectors(v));
     }
     AM.n_model = n;
-    AM.bc_hash = mask_hash(db, fx);
+    RETCHK(mask_signature(&AM.bc_hash));
     AM.ready = true;
     AM.t_setup = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     if (getenv("LPMX_PROGRESS"))

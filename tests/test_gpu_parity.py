@@ -680,6 +680,35 @@ def test_amg_solve_matches_block_jacobi(capfd, prec):
     assert iters[1] * 4 < iters[0]
 
 
+def test_amg_hierarchy_follows_constraint_set():
+    """The loop keeps the AMG hierarchy across load steps whose constraint set is the
+    same (the device fingerprint of the constrained DOFs matches) and rebuilds it when
+    the set changes (the top face also held in x)."""
+    L = P.lib()
+    k = 12
+    n, x, t, c = P.build_sc_model(P.sc_box_for(k))
+    steps = P.sc_uniaxial_steps(k - 1.0, np.array([2e-4, 4e-4, 6e-4]))
+    s = P.solver(precision=P.LPMX_FP64, rtol2=1e-8, atol2=0.0, maxit=100 * 3 * n, newton_tol=1e-10,
+                 newton_maxit=600, accel_depth=8, precond=2)
+
+    def setup_time():
+        lev, ts, dev = C.c_int(), C.c_double(), C.c_int64()
+        P.check(L.lpmx_amg_info(C.byref(lev), C.byref(ts), C.byref(dev)), "amg info")
+        assert lev.value >= 2
+        return ts.value
+
+    rc, _ = P.run_steps(steps[:1], s)
+    assert rc == 0, P.lib().lpmx_last_error()
+    t1 = setup_time()
+    rc, _ = P.run_steps(steps[1:2], s)
+    assert rc == 0, P.lib().lpmx_last_error()
+    assert setup_time() == t1, "same constraint set: the hierarchy is kept"
+    d, f = steps[2]
+    rc, _ = P.run_steps([(d + [(1, "x", 0.0)], f)], s)
+    assert rc == 0, P.lib().lpmx_last_error()
+    assert setup_time() != t1, "another constraint set: the hierarchy is rebuilt"
+
+
 @pytest.mark.parametrize("prec", [P.LPMX_FP64, P.LPMX_MIXED])
 def test_load_steps_amg_match_oracle(oracle, prec):
     """The fused loop with the AMG preconditioner (12^3: two levels) reaches the
