@@ -13,7 +13,7 @@
  * Execution is serial, which is the race-free specification of the reference's
  * OpenMP loops.
  *
- * Pins (what each function is checked against; see tests/test_oracle_*.py):
+ * Pins (what each function is checked against; see tests/test_oracle.py):
  *   orc_create_lattice      closed-form particle counts and spacings
  *   orc_neighbor_search     closed-form lattice shells; brute force vs cell list
  *   orc_afem_connectivity   closed form |conn| = 61 (SC/FCC interior), 31 (hex);
@@ -25,6 +25,14 @@
  *                           test (interior stress = C:eps); radial-return closed form
  *   orc_solve_cg            dense direct solve (numpy) on small systems
  *   orc_damage_*            uniform-field closed form; brute force shell sort
+ *   orc_slip_systems_fcc    the 24 Schmid tensors: traceless, unit norm, [001] Schmid
+ *                           factors 1/sqrt(6) on the eight tied systems
+ *   orc_lu_solve            dense direct solve (numpy)
+ *   orc_compute_cab         C_ab = -dRSS_a/dgamma_b of the return map's stress path
+ *   orc_bond_force_cp       single-slip return map against its scalar closed form
+ *   orc_run_steps           Anderson-accelerated and plain loops reach one equilibrium
+ *   orc_vjp_j2              central differences of orc_eval_j2 along random directions
+ *   orc_adjoint_run         central differences of the whole forward history
  * MKL's internal iterate sequence (dcg) is closed source: parity of solverCG is
  * checked on converged solutions only ("parity unpinned" for iterates).
  */
