@@ -22,7 +22,8 @@
  *   orc_tangent_fd          symmetry, translation invariance, analytic Jacobian
  *   orc_tangent_analytic    central finite differences of orc_internal_force
  *   orc_bond_force_*        zero force at the reference configuration; affine patch
- *                           test (interior stress = C:eps); radial-return closed form
+ *                           test (interior stress = C:eps); radial-return closed form;
+ *                           the predictor continues the elastic law between two states
  *   orc_solve_cg            dense direct solve (numpy) on small systems
  *   orc_damage_*            uniform-field closed form; brute force shell sort
  *   orc_slip_systems_fcc    the 24 Schmid tensors: traceless, unit norm, [001] Schmid

@@ -173,6 +173,38 @@ def test_reference_config_and_affine_patch(oracle, kind):
     assert np.abs(m.Pin.reshape(-1, 3)[deep]).max() < 1e-6 * np.abs(m.F).max()
 
 
+@pytest.mark.parametrize("kind", ["sc", "hex"])
+def test_predictor_continues_the_elastic_law(oracle, kind):
+    """The incremental predictor (plmode 4) adds the elastic law's response to the
+    stretch increment dis(x) - dis(x_temp) to F_temp.  The elastic law is linear in
+    the stretch, so with F_temp the elastic forces at x_temp the predicted forces at x
+    are the elastic forces at x, to round-off, for any two deformed states."""
+    m, _ = sc(oracle, 7.1, sigmay=1e9) if kind == "sc" else hexm(oracle, 12.0, 10.0, sigmay=1e9)
+    rng = np.random.default_rng(31)
+    e1 = 2e-3 * rng.standard_normal((3, 3))
+    e2 = 1e-3 * rng.standard_normal((3, 3))
+    if kind == "hex":
+        e1[2, :] = e1[:, 2] = e2[2, :] = e2[:, 2] = 0.0
+    x0 = m.x0.copy()
+    noise = 1e-4 * rng.standard_normal(x0.shape) * (1 if kind == "sc" else np.array([1.0, 1.0, 0.0]))
+    xt = x0 + x0 @ e1.T + noise
+    x1 = xt + x0 @ e2.T + 0.5 * noise[::-1]
+    m.x[:] = xt
+    oracle.lib().orc_bond_force_general(m.p, 6)
+    m.Ftemp[:] = m.F
+    m.xt[:] = xt
+    m.x[:] = x1
+    oracle.lib().orc_bond_force_general(m.p, 4)
+    fpred = m.F.copy()
+    oracle.lib().orc_bond_force_general(m.p, 6)
+    live = m.nbr >= 0
+    scale = np.abs(m.F[live]).max()
+    assert scale > 0
+    assert np.abs(fpred[live] - m.F[live]).max() <= 1e-10 * scale
+    # and a plausible slip (the increment's sign) is visible at that tolerance
+    assert np.abs(m.Ftemp[live] - m.F[live]).max() > 1e-3 * scale
+
+
 def test_hex_affine_patch_plane(oracle):
     m, (c11, c12, c44) = hexm(oracle, 12.0, 10.0, sigmay=1e9)
     eps = 1e-7 * np.array([[1.0, 0.4, 0], [0.4, -0.6, 0], [0, 0, 0]])
