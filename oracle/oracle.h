@@ -24,6 +24,8 @@
  *   orc_bond_force_*        zero force at the reference configuration; affine patch
  *                           test (interior stress = C:eps); radial-return closed form;
  *                           the predictor continues the elastic law between two states
+ *   orc_bond_force_j2energy serial order = Jacobi form bit for bit; the elastic branch
+ *                           is the elastic law (the plastic energy formula: unpinned)
  *   orc_solve_cg            dense direct solve (numpy) on small systems
  *   orc_damage_*            uniform-field closed form; brute force shell sort
  *   orc_slip_systems_fcc    the 24 Schmid tensors: traceless, unit norm, [001] Schmid
@@ -209,6 +211,13 @@ void orc_vjp_j2(orc_model *m, const double *Pb, const double *dLp2b, const doubl
 int orc_adjoint_run(orc_model *m, const orc_solver *s, int nsteps, const int *nbd, const orc_dispbc *dbc,
                     const int *nbf, const orc_forcebc *fbc, int maxbc, char axis, int t1, const double *target,
                     double *J, double *reactions, double *g_emod, double *g_sigmay, double *g_H);
+/* computeBondForceJ2energyReturnMap (plmode 3, reference src/constitutive.c:286-463)
+ * for every particle with load indicator t (+1 tension, -1 compression), then the
+ * stress and switchStateV(2).  jacobi = 0: the reference's serial order, each
+ * particle recomputing its list from slot 0; jacobi = 1: every particle's update
+ * once, all second stretch passes, then every force (the form a parallel version
+ * takes; without damage it gives the same values). */
+void orc_bond_force_j2energy(orc_model *m, int t, int jacobi);
 void orc_compute_stress(orc_model *m);
 void orc_switch_state(orc_model *m, int flag);
 void orc_bond_force_general(orc_model *m, int mode);

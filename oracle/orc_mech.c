@@ -307,6 +307,181 @@ void orc_bond_force_j2(orc_model *m, int ii)
     free(xdLp);
 }
 
+/* The energy-based J2 return map of one particle i of a list (reference
+ * src/constitutive.c:336-398): trial distortional energy from the particle's own
+ * stretches and shell sums (the first stretch pass), equivalent stress, yield test
+ * against the slot-0 state, bisection for the multiplier on [0, 1] to TOLITER, then
+ * the updated equivalent plastic strain and back stress and the plastic stretch
+ * increment of each of the first nb[i] bonds, added to xdLp.  Reads only particle i's
+ * arrays and its slot-0 state. */
+#define ORC_TOLITER 1e-4 /* the reference's TOLITER (include/lpm.h:41) */
+static void j2energy_update(orc_model *m, int i, int t, double *xdLp, double *xalpha, double *xbeta_eq,
+                            double *xdlam)
+{
+    int nb1 = 0;
+    for (int c = 0; c < m->nb1max; c++)
+        if (m->nbr1[(int64_t)i * m->nb1max + c] != -1)
+            nb1++;
+    int nb2 = m->nb[i] - nb1;
+    double U_d = 0.0, J2_k = 0.0, J2_sigma = 0.0;
+    double J2_V = m->volume * m->nb[i] / m->nbmax;
+    for (int j = 0; j < m->nb[i]; j++)
+    {
+        int64_t b = B(m, i, j);
+        if (m->nsign[b] == 0)
+        {
+            U_d += 0.5 * m->Kn[b] * (m->dL[b] - m->dLtot[2 * i + 0] / nb1) * (m->dL[b] - m->dLtot[2 * i + 0] / nb1);
+            J2_k = m->Kn[b];
+        }
+        else if (m->nsign[b] == 1)
+            U_d += 0.5 * m->Kn[b] * (m->dL[b] - m->dLtot[2 * i + 1] / nb2) * (m->dL[b] - m->dLtot[2 * i + 1] / nb2);
+    }
+    const double c44 = m->Ce[3 * m->type[i] + 2] * emod(m, i);
+    J2_sigma = sqrt(6.0 * c44 * U_d / J2_V);
+    const double beq0 = m->beta_eq[3 * i + 0], al0 = m->alpha[3 * i + 0], H = m->J2_H, xi = m->J2_xi;
+    double dlambda = 0.0;
+    double yield_func = fabs(t * J2_sigma - beq0) - (m->sigmay[i] + (1.0 - xi) * H * al0);
+    if (yield_func > 0.0)
+    {
+        m->pl_flag[i] = 1;
+        double a = 0.0, bb = 1.0, ya = yield_func;
+        while ((bb - a) > ORC_TOLITER)
+        {
+            dlambda = (a + bb) / 2.0;
+            yield_func = fabs(t * J2_sigma / (1.0 + 0.5 * dlambda) -
+                              (beq0 + t * xi * H * dlambda * J2_sigma * sqrt(m->radius / J2_k / c44) / 6.0 /
+                                          (1.0 + 0.5 * dlambda))) -
+                         (m->sigmay[i] + (1.0 - xi) * H * al0 +
+                          (1.0 - xi) * H * dlambda * J2_sigma * sqrt(m->radius / J2_k / c44) / 6.0 /
+                              (1.0 + 0.5 * dlambda));
+            if (yield_func * ya < 0.0)
+                bb = dlambda;
+            else
+            {
+                a = dlambda;
+                ya = yield_func;
+            }
+        }
+    }
+    *xdlam = dlambda;
+    *xalpha += dlambda / (1.0 + 0.5 * dlambda) * sqrt(m->radius / 6.0 / J2_k * U_d / J2_V);
+    *xbeta_eq = (*xbeta_eq + t * xi * H * dlambda / (1.0 + 0.5 * dlambda) * sqrt(m->radius / 6.0 / J2_k * U_d / J2_V));
+    double f_d = 0.0;
+    for (int j = 0; j < m->nb[i]; j++)
+    {
+        int64_t b = B(m, i, j);
+        int s = m->nsign[b];
+        if (s == 0)
+            f_d = 2.0 * m->Kn[b] / (1 + dlambda / 2.0) * (m->dL[b] - m->dLtot[2 * i + s] / nb1);
+        if (s == 1)
+            f_d = 2.0 * m->Kn[b] / (1 + dlambda / 2.0) * (m->dL[b] - m->dLtot[2 * i + s] / nb2);
+        m->ddLp[b] = dlambda * f_d / (4.0 * m->Kn[b]);
+        m->ddLp[b] *= m->dmg_broken[b];
+        xdLp[j] += m->ddLp[b];
+    }
+}
+
+/* Final force of particle ii from the post-update stretches of itself and its
+ * partners (reference src/constitutive.c:422-440): half-averaged elastic stretch,
+ * shell sums of both ends, times 1 - damage_D of slot 0. */
+static void j2energy_force(orc_model *m, int ii)
+{
+    const int i = ii, nbm = m->nbmax;
+    double *P = m->Pin + 3 * i;
+    P[0] = 0.0, P[1] = 0.0, P[2] = 0.0;
+    for (int j = 0; j < m->nb_initial[i]; j++)
+    {
+        int64_t b = B(m, i, j);
+        int q = m->nbr[b], s = m->nsign[b];
+        for (int jj = 0; jj < nbm; jj++)
+            if (m->nbr[B(m, q, jj)] == i)
+                m->dL_ave[b] = 0.5 * (m->dL[b] + m->dL[B(m, q, jj)]);
+        m->F[b] = 2.0 * m->Kn[b] * m->dL_ave[b] + 0.5 * (m->TdLtot[2 * i + s] + m->TdLtot[2 * q + s]) +
+                  0.5 * m->Tv[b] * (m->dLtot[2 * i + s] + m->dLtot[2 * q + s]);
+        m->F[b] *= (1.0 - m->dmgD[2 * b + 0]);
+        P[0] += m->cs[3 * b + 0] * m->F[b];
+        P[1] += m->cs[3 * b + 1] * m->F[b];
+        P[2] += m->cs[3 * b + 2] * m->F[b];
+    }
+}
+
+/* Slot 2 of particle ii (reference src/constitutive.c:442-447). */
+static void j2energy_store(orc_model *m, int ii, const double *xdLp, double xalpha, double xbeta_eq, double xdlam)
+{
+    for (int j = 0; j < m->nbmax; j++)
+        m->dLp[3 * B(m, ii, j) + 2] = m->dmg_broken[B(m, ii, j)] * xdLp[j];
+    m->beta_eq[3 * ii + 2] = xbeta_eq;
+    m->alpha[3 * ii + 2] = xalpha;
+    m->dlambda[ii] = xdlam;
+}
+
+/* computeBondForceJ2energyReturnMap (reference src/constitutive.c:286-463), one
+ * particle ii in the reference's order: its list {ii} and unbroken partners from
+ * slot 0, first stretch pass of the list, the update of every list member, second
+ * stretch pass, the force of ii, slot 2 of ii.  The shared scratch (dL, the shell
+ * sums, the cosines, ddLp) is written for the whole list, as in the reference. */
+static void j2energy_particle(orc_model *m, int ii, int t)
+{
+    int lst[64];
+    int nl = particle_list(m, ii, lst);
+    const int nbm = m->nbmax;
+    double *xdLp = malloc(sizeof(double) * nl * nbm);
+    double xalpha[64], xbeta_eq[64], xdlam[64];
+    for (int k = 0; k < nl; k++)
+    {
+        int i = lst[k];
+        for (int j = 0; j < nbm; j++)
+            xdLp[k * nbm + j] = m->dLp[3 * B(m, i, j) + 0];
+        xbeta_eq[k] = m->beta_eq[3 * i + 0];
+        xalpha[k] = m->alpha[3 * i + 0];
+        xdlam[k] = 0.0;
+    }
+    for (int k = 0; k < nl; k++)
+        stretch_pass(m, lst[k], xdLp + k * nbm);
+    for (int k = 0; k < nl; k++)
+        j2energy_update(m, lst[k], t, xdLp + k * nbm, &xalpha[k], &xbeta_eq[k], &xdlam[k]);
+    for (int k = 0; k < nl; k++)
+        stretch_pass(m, lst[k], xdLp + k * nbm);
+    j2energy_force(m, ii);
+    j2energy_store(m, ii, xdLp, xalpha[0], xbeta_eq[0], xdlam[0]);
+    free(xdLp);
+}
+
+void orc_bond_force_j2energy(orc_model *m, int t, int jacobi)
+{
+    const int n = m->n, nbm = m->nbmax;
+    if (!jacobi)
+        for (int ii = 0; ii < n; ii++)
+            j2energy_particle(m, ii, t);
+    else
+    {
+        double *xdLp = malloc(sizeof(double) * (size_t)n * nbm), *xa = malloc(sizeof(double) * n),
+               *xb = malloc(sizeof(double) * n), *xl = malloc(sizeof(double) * n);
+        for (int i = 0; i < n; i++)
+        {
+            for (int j = 0; j < nbm; j++)
+                xdLp[(int64_t)i * nbm + j] = m->dLp[3 * B(m, i, j) + 0];
+            xb[i] = m->beta_eq[3 * i + 0];
+            xa[i] = m->alpha[3 * i + 0];
+            xl[i] = 0.0;
+        }
+        for (int i = 0; i < n; i++)
+            stretch_pass(m, i, xdLp + (int64_t)i * nbm);
+        for (int i = 0; i < n; i++)
+            j2energy_update(m, i, t, xdLp + (int64_t)i * nbm, &xa[i], &xb[i], &xl[i]);
+        for (int i = 0; i < n; i++)
+            stretch_pass(m, i, xdLp + (int64_t)i * nbm);
+        for (int i = 0; i < n; i++)
+        {
+            j2energy_force(m, i);
+            j2energy_store(m, i, xdLp + (int64_t)i * nbm, xa[i], xb[i], xl[i]);
+        }
+        free(xdLp), free(xa), free(xb), free(xl);
+    }
+    orc_compute_stress(m);
+    orc_switch_state(m, 2);
+}
+
 /* computeBondForceIncrementalUpdating (reference src/constitutive.c:167-225): the
  * elastic predictor with stretch increments from xyz_temp; Pin uses the direction
  * cosines left by the previous force evaluation. */

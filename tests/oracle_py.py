@@ -113,6 +113,8 @@ def lib():
                   "orc_switch_state", "orc_bond_force_general", "orc_bond_force_cp"):
             getattr(L, f).argtypes = [P, C.c_int]
             getattr(L, f).restype = None
+        L.orc_bond_force_j2energy.argtypes = [P, C.c_int, C.c_int]
+        L.orc_bond_force_j2energy.restype = None
         L.orc_internal_force.argtypes = [P, _dp, _dp]
         L.orc_set_disp_bc.argtypes = [P, C.c_int, C.POINTER(OrcDispBC)]
         L.orc_set_force_bc.argtypes = [P, C.c_int, C.POINTER(OrcForceBC)]

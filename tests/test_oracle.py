@@ -234,6 +234,50 @@ def test_radial_return_closed_form(oracle):
     assert np.all(m.pl_flag[deep] == 1)
 
 
+def j2energy_pair(oracle, sigmay, seed):
+    """Two identical SC models with a random slot-0 state and a random deformation."""
+    out = []
+    for _ in range(2):
+        m, c = sc(oracle, 7.1, sigmay=sigmay, J2_H=2000.0, J2_xi=0.3)
+        rng = np.random.default_rng(seed)
+        live = m.nbr >= 0
+        if sigmay < 1e6:
+            m.dLp[:, :, 0] = 2e-5 * rng.standard_normal(live.shape) * live
+            m.alpha[:, 0] = 1e-3 * rng.uniform(size=m.n)
+            m.beta_eq[:, 0] = 20.0 * rng.standard_normal(m.n)
+        eps = 4e-3 * np.array([[1.0, 0.2, 0.0], [0.2, -0.5, 0.1], [0.0, 0.1, -0.5]])
+        m.x[:] = m.x0 + m.x0 @ eps.T + 1e-4 * rng.standard_normal(m.x.shape)
+        out.append(m)
+    return out
+
+
+def test_j2energy_serial_order_equals_jacobi_form(oracle):
+    """The energy-based J2 law (plmode 3) recomputes each particle's list from slot 0
+    and writes only the particle's own slot 2, so the reference's serial sweep gives
+    the values of the Jacobi form (every update once, all second stretch passes,
+    then every force), bit for bit, with a good share of the particles yielding."""
+    ms, mj = j2energy_pair(oracle, 200.0, 41)
+    oracle.lib().orc_bond_force_j2energy(ms.p, 1, 0)
+    oracle.lib().orc_bond_force_j2energy(mj.p, 1, 1)
+    assert ms.pl_flag.sum() > ms.n // 4
+    assert 0 < ms.dlambda.max() < 1
+    for k in ("F", "Pin", "dL", "dL_ave", "ddLp", "dLtot", "TdLtot", "cs", "stress", "dLp", "alpha", "beta_eq",
+              "dlambda", "pl_flag"):
+        np.testing.assert_array_equal(getattr(ms, k), getattr(mj, k), err_msg=k)
+
+
+def test_j2energy_elastic_branch_is_the_elastic_law(oracle):
+    """Below yield the energy-based J2 law has no plastic stretch, the half-averaged
+    stretch of a bond equals its own, and its forces are the elastic law's."""
+    m3, m6 = j2energy_pair(oracle, 1e9, 43)
+    oracle.lib().orc_bond_force_j2energy(m3.p, 1, 0)
+    oracle.lib().orc_bond_force_general(m6.p, 6)
+    assert m3.pl_flag.sum() == 0 and np.all(m3.dlambda == 0)
+    np.testing.assert_array_equal(m3.F, m6.F)
+    np.testing.assert_array_equal(m3.Pin, m6.Pin)
+    np.testing.assert_array_equal(m3.stress, m6.stress)
+
+
 # ---------------------------------------------------------------- tangent --------
 def test_analytic_jacobian_matches_central_differences(oracle):
     m, _ = sc(oracle, 3.2)
