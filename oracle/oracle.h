@@ -26,6 +26,8 @@
  *                           the predictor continues the elastic law between two states
  *   orc_bond_force_j2energy serial order = Jacobi form bit for bit; the elastic branch
  *                           is the elastic law (the plastic energy formula: unpinned)
+ *   orc_bond_force_j2nonlinear the elastic branch is the elastic law; the Jacobi
+ *                           multiplier is the root of the hardening law to TOLITER
  *   orc_solve_cg            dense direct solve (numpy) on small systems
  *   orc_damage_*            uniform-field closed form; brute force shell sort
  *   orc_slip_systems_fcc    the 24 Schmid tensors: traceless, unit norm, [001] Schmid
@@ -218,6 +220,14 @@ int orc_adjoint_run(orc_model *m, const orc_solver *s, int nsteps, const int *nb
  * once, all second stretch passes, then every force (the form a parallel version
  * takes; without damage it gives the same values). */
 void orc_bond_force_j2energy(orc_model *m, int t, int jacobi);
+/* computeBondForceJ2nonlinearIso (plmode 5, reference src/constitutive.c:689-863)
+ * with kinematic modulus C (the reference's J2_C), then the stress and
+ * switchStateV(2).  jacobi = 0: the reference's serial order, in which every member
+ * of a particle's list advances its slot-0 state in place, so a particle is advanced
+ * once per list it is in; jacobi = 1: every particle advanced once from the state at
+ * the start.  Neither writes slot 2, so the switch that follows restores slot 0 from
+ * it, as in the reference. */
+void orc_bond_force_j2nonlinear(orc_model *m, double C, int jacobi);
 void orc_compute_stress(orc_model *m);
 void orc_switch_state(orc_model *m, int flag);
 void orc_bond_force_general(orc_model *m, int mode);

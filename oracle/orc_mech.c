@@ -482,6 +482,149 @@ void orc_bond_force_j2energy(orc_model *m, int t, int jacobi)
     orc_switch_state(m, 2);
 }
 
+/* The reference's isotropic hardening law SY (include/lpm.h:50), MPa. */
+static inline double orc_SY(double x) { return 620.0 + 3300.0 * (1.0 - exp(-0.4 * x)); }
+
+/* The nonlinear-isotropic J2 return map of one particle i (reference
+ * src/constitutive.c:726-812): trial bond forces and stress of i from its first-pass
+ * stretches, deviator minus the back stress, bisection of the multiplier on [0, 1] to
+ * TOLITER, then the multiplier, the accumulated plastic strain, the back stress and
+ * the plastic stretch of slot 0 advanced in place.  C is the kinematic modulus J2_C. */
+static void j2nl_update(orc_model *m, int i, double C)
+{
+    const double J2_V = m->volume * m->nb[i] / m->nbmax;
+    double sl[6] = {0, 0, 0, 0, 0, 0}, dpl[6];
+    for (int j = 0; j < m->nb_initial[i]; j++)
+    {
+        int64_t b = B(m, i, j);
+        int s = m->nsign[b];
+        m->F[b] = 2.0 * m->Kn[b] * m->dL[b] + m->TdLtot[2 * i + s] + m->Tv[b] * m->dLtot[2 * i + s];
+        m->F[b] *= m->dmg_broken[b];
+        const double cx = m->cs[3 * b], cy = m->cs[3 * b + 1], cz = m->cs[3 * b + 2];
+        sl[0] += 0.5 / J2_V * m->L0[b] * m->F[b] * cx * cx;
+        sl[1] += 0.5 / J2_V * m->L0[b] * m->F[b] * cy * cy;
+        sl[2] += 0.5 / J2_V * m->L0[b] * m->F[b] * cz * cz;
+        sl[3] += 0.5 / J2_V * m->L0[b] * m->F[b] * cy * cz;
+        sl[4] += 0.5 / J2_V * m->L0[b] * m->F[b] * cx * cz;
+        sl[5] += 0.5 / J2_V * m->L0[b] * m->F[b] * cx * cy;
+    }
+    double temp = 1.0 / 3.0 * (sl[0] + sl[1] + sl[2]);
+    for (int c = 0; c < 3; c++)
+        sl[c] -= temp;
+    for (int c = 0; c < 6; c++)
+        sl[c] -= m->beta[(6 * i + c) * 3 + 0];
+    double seq = 0.0;
+    for (int c = 0; c < 6; c++)
+    {
+        if (c < 3)
+            seq += sl[c] * sl[c];
+        else
+            seq += 2.0 * sl[c] * sl[c];
+    }
+    seq = sqrt(3.0 / 2.0 * seq);
+    const double c44 = m->Ce[3 * m->type[i] + 2] * emod(m, i), al0 = m->alpha[3 * i + 0];
+    double dlambda = 0.0;
+    double yield_func = seq - orc_SY(al0);
+    if (yield_func > 0.0)
+    {
+        double a = 0.0, bb = 1.0, ya = yield_func;
+        while ((bb - a) > ORC_TOLITER)
+        {
+            dlambda = (a + bb) / 2.0;
+            yield_func = seq - 1.5 * dlambda * (2.0 * c44 + C) - orc_SY(al0 + dlambda);
+            if (yield_func * ya < 0.0)
+                bb = dlambda;
+            else
+            {
+                a = dlambda;
+                ya = yield_func;
+            }
+        }
+    }
+    m->dlambda[i] = dlambda;
+    m->alpha[3 * i + 0] += m->dlambda[i];
+    for (int c = 0; c < 6; c++)
+    {
+        dpl[c] = dlambda * 1.5 * sl[c] / seq;
+        m->beta[(6 * i + c) * 3 + 0] += C * dpl[c];
+    }
+    for (int j = 0; j < m->nb_initial[i]; j++)
+    {
+        int64_t b = B(m, i, j);
+        const double cx = m->cs[3 * b], cy = m->cs[3 * b + 1], cz = m->cs[3 * b + 2];
+        m->ddLp[b] = m->L0[b] * (dpl[0] * cx * cx + dpl[1] * cy * cy + dpl[2] * cz * cz + 2 * dpl[3] * cy * cz +
+                                 2 * dpl[4] * cx * cz + 2 * dpl[5] * cx * cy);
+        m->dLp[3 * b + 0] += m->ddLp[b];
+    }
+}
+
+/* Stretch pass of particle i with its current slot-0 plastic stretch. */
+static void j2nl_stretch(orc_model *m, int i)
+{
+    double xd[64];
+    for (int j = 0; j < m->nbmax; j++)
+        xd[j] = m->dLp[3 * B(m, i, j) + 0];
+    stretch_pass(m, i, xd);
+}
+
+/* Final force of particle ii (reference src/constitutive.c:834-856). */
+static void j2nl_force(orc_model *m, int ii)
+{
+    const int i = ii;
+    for (int c = 0; c < 6; c++)
+        m->stress[6 * i + c] = 0.0;
+    double *P = m->Pin + 3 * i;
+    P[0] = 0.0, P[1] = 0.0, P[2] = 0.0;
+    for (int j = 0; j < m->nb_initial[i]; j++)
+    {
+        int64_t b = B(m, i, j);
+        int q = m->nbr[b], s = m->nsign[b];
+        for (int jj = 0; jj < m->nbmax; jj++)
+            if (m->nbr[B(m, q, jj)] == i)
+                m->dL_ave[b] = 0.5 * (m->dL[b] + m->dL[B(m, q, jj)]);
+        m->F[b] = 2.0 * m->Kn[b] * m->dL_ave[b] + 0.5 * (m->TdLtot[2 * i + s] + m->TdLtot[2 * q + s]) +
+                  0.5 * m->Tv[b] * (m->dLtot[2 * i + s] + m->dLtot[2 * q + s]);
+        m->F[b] *= m->dmg_w[b];
+        P[0] += m->cs[3 * b + 0] * m->F[b];
+        P[1] += m->cs[3 * b + 1] * m->F[b];
+        P[2] += m->cs[3 * b + 2] * m->F[b];
+    }
+}
+
+void orc_bond_force_j2nonlinear(orc_model *m, double C, int jacobi)
+{
+    const int n = m->n;
+    if (!jacobi)
+        /* computeBondForceJ2nonlinearIso (reference src/constitutive.c:689-863) per
+         * particle ii in order: the list's stretches from the current slot 0, every
+         * list member's update in place, the list's stretches again, the force of ii */
+        for (int ii = 0; ii < n; ii++)
+        {
+            int lst[64];
+            int nl = particle_list(m, ii, lst);
+            for (int k = 0; k < nl; k++)
+                j2nl_stretch(m, lst[k]);
+            for (int k = 0; k < nl; k++)
+                j2nl_update(m, lst[k], C);
+            for (int k = 0; k < nl; k++)
+                j2nl_stretch(m, lst[k]);
+            j2nl_force(m, ii);
+        }
+    else
+    {
+        for (int i = 0; i < n; i++)
+            j2nl_stretch(m, i);
+        for (int i = 0; i < n; i++)
+            j2nl_update(m, i, C);
+        for (int i = 0; i < n; i++)
+            j2nl_stretch(m, i);
+        for (int i = 0; i < n; i++)
+            j2nl_force(m, i);
+    }
+    orc_compute_stress(m);
+    orc_switch_state(m, 2);
+}
+
 /* computeBondForceIncrementalUpdating (reference src/constitutive.c:167-225): the
  * elastic predictor with stretch increments from xyz_temp; Pin uses the direction
  * cosines left by the previous force evaluation. */

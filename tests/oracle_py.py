@@ -115,6 +115,8 @@ def lib():
             getattr(L, f).restype = None
         L.orc_bond_force_j2energy.argtypes = [P, C.c_int, C.c_int]
         L.orc_bond_force_j2energy.restype = None
+        L.orc_bond_force_j2nonlinear.argtypes = [P, C.c_double, C.c_int]
+        L.orc_bond_force_j2nonlinear.restype = None
         L.orc_internal_force.argtypes = [P, _dp, _dp]
         L.orc_set_disp_bc.argtypes = [P, C.c_int, C.POINTER(OrcDispBC)]
         L.orc_set_force_bc.argtypes = [P, C.c_int, C.POINTER(OrcForceBC)]

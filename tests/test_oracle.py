@@ -266,6 +266,65 @@ def test_j2energy_serial_order_equals_jacobi_form(oracle):
         np.testing.assert_array_equal(getattr(ms, k), getattr(mj, k), err_msg=k)
 
 
+def j2nl_model(oracle, strain, noise):
+    m, c = sc(oracle, 7.1, sigmay=1e9)
+    eps = strain * np.array([[1.0, 0.2, 0.0], [0.2, -0.5, 0.1], [0.0, 0.1, -0.5]])
+    m.x[:] = m.x0 + m.x0 @ eps.T + noise * np.random.default_rng(5).standard_normal(m.x.shape)
+    return m, c, eps
+
+
+@pytest.mark.parametrize("jacobi", [0, 1])
+def test_j2nonlinear_elastic_branch_is_the_elastic_law(oracle, jacobi):
+    """Below SY(0) = 620 MPa the nonlinear-isotropic J2 law (plmode 5) leaves the state
+    alone and its internal forces are the elastic law's."""
+    m5, _, _ = j2nl_model(oracle, 5e-4, 1e-5)
+    m6, _, _ = j2nl_model(oracle, 5e-4, 1e-5)
+    oracle.lib().orc_bond_force_j2nonlinear(m5.p, 0.0, jacobi)
+    oracle.lib().orc_bond_force_general(m6.p, 6)
+    assert np.all(m5.dlambda == 0)
+    np.testing.assert_array_equal(m5.Pin, m6.Pin)
+
+
+def test_j2nonlinear_return_map_closed_form(oracle):
+    """Under a uniform deviatoric strain a deep-interior particle's trial stress is
+    C:eps, and the multiplier of the Jacobi form is the root of
+    sigma_eq - 3 mu dl - SY(dl) = 0 (SY(a) = 620 + 3300 (1 - exp(-0.4 a)), no
+    kinematic hardening) to the bisection's bracket TOLITER = 1e-4."""
+    from scipy.optimize import brentq
+
+    m, c, eps = j2nl_model(oracle, 4e-3, 0.0)
+    oracle.lib().orc_bond_force_j2nonlinear(m.p, 0.0, 1)
+    deep = deep_interior(m)
+    ev = np.array([eps[0, 0], eps[1, 1], eps[2, 2], 2 * eps[1, 2], 2 * eps[0, 2], 2 * eps[0, 1]])
+    s = voigt_C(*c) @ ev
+    s[:3] -= s[:3].mean()
+    seq = np.sqrt(1.5 * (np.sum(s[:3] ** 2) + 2 * np.sum(s[3:] ** 2)))
+    assert seq > 620.0
+    root = brentq(lambda dl: seq - 1.5 * dl * 2.0 * c[2] - (620.0 + 3300.0 * (1.0 - np.exp(-0.4 * dl))), 0.0, 1.0,
+                  xtol=1e-15)
+    assert np.all(np.abs(m.dlambda[deep] - root) <= 1.5e-4), (m.dlambda[deep].min(), m.dlambda[deep].max(), root)
+
+
+def test_j2nonlinear_keeps_no_plastic_state(oracle):
+    """plmode 5 advances slot 0 in place and writes no slot 2; the switchStateV(2) that
+    ends every evaluation copies slot 2 over slot 0, so a yielding evaluation leaves no
+    plastic state behind (the reference's behaviour, kept by the restatement).  In the
+    serial order a particle's multiplier is the one of its last recomputation, from a
+    later neighbour's list, which starts on the yield surface and finds none."""
+    mj, _, _ = j2nl_model(oracle, 4e-3, 1e-5)
+    ms, _, _ = j2nl_model(oracle, 4e-3, 1e-5)
+    oracle.lib().orc_bond_force_j2nonlinear(mj.p, 0.0, 1)
+    oracle.lib().orc_bond_force_j2nonlinear(ms.p, 0.0, 0)
+    assert (mj.dlambda > 0).sum() > mj.n // 4, "the deformation yields"
+    for m in (mj, ms):
+        assert np.all(m.alpha[:, 0] == 0) and np.all(m.dLp[:, :, 0] == 0)
+    # the serial sweep did advance the particles (its forces are not the elastic law's)
+    me, _, _ = j2nl_model(oracle, 4e-3, 1e-5)
+    oracle.lib().orc_bond_force_general(me.p, 6)
+    assert np.abs(ms.Pin - me.Pin).max() > 1e-3 * np.abs(me.Pin).max()
+    assert np.all(ms.dlambda[deep_interior(ms)] == 0)
+
+
 def test_j2energy_elastic_branch_is_the_elastic_law(oracle):
     """Below yield the energy-based J2 law has no plastic stretch, the half-averaged
     stretch of a bond equals its own, and its forces are the elastic law's."""
