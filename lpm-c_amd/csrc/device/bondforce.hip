@@ -238,6 +238,7 @@ __global__ __launch_bounds__(256) void k_j2_particle_u(
     double *__restrict__ ddLp, double *__restrict__ dLp2, double *__restrict__ A, double *__restrict__ Bt,
     double *__restrict__ beta2, double *__restrict__ alpha2, double *__restrict__ dlam_, int *__restrict__ pl_flag)
 {
+    static_assert(NBM <= 32, "the shell bits of a particle's bonds share one 32-bit word");
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n)
         return;
