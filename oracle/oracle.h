@@ -28,6 +28,8 @@
  *                           is the elastic law (the plastic energy formula: unpinned)
  *   orc_bond_force_j2nonlinear the elastic branch is the elastic law; the Jacobi
  *                           multiplier is the root of the hardening law to TOLITER
+ *   orc_damage_bwise_local  reverse particle order gives the same state; pairwise
+ *                           brute force of the rule, breaks, counts and weights
  *   orc_solve_cg            dense direct solve (numpy) on small systems
  *   orc_damage_*            uniform-field closed form; brute force shell sort
  *   orc_slip_systems_fcc    the 24 Schmid tensors: traceless, unit norm, [001] Schmid
@@ -251,6 +253,9 @@ double orc_cg_baseline(const orc_model *m, int nthreads, int iters, double *rr_o
 
 /* ---- damage and cracks ------------------------------------------------------ */
 int orc_damage_pwise_nonlocal(orc_model *m);
+/* updateDuctileDamageBwiseLocal (the damage of plmode 5): order 0 is the reference's
+ * particle order, 1 the reverse (same result).  Returns the bonds broken. */
+int orc_damage_bwise_local(orc_model *m, int order);
 int orc_damage_brittle(orc_model *m, int *broken_i, int *broken_j, int cap);
 void orc_update_crack(orc_model *m);
 int orc_shell_sort_select(int k, const double *bstrain, int nbreak, int *order);

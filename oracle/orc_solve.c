@@ -318,6 +318,62 @@ int orc_shell_sort_select(int k, const double *bstrain, int nbreak, int *order)
     return k;
 }
 
+/* updateDuctileDamageBwiseLocal (reference src/constitutive.c:1607-1695): local
+ * damage accumulated from the plastic multiplier, bond damage as the mean of the two
+ * ends, a bond (and its reverse entry) broken when that mean passes the threshold,
+ * then the weights.  `order` = 0 visits the particles in the reference's order,
+ * 1 in reverse: the decision on a bond depends only on its two ends, so both give the
+ * same state (the tests check it).  Returns the number of bonds broken (each pair
+ * once). */
+int orc_damage_bwise_local(orc_model *m, int order)
+{
+    const int n = m->n, nbm = m->nbmax;
+    for (int i = 0; i < n; i++)
+    {
+        double damage_f = (1.0 + m->damagec_A * m->triax[i]);
+        if (damage_f > 0.0 && m->dloc[2 * i + 0] <= m->damage_threshold)
+            m->dloc[2 * i + 0] += damage_f * m->dlambda[i];
+        else if (m->dloc[2 * i + 0] > 1.0)
+            m->dloc[2 * i + 0] = 1.0;
+    }
+    int k = 0;
+    for (int t = 0; t < n; t++)
+    {
+        const int i = order ? n - 1 - t : t;
+        m->nb[i] = m->nb_initial[i];
+        for (int j = 0; j < m->nb_initial[i]; j++)
+        {
+            const int64_t b = (int64_t)i * nbm + j;
+            const int q = m->nbr[b];
+            m->dmgD[2 * b + 0] = 0.5 * (m->dloc[2 * i + 0] + m->dloc[2 * q + 0]);
+            if (m->dmgD[2 * b + 0] > m->damage_threshold && m->dmg_broken[b] > O_EPS)
+            {
+                m->dmgD[2 * b + 0] = 1.0;
+                m->dmg_broken[b] = 0.0;
+                for (int jj = 0; jj < nbm; jj++)
+                    if (m->nbr[(int64_t)q * nbm + jj] == i)
+                    {
+                        m->dmgD[2 * ((int64_t)q * nbm + jj) + 0] = 1.0;
+                        m->dmg_broken[(int64_t)q * nbm + jj] = 0.0;
+                    }
+                k++;
+            }
+            if (m->dmg_broken[b] <= O_EPS)
+                m->nb[i] -= 1;
+        }
+    }
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < m->nb_initial[i]; j++)
+        {
+            const int64_t b = (int64_t)i * nbm + j;
+            const int q = m->nbr[b];
+            if (fabs(m->dmg_broken[b]) < O_EPS || m->nb[i] == 0 || m->nb[q] == 0)
+                m->dmgD[2 * b + 0] = 1.0;
+            m->dmg_w[b] = 1.0 - m->dmgD[2 * b + 0];
+        }
+    return k;
+}
+
 /* updateBrittleDamage (reference src/constitutive.c:1437-1526): bonds with
  * dL/L0 >= critical (per-bond critical stretch with the extension), collected in
  * (i, slot) order; all break if there are at most nbreak, otherwise the nbreak

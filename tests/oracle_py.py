@@ -125,6 +125,8 @@ def lib():
         L.orc_spmv_upper.argtypes = [P, _dp, _dp]
         L.orc_solve_cg.argtypes = [P, C.POINTER(OrcSolver), _dp]
         L.orc_solve_cg.restype = C.c_int
+        L.orc_damage_bwise_local.argtypes = [P, C.c_int]
+        L.orc_damage_bwise_local.restype = C.c_int
         L.orc_damage_brittle.argtypes = [P, _ip, _ip, C.c_int]
         L.orc_damage_brittle.restype = C.c_int
         L.orc_shell_sort_select.argtypes = [C.c_int, _dp, C.c_int, _ip]

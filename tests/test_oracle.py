@@ -325,6 +325,60 @@ def test_j2nonlinear_keeps_no_plastic_state(oracle):
     assert np.all(ms.dlambda[deep_interior(ms)] == 0)
 
 
+def bwise_model(oracle, seed):
+    """SC model with a random local damage field, multipliers, triaxialities and a few
+    bonds broken beforehand (both entries)."""
+    m, _ = sc(oracle, 7.1, damage_threshold=0.9, damagec_A=0.5)
+    rng = np.random.default_rng(seed)
+    m.dloc[:, 0] = rng.uniform(0.6, 1.05, m.n)
+    m.dlambda[:] = 0.05 * rng.uniform(size=m.n)
+    m.triax[:] = rng.uniform(-3.0, 1.0, m.n)
+    for _ in range(10):
+        i = int(rng.integers(m.n))
+        j = int(rng.integers(m.nb_initial[i]))
+        q = m.nbr[i, j]
+        m.dmg_broken[i, j] = 0.0
+        m.dmg_broken[q, list(m.nbr[q]).index(i)] = 0.0
+    return m
+
+
+def test_bwise_local_damage_does_not_depend_on_the_order(oracle):
+    """The bond-wise local damage (the damage of plmode 5) decides each bond from its
+    two ends only: visiting the particles in reverse gives the same state and count."""
+    a, b = bwise_model(oracle, 51), bwise_model(oracle, 51)
+    ka = oracle.lib().orc_damage_bwise_local(a.p, 0)
+    kb = oracle.lib().orc_damage_bwise_local(b.p, 1)
+    assert ka == kb and ka > 10
+    for k in ("dloc", "dmgD", "dmg_broken", "nb", "dmg_w"):
+        np.testing.assert_array_equal(getattr(a, k), getattr(b, k), err_msg=k)
+
+
+def test_bwise_local_damage_brute_force(oracle):
+    """Against a pairwise restatement: the local damage rule, a bond broken exactly when
+    the mean damage of its unbroken ends passes the threshold, the counts, and the
+    weights 1 - D with D = 1 on broken bonds and at isolated particles."""
+    m = bwise_model(oracle, 52)
+    thr, A = 0.9, 0.5
+    d0, dl, tx = m.dloc[:, 0].copy(), m.dlambda.copy(), m.triax.copy()
+    live = m.nbr >= 0
+    broken0 = (m.dmg_broken == 0) & live
+    k = oracle.lib().orc_damage_bwise_local(m.p, 0)
+    f = 1.0 + A * tx
+    d1 = np.where((f > 0) & (d0 <= thr), d0 + f * dl, np.where(d0 > 1.0, 1.0, d0))
+    np.testing.assert_allclose(m.dloc[:, 0], d1, rtol=0, atol=1e-15)
+    q = np.where(live, m.nbr, 0)
+    mean = 0.5 * (d1[:, None] + d1[q])
+    newly = live & ~broken0 & (mean > thr)
+    broken = broken0 | newly
+    np.testing.assert_array_equal(m.dmg_broken == 0, broken)
+    assert k == int(newly.sum()) // 2
+    nb = m.nb_initial - broken.sum(axis=1)
+    np.testing.assert_array_equal(m.nb, nb)
+    iso = (nb[:, None] == 0) | (nb[q] == 0)
+    D = np.where(broken | iso, 1.0, mean)
+    np.testing.assert_allclose(m.dmg_w[live], 1.0 - D[live], rtol=0, atol=1e-15)
+
+
 def test_j2energy_elastic_branch_is_the_elastic_law(oracle):
     """Below yield the energy-based J2 law has no plastic stretch, the half-averaged
     stretch of a bond equals its own, and its forces are the elastic law's."""
