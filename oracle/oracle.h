@@ -30,6 +30,7 @@
  *                           multiplier is the root of the hardening law to TOLITER
  *   orc_damage_bwise_local  reverse particle order gives the same state; pairwise
  *                           brute force of the rule, breaks, counts and weights
+ *   orc_damage_pwise_local, orc_damage_bwise_nonlocal  the same (order, brute force)
  *   orc_solve_cg            dense direct solve (numpy) on small systems
  *   orc_damage_*            uniform-field closed form; brute force shell sort
  *   orc_slip_systems_fcc    the 24 Schmid tensors: traceless, unit norm, [001] Schmid
@@ -253,6 +254,11 @@ double orc_cg_baseline(const orc_model *m, int nthreads, int iters, double *rr_o
 
 /* ---- damage and cracks ------------------------------------------------------ */
 int orc_damage_pwise_nonlocal(orc_model *m);
+/* updateDuctileDamageBwiseNonlocal and updateDuctileDamagePwiseLocal, the other
+ * ductile damage variants of the reference (selectable in updateDamageGeneral).
+ * Return the directed bond entries broken and the particles failed. */
+int orc_damage_bwise_nonlocal(orc_model *m);
+int orc_damage_pwise_local(orc_model *m, int order);
 /* updateDuctileDamageBwiseLocal (the damage of plmode 5): order 0 is the reference's
  * particle order, 1 the reverse (same result).  Returns the bonds broken. */
 int orc_damage_bwise_local(orc_model *m, int order);

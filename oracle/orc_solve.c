@@ -318,6 +318,114 @@ int orc_shell_sort_select(int k, const double *bstrain, int nbreak, int *order)
     return k;
 }
 
+/* updateDuctileDamageBwiseNonlocal (reference src/constitutive.c:1529-1605): the
+ * nonlocal damage of every particle below the threshold grows by the Gaussian-
+ * weighted mean (DAM_PHI of the initial bond length, include/lpm.h:51) of the local
+ * rates (1 + A triaxiality) dlambda of itself and its list; then every directed bond
+ * whose two ends average above the threshold breaks, and the weights follow.  Each
+ * update reads only multipliers and triaxialities, and each break only its two ends,
+ * so the particle order does not matter.  Returns the directed entries broken. */
+int orc_damage_bwise_nonlocal(orc_model *m)
+{
+    const int n = m->n, nbm = m->nbmax;
+    const double L = m->damage_L, V = m->volume;
+    for (int i = 0; i < n; i++)
+    {
+        if (m->dnl[2 * i + 0] > m->damage_threshold)
+        {
+            if (m->dnl[2 * i + 0] > 1.0)
+                m->dnl[2 * i + 0] = 1.0;
+            continue;
+        }
+        double DdotLocal = 0;
+        double damage_f = (1.0 + m->damagec_A * m->triax[i]);
+        if (damage_f > 0.0)
+            DdotLocal = m->dlambda[i] * (1.0 + m->damagec_A * m->triax[i]);
+        double Ddot = DdotLocal * V;
+        double A = V;
+        for (int j = 0; j < m->nb_initial[i]; j++)
+        {
+            const int64_t b = (int64_t)i * nbm + j;
+            const int q = m->nbr[b];
+            DdotLocal = 0;
+            damage_f = (1.0 + m->damagec_A * m->triax[q]);
+            if (damage_f > 0.0)
+                DdotLocal = m->dlambda[q] * (1.0 + m->damagec_A * m->triax[q]);
+            const double x = m->L0[b];
+            const double phi = 1.0 / L / sqrt(2 * O_PI) * exp(-0.5 * x * x / L / L);
+            Ddot += DdotLocal * phi * V;
+            A += phi * V;
+        }
+        if (Ddot > 0.0)
+            m->dnl[2 * i + 0] += 1.0 / A * Ddot;
+    }
+    int k = 0;
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < m->nb_initial[i]; j++)
+        {
+            const int64_t b = (int64_t)i * nbm + j;
+            const int q = m->nbr[b];
+            if (0.5 * (m->dnl[2 * i + 0] + m->dnl[2 * q + 0]) > m->damage_threshold && fabs(m->dmg_broken[b]) > O_EPS)
+            {
+                m->dmg_broken[b] = 0.0;
+                m->dmgD[2 * b + 0] = 1.0;
+                k++;
+            }
+        }
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < m->nb_initial[i]; j++)
+        {
+            const int64_t b = (int64_t)i * nbm + j;
+            const int q = m->nbr[b];
+            if (fabs(m->dmg_broken[b]) > O_EPS)
+                m->dmgD[2 * b + 0] = 0.5 * (m->dnl[2 * i + 0] + m->dnl[2 * q + 0]);
+            m->dmg_w[b] = 1.0 - m->dmgD[2 * b + 0];
+        }
+    return k;
+}
+
+/* updateDuctileDamagePwiseLocal (reference src/constitutive.c:1698-1755): local
+ * damage from the plastic multiplier; a particle passing the threshold (and not yet
+ * at 1) is set to 1 and all its bonds and their reverse entries break; then the
+ * bond damage is the larger of its two ends.  Each particle's decision reads only
+ * its own state: `order` 1 (reverse) gives the state of 0 (the reference's order).
+ * Returns the particles that failed. */
+int orc_damage_pwise_local(orc_model *m, int order)
+{
+    const int n = m->n, nbm = m->nbmax;
+    int k = 0;
+    for (int t = 0; t < n; t++)
+    {
+        const int i = order ? n - 1 - t : t;
+        double damage_f = (1.0 + m->damagec_A * m->triax[i]);
+        if (damage_f > 0.0 && m->dloc[2 * i + 0] <= m->damage_threshold)
+            m->dloc[2 * i + 0] += damage_f * m->dlambda[i];
+        if (m->dloc[2 * i + 0] > m->damage_threshold && fabs(m->dloc[2 * i + 0] - 1.0) > O_EPS)
+        {
+            m->dloc[2 * i + 0] = 1.0;
+            for (int j = 0; j < m->nb_initial[i]; j++)
+            {
+                const int64_t b = (int64_t)i * nbm + j;
+                const int q = m->nbr[b];
+                m->dmg_broken[b] = 0.0;
+                for (int jj = 0; jj < nbm; jj++)
+                    if (m->nbr[(int64_t)q * nbm + jj] == i)
+                        m->dmg_broken[(int64_t)q * nbm + jj] = 0.0;
+            }
+            k++;
+        }
+    }
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < m->nb_initial[i]; j++)
+        {
+            const int64_t b = (int64_t)i * nbm + j;
+            const int q = m->nbr[b];
+            m->dmgD[2 * b + 0] = m->dloc[2 * i + 0] < m->dloc[2 * q + 0] ? m->dloc[2 * q + 0] : m->dloc[2 * i + 0];
+            m->dmg_w[b] = 1.0 - m->dmgD[2 * b + 0];
+        }
+    return k;
+}
+
 /* updateDuctileDamageBwiseLocal (reference src/constitutive.c:1607-1695): local
  * damage accumulated from the plastic multiplier, bond damage as the mean of the two
  * ends, a bond (and its reverse entry) broken when that mean passes the threshold,

@@ -125,6 +125,10 @@ def lib():
         L.orc_spmv_upper.argtypes = [P, _dp, _dp]
         L.orc_solve_cg.argtypes = [P, C.POINTER(OrcSolver), _dp]
         L.orc_solve_cg.restype = C.c_int
+        L.orc_damage_bwise_nonlocal.argtypes = [P]
+        L.orc_damage_bwise_nonlocal.restype = C.c_int
+        L.orc_damage_pwise_local.argtypes = [P, C.c_int]
+        L.orc_damage_pwise_local.restype = C.c_int
         L.orc_damage_bwise_local.argtypes = [P, C.c_int]
         L.orc_damage_bwise_local.restype = C.c_int
         L.orc_damage_brittle.argtypes = [P, _ip, _ip, C.c_int]

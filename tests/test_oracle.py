@@ -379,6 +379,63 @@ def test_bwise_local_damage_brute_force(oracle):
     np.testing.assert_allclose(m.dmg_w[live], 1.0 - D[live], rtol=0, atol=1e-15)
 
 
+def test_pwise_local_damage_order_and_brute_force(oracle):
+    """Point-wise local damage: a particle whose damage passes the threshold fails, all
+    its bonds break both ways, and a bond's damage is the larger of its ends; the
+    reverse particle order gives the same state."""
+    a, b = bwise_model(oracle, 53), bwise_model(oracle, 53)
+    thr, A = 0.9, 0.5
+    d0, dl, tx = a.dloc[:, 0].copy(), a.dlambda.copy(), a.triax.copy()
+    live = a.nbr >= 0
+    broken0 = (a.dmg_broken == 0) & live
+    ka = oracle.lib().orc_damage_pwise_local(a.p, 0)
+    kb = oracle.lib().orc_damage_pwise_local(b.p, 1)
+    assert ka == kb
+    for k in ("dloc", "dmgD", "dmg_broken", "dmg_w"):
+        np.testing.assert_array_equal(getattr(a, k), getattr(b, k), err_msg=k)
+    f = 1.0 + A * tx
+    d1 = np.where((f > 0) & (d0 <= thr), d0 + f * dl, d0)
+    fail = (d1 > thr) & (np.abs(d1 - 1.0) > 1e-6)
+    assert ka == int(fail.sum()) and ka > 10
+    d1 = np.where(fail, 1.0, d1)
+    np.testing.assert_allclose(a.dloc[:, 0], d1, rtol=0, atol=1e-15)
+    q = np.where(live, a.nbr, 0)
+    broken = broken0 | (live & (fail[:, None] | fail[q]))
+    np.testing.assert_array_equal(a.dmg_broken == 0, broken)
+    D = np.maximum(d1[:, None], d1[q])
+    np.testing.assert_array_equal(a.dmg_w[live], 1.0 - D[live])
+
+
+def test_bwise_nonlocal_damage_brute_force(oracle):
+    """Bond-wise nonlocal damage: a particle below the threshold gains the Gaussian-
+    weighted mean of the local rates of itself and its list, and every directed bond
+    whose ends average above the threshold breaks."""
+    m = bwise_model(oracle, 54)
+    thr, A, Ld = 0.9, 0.5, m.s.damage_L
+    rng = np.random.default_rng(55)
+    m.dnl[:, 0] = rng.uniform(0.6, 1.05, m.n)
+    live = m.nbr >= 0
+    broken0 = (m.dmg_broken == 0) & live
+    m.dmgD[:, :, 0] = np.where(broken0, 1.0, 0.0)
+    d0, dl, tx = m.dnl[:, 0].copy(), m.dlambda.copy(), m.triax.copy()
+    k = oracle.lib().orc_damage_bwise_nonlocal(m.p)
+    f = 1.0 + A * tx
+    r = np.where(f > 0, dl * f, 0.0)
+    q = np.where(live, m.nbr, 0)
+    phi = np.where(live, 1.0 / Ld / np.sqrt(2 * np.pi) * np.exp(-0.5 * m.L0 ** 2 / Ld ** 2), 0.0)
+    V = m.s.volume
+    Ddot = V * r + (r[q] * phi * V).sum(axis=1)
+    Aw = V + (phi * V).sum(axis=1)
+    d1 = np.where(d0 > thr, np.minimum(d0, 1.0), np.where(Ddot > 0, d0 + Ddot / Aw, d0))
+    np.testing.assert_allclose(m.dnl[:, 0], d1, rtol=1e-14, atol=0)
+    mean = 0.5 * (d1[:, None] + d1[q])
+    newly = live & ~broken0 & (mean > thr)
+    assert k == int(newly.sum()) and k > 10
+    np.testing.assert_array_equal(m.dmg_broken == 0, broken0 | newly)
+    D = np.where(broken0 | newly, 1.0, mean)
+    np.testing.assert_allclose(m.dmg_w[live], 1.0 - D[live], rtol=0, atol=1e-14)
+
+
 def test_j2energy_elastic_branch_is_the_elastic_law(oracle):
     """Below yield the energy-based J2 law has no plastic stretch, the half-averaged
     stretch of a bond equals its own, and its forces are the elastic law's."""
