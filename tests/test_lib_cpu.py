@@ -113,3 +113,34 @@ def test_reference_drivers_compile_and_link_unchanged(tmp_path, driver):
                         "-o", str(out)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     assert out.exists()
+
+
+def test_compute_strain_recovers_an_affine_strain(oracle):
+    """computeStrain (host, output only) fits the strain to the bond stretches by
+    weighted least squares.  Under a small affine strain the stretch of a bond is
+    e.eps.e to first order, so every particle's fit returns eps, with engineering
+    shears, in the reference's order s11 s22 s33 s23 s13 s12."""
+    L = P.sc_box_for(6)
+    n = P.make_lattice(2, 0.5, [0, L, 0, L, 0, L])
+    x = P.positions()
+    m = oracle.Model(x, 2, 0.5, np.zeros(n, np.int32), [oracle.elastic_constants(200e3, 0.3)])
+    m.setup()
+    nb = P.gint("nneighbors").value
+    rows = {name: P.gptr(name, 2, ct) for name, ct in (("neighbors", C.c_int), ("nsign", C.c_int),
+                                                         ("distance_initial", C.c_double), ("dL", C.c_double))}
+    nbi = P.gptr("nb_initial", 1, C.c_int)
+    eps = 1e-6 * np.array([[1.0, 0.3, -0.2], [0.3, -0.5, 0.4], [-0.2, 0.4, 0.8]])
+    xd = x + x @ eps.T
+    for i in range(n):
+        nbi[i] = int(m.nb_initial[i])
+        for j in range(nb):
+            rows["neighbors"][i][j] = int(m.nbr[i, j])
+            rows["nsign"][i][j] = int(m.nsign[i, j])
+            if m.nbr[i, j] >= 0:
+                q = m.nbr[i, j]
+                rows["distance_initial"][i][j] = m.L0[i, j]
+                rows["dL"][i][j] = np.linalg.norm(xd[q] - xd[i]) - m.L0[i, j]
+    P.lib().computeStrain()
+    st = P.read2d("strain_tensor", n, 6)
+    want = np.array([eps[0, 0], eps[1, 1], eps[2, 2], 2 * eps[1, 2], 2 * eps[0, 2], 2 * eps[0, 1]])
+    np.testing.assert_allclose(st, np.broadcast_to(want, st.shape), rtol=0, atol=1e-4 * np.abs(want).max())
