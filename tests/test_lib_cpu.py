@@ -144,3 +144,29 @@ def test_compute_strain_recovers_an_affine_strain(oracle):
     st = P.read2d("strain_tensor", n, 6)
     want = np.array([eps[0, 0], eps[1, 1], eps[2, 2], 2 * eps[1, 2], 2 * eps[0, 2], 2 * eps[0, 1]])
     np.testing.assert_allclose(st, np.broadcast_to(want, st.shape), rtol=0, atol=1e-4 * np.abs(want).max())
+
+
+def test_write_disp_reports_the_mean_displacement(tmp_path):
+    """writeDisp (host, src/data_handler.c:278-306) appends "step mean-displacement" of
+    one particle type along an axis: under a uniform translation every type reports
+    the translation, a line per call."""
+    L = P.sc_box_for(5)
+    n = P.make_lattice(2, 0.5, [0, L, 0, L, 0, L])
+    x = P.positions()
+    t = P.sc_types(x)
+    P.set_types(t, 5)
+    u = np.array([0.125, -0.5, 0.25])
+    xyz = P.gptr("xyz", 2)
+    for i in range(n):
+        for c in range(3):
+            xyz[i][c] = x[i, c] + u[c]
+    f = str(tmp_path / "disp.txt")
+    fn = P.lib().writeDisp
+    fn.argtypes = [C.c_char_p, C.c_char, C.c_int, C.c_int]
+    fn.restype = None
+    for step, (axis, typ) in enumerate([(b"z", 1), (b"x", 2), (b"y", 0)], start=3):
+        fn(f.encode(), axis, typ, step)
+    lines = open(f).read().split("\n")
+    assert lines[-1] == ""
+    got = [(int(a), float(b)) for a, b in (ln.split() for ln in lines[:-1])]
+    assert got == [(3, 0.25), (4, 0.125), (5, -0.5)]
