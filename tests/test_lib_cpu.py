@@ -170,3 +170,32 @@ def test_write_disp_reports_the_mean_displacement(tmp_path):
     assert lines[-1] == ""
     got = [(int(a), float(b)) for a, b in (ln.split() for ln in lines[:-1])]
     assert got == [(3, 0.25), (4, 0.125), (5, -0.5)]
+
+
+def test_write_dump_lammps_layout(tmp_path):
+    """writeDump (host, src/data_handler.c:42-84) writes a LAMMPS dump: the timestep,
+    the particle count, the box, the column header, then one row per particle with
+    its id, type, position and displacement."""
+    L = P.sc_box_for(4)
+    n = P.make_lattice(2, 0.5, [0, L, 0, L, 0, L])
+    x = P.positions()
+    P.set_types(P.sc_types(x), 5)
+    u = np.array([0.0, 0.0, 0.03125])
+    xyz = P.gptr("xyz", 2)
+    for i in range(n):
+        xyz[i][2] = x[i, 2] + u[2]
+    f = str(tmp_path / "dump.txt")
+    fn = P.lib().writeDump
+    fn.argtypes = [C.c_char_p, C.c_int, C.c_char]
+    fn.restype = None
+    fn(f.encode(), 7, b"s")
+    lines = open(f).read().splitlines()
+    assert lines[:4] == ["ITEM: TIMESTEP", "7", "ITEM: NUMBER OF ATOMS", str(n)]
+    assert lines[4] == "ITEM: BOX BOUNDS pp pp pp"
+    assert lines[8].startswith("ITEM: ATOMS id type x y z dx dy dz")
+    rows = [ln.split() for ln in lines[9:]]
+    assert len(rows) == n
+    t = P.sc_types(x)
+    for i, r in enumerate(rows):
+        assert int(r[0]) == i and int(r[1]) == t[i]
+        np.testing.assert_allclose([float(v) for v in r[2:8]], list(x[i] + u) + list(u), rtol=1e-4, atol=1e-4)
