@@ -18,8 +18,17 @@ void writeDump(const char *dataName, int step, char flag)
     fprintf(f, "ITEM: TIMESTEP\n%d\nITEM: NUMBER OF ATOMS\n%d\nITEM: BOX BOUNDS pp pp pp\n", step, nparticle);
     for (int c = 0; c < 3; c++)
         fprintf(f, "%8.8f %8.8f\n", box[2 * c], box[2 * c + 1]);
-    fprintf(f, "ITEM: ATOMS id type x y z dx dy dz s11 s22 s33 s23 s13 s12 stress_eq strain_eq stress_m stress_tri "
-               "damage_local damage_nonlocal pl damage\n");
+    /* crystal plasticity (plmode 1) reports the accumulated slips of eight of the 24
+     * systems (0-based 1, 4, 10, 19, 9, 14, 17, 22) and the total slip; the other
+     * laws the J2 and damage columns (data_handler.c:59-83) */
+    static const int slip_cols[8] = {1, 4, 10, 19, 9, 14, 17, 22};
+    const int cp = plmode == 1;
+    if (cp)
+        fprintf(f, "ITEM: ATOMS id type x y z dx dy dz s11 s22 s33 s23 s13 s12 gam11 gam12 gam13 gam14 gam21 gam22 "
+                   "gam23 gam24 pl gam_total\n");
+    else
+        fprintf(f, "ITEM: ATOMS id type x y z dx dy dz s11 s22 s33 s23 s13 s12 stress_eq strain_eq stress_m "
+                   "stress_tri damage_local damage_nonlocal pl damage\n");
     for (int i = 0; i < nparticle; i++)
     {
         fprintf(f, "%d %d", i, type[i]);
@@ -29,8 +38,15 @@ void writeDump(const char *dataName, int step, char flag)
             fprintf(f, " %.4e", xyz[i][c] - xyz_initial[i][c]);
         for (int c = 0; c < 6; c++)
             fprintf(f, " %.4e", stress_tensor[i][c]);
-        fprintf(f, " %.4e %.4e %.4e %.4e %.4e %.4e %d %.4e\n", J2_stresseq[i], J2_alpha[i][0], J2_stressm[i],
-                J2_triaxiality[i], damage_local[i][0], damage_nonlocal[i][0], pl_flag[i], damage_visual[i]);
+        if (cp)
+        {
+            for (int c = 0; c < 8; c++)
+                fprintf(f, " %.4e", cp_A_single[i][slip_cols[c]][0]);
+            fprintf(f, " %d %.4e\n", pl_flag[i], cp_A[i][0]);
+        }
+        else
+            fprintf(f, " %.4e %.4e %.4e %.4e %.4e %.4e %d %.4e\n", J2_stresseq[i], J2_alpha[i][0], J2_stressm[i],
+                    J2_triaxiality[i], damage_local[i][0], damage_nonlocal[i][0], pl_flag[i], damage_visual[i]);
     }
     fclose(f);
 }
@@ -194,9 +210,10 @@ void writeReaction(const char *dataName, char c, int t1, int tstep)
     FILE *f = fopen(dataName, "a+");
     if (!f)
         return;
-    double **cs = c == 'x' ? csx : (c == 'y' ? csy : csz);
+    /* an axis character other than x, y, z adds nothing (data_handler.c:320-327) */
+    double **cs = c == 'x' ? csx : (c == 'y' ? csy : (c == 'z' ? csz : NULL));
     double force = 0.;
-    for (int i = 0; i < nparticle; i++)
+    for (int i = 0; cs && i < nparticle; i++)
         if (type[i] == t1)
             for (int j = 0; j < nb_initial[i]; j++)
                 force += -cs[i][j] * F[i][j];
@@ -211,11 +228,12 @@ void writeForce(const char *dataname, char c, double p, int tstep)
     FILE *f = fopen(dataname, "a+");
     if (!f)
         return;
-    int k = c == 'x' ? 0 : (c == 'y' ? 1 : 2);
-    double lim = box[2 * k] + p * (k == 0 ? box_x : (k == 1 ? box_y : box_z));
+    /* an axis character other than x, y, z selects no particle (data_handler.c:346-373) */
+    int k = c == 'x' ? 0 : (c == 'y' ? 1 : (c == 'z' ? 2 : -1));
+    double lim = k < 0 ? 0.0 : box[2 * k] + p * (k == 0 ? box_x : (k == 1 ? box_y : box_z));
     double **cs = k == 0 ? csx : (k == 1 ? csy : csz);
     double force = 0.;
-    for (int i = 0; i < nparticle; i++)
+    for (int i = 0; k >= 0 && i < nparticle; i++)
         if (xyz_initial[i][k] < lim)
             for (int j = 0; j < nb_initial[i]; j++)
                 force += -cs[i][j] * F[i][j];

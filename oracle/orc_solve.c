@@ -581,8 +581,11 @@ void orc_update_crack(orc_model *m)
 /* writeReaction / writeDisp values (reference src/data_handler.c:278-335). */
 double orc_reaction(const orc_model *m, char axis, int t1)
 {
-    int c = axis == 'x' ? 0 : axis == 'y' ? 1 : 2;
+    /* an axis character other than x, y, z adds nothing (data_handler.c:320-327) */
+    int c = axis == 'x' ? 0 : axis == 'y' ? 1 : axis == 'z' ? 2 : -1;
     double force = 0.;
+    if (c < 0)
+        return force;
     for (int i = 0; i < m->n; i++)
         if (m->type[i] == t1)
             for (int j = 0; j < m->nb_initial[i]; j++)

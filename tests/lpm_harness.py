@@ -41,3 +41,22 @@ def push(name, arr):
 
 def get(name, shape):
     return P.field(name, int(np.prod(shape))).reshape(shape)
+
+
+# FP64 displacements and reactions must agree with the oracle to 1e-10 relative;
+# the mixed form states 1e-8.
+TOL_FP64 = 1e-10
+TOL_MIXED = 1e-8
+
+
+def tol_for(prec):
+    return TOL_FP64 if prec == P.LPMX_FP64 else TOL_MIXED
+
+
+def assert_rel(label, a, ref, tol):
+    """max|a - ref| <= tol * max|ref|, with the measured ratio printed first."""
+    a, ref = np.asarray(a, np.float64), np.asarray(ref, np.float64)
+    scale = float(np.abs(ref).max())
+    err = float(np.abs(a - ref).max()) / scale if scale > 0 else float(np.abs(a - ref).max())
+    print(f"{label}: relative error {err:.3e} (bound {tol:.0e})", flush=True)
+    assert err <= tol, (label, err, tol)

@@ -10,7 +10,9 @@ import os
 import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
+# LPM_ORACLE_SO names another build of the oracle (a copy with a planted mistake,
+# to check that the pins catch it)
+ORACLE_SO = os.environ.get("LPM_ORACLE_SO") or os.path.join(ROOT, "oracle", "liboracle.so")
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
