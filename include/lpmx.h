@@ -48,10 +48,12 @@ typedef struct lpmx_model_desc
 
 typedef struct lpmx_params
 {
-    int plmode;               /* 0 J2 mixed-linear, 6 elastic-brittle */
+    int plmode;               /* 0 J2 mixed-linear, 1 crystal plasticity, 3 energy-based J2,
+                               * 5 nonlinear-isotropic J2, 6 elastic-brittle */
     double J2_H, J2_xi;
     double damage_L, damage_threshold, damagec_A, critical_bstrain;
     int nbreak;
+    double J2_C;              /* kinematic hardening modulus of plmode 5 */
 } lpmx_params;
 
 /* Create the device model from positions (xyz = xyz_initial = xyz_in, [n][3]). */
@@ -97,6 +99,8 @@ enum lpmx_field
     LPMX_F_CP_DGY, LPMX_F_CP_DA_SINGLE, LPMX_F_CP_RSS,                   /* double [n][nslip] */
     LPMX_F_CP_JACT,                                                      /* int [n][nslip] */
     LPMX_F_CP_CAB,                                                       /* double [n][nslip*nslip] */
+    LPMX_F_J2_BETA_EQ0, LPMX_F_J2_BETA_EQ1, LPMX_F_J2_BETA_EQ2,          /* double [n] */
+    LPMX_F_DAMAGE_LOCAL0, LPMX_F_DAMAGE_LOCAL1,                          /* double [n] */
     LPMX_F_COUNT
 };
 
@@ -165,11 +169,18 @@ int lpmx_vjp(const double *Pbar, const double *dLp2bar, const double *beta2bar, 
 
 /* ---- constitutive, residual, boundary conditions, damage ------------------------------ */
 int lpmx_computedL(void);
-/* computeBondForceGeneral(mode): 0 J2, 1 crystal plasticity, 4 predictor, 6 elastic; then stress and
- * switchStateV(2).  The _range form runs the per-particle law for [first, first+count)
- * only (no stress, no state switch). */
+/* computeBondForceGeneral(mode, t) (reference src/constitutive.c:88-146): 0 J2 mixed-linear,
+ * 1 crystal plasticity, 3 energy-based J2 with load indicator t (+1 tension, -1
+ * compression), 4 predictor, 5 nonlinear-isotropic J2, 6 elastic; then stress and
+ * switchStateV(2).  lpmx_bond_force(mode) is t = 1.  The _range form runs the
+ * per-particle law for [first, first+count) only (no stress, no state switch);
+ * modes 3 and 5 take the whole range only.  Mode 5 advances each particle's slot-0
+ * state once per evaluation, where the reference advances it once per list it is
+ * in, in its particle order. */
 int lpmx_bond_force(int mode);
+int lpmx_bond_force_ex(int mode, int t);
 int lpmx_bond_force_range(int mode, int first, int count);
+int lpmx_bond_force_range_ex(int mode, int t, int first, int count);
 int lpmx_compute_stress(void);
 int lpmx_switch_state(int flag);
 /* updateRR: residual and reaction on the device; norms returned (NULL allowed). */
@@ -181,10 +192,25 @@ typedef struct lpmx_forcebc { int type; double step[3]; } lpmx_forcebc;
 int lpmx_set_disp_bc(int nbd, const lpmx_dispbc *bc);
 int lpmx_set_force_bc(int nbf, const lpmx_forcebc *bc);
 
-/* updateDamageGeneral for plmode 0 (nonlocal particle-wise ductile) and 6 (brittle);
+/* updateDamageGeneral for plmode 0 (nonlocal particle-wise ductile), 5 (local bond-wise
+ * ductile) and 6 (brittle); plmode 3 breaks nothing;
  * *broken gets the reference's return value.  Broken pairs of the brittle law can be
  * read back with lpmx_get_broken (up to cap pairs). */
 int lpmx_damage(int plmode, int *broken);
+/* One damage update of the reference by name: updateDuctileDamagePwiseNonlocal (0),
+ * updateDuctileDamagePwiseLocal (1), updateDuctileDamageBwiseNonlocal (2),
+ * updateDuctileDamageBwiseLocal (3), updateBrittleDamage (6) (reference
+ * src/constitutive.c:1437-1862).  *count gets the reference's return value: directed
+ * entries broken (0, 2), particles failed (1), pairs broken (3), candidates (6). */
+enum
+{
+    LPMX_DMG_PWISE_NONLOCAL = 0,
+    LPMX_DMG_PWISE_LOCAL = 1,
+    LPMX_DMG_BWISE_NONLOCAL = 2,
+    LPMX_DMG_BWISE_LOCAL = 3,
+    LPMX_DMG_BRITTLE = 6
+};
+int lpmx_damage_law(int law, int *count);
 int lpmx_get_broken(int *bi, int *bj, int cap, int *count);
 int lpmx_update_crack(void);
 /* writeReaction / writeDisp values for particles of type t1 along axis 'x','y','z'. */

@@ -25,7 +25,7 @@ DISTANCE DAMAGE_BROKEN DAMAGE_W DLP0 DLP1 DLP2 DAMAGE_D0 DAMAGE_D1 DL_TOTAL TDL_
 J2_BETA0 J2_BETA1 J2_BETA2 STRESS J2_ALPHA0 J2_ALPHA1 J2_ALPHA2 J2_DLAMBDA J2_STRESSEQ J2_STRESSM
 J2_TRIAXIALITY DAMAGE_NONLOCAL0 DAMAGE_NONLOCAL1 DAMAGE_VISUAL SIGMAY PIN PEX PEX_TEMP RESIDUAL DISP
 DISPBC_INDEX FIX_INDEX CP_GY0 CP_GY1 CP_GY2 CP_A_SINGLE0 CP_A_SINGLE1 CP_A_SINGLE2 CP_A0 CP_A1 CP_A2 CP_DA
-CP_DGY CP_DA_SINGLE CP_RSS CP_JACT CP_CAB""".split()
+CP_DGY CP_DA_SINGLE CP_RSS CP_JACT CP_CAB J2_BETA_EQ0 J2_BETA_EQ1 J2_BETA_EQ2 DAMAGE_LOCAL0 DAMAGE_LOCAL1""".split()
 F = {name: k for k, name in enumerate(FIELDS)}
 INT_FIELDS = {"NEIGHBORS", "NSIGN", "NB", "NB_INITIAL", "NB_CONN", "TYPE", "PL_FLAG", "CONN", "K_POINTER",
               "DISPBC_INDEX", "FIX_INDEX", "CP_JACT"}
@@ -45,7 +45,7 @@ class ModelDesc(C.Structure):
 class Params(C.Structure):
     _fields_ = [("plmode", C.c_int), ("J2_H", C.c_double), ("J2_xi", C.c_double), ("damage_L", C.c_double),
                 ("damage_threshold", C.c_double), ("damagec_A", C.c_double), ("critical_bstrain", C.c_double),
-                ("nbreak", C.c_int)]
+                ("nbreak", C.c_int), ("J2_C", C.c_double)]
 
 
 class DispBC(C.Structure):
@@ -472,6 +472,21 @@ def sc_uniaxial_steps(height, strains):
         steps.append(([(2, "z", 0.0), (3, "x", 0.0), (3, "y", 0.0), (3, "z", 0.0), (4, "y", 0.0), (4, "z", 0.0),
                        (1, "z", du)], []))
     return steps
+
+
+def cyclic_strains(amplitude, steps_per_half, cycles):
+    """Total strain after each load step of a symmetric triangular strain cycle
+    repeated `cycles` times: 0 -> +a -> -a -> 0, each half-cycle (a peak to the
+    opposite peak) in `steps_per_half` equal increments 2a/steps_per_half.  Strains
+    are integer multiples of the increment, so the path returns to 0 exactly."""
+    q = steps_per_half // 2
+    de = 2.0 * amplitude / steps_per_half
+    count, out = 0, []
+    for _ in range(cycles):
+        for k in range(2 * steps_per_half):
+            count += 1 if (k < q or k >= q + steps_per_half) else -1
+            out.append(count * de)
+    return np.array(out)
 
 
 def uniform_modulus(n, seed, amp=0.05):

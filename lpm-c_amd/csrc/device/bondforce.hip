@@ -16,51 +16,7 @@
 
 #include "common.h"
 
-#define BI(j, i) ((int64_t)(j) * v.n + (i))
-
-__device__ __forceinline__ double dist_xyz(const double *x, int i, int j)
-{
-    double dx = x[3 * i] - x[3 * j], dy = x[3 * i + 1] - x[3 * j + 1], dz = x[3 * i + 2] - x[3 * j + 2];
-    return sqrt(dx * dx + dy * dy + dz * dz);
-}
-
-// Opposite-bond factor of the stress sums (constitutive.c:540-559,
-// lpm_basic.c:71-90); the first opposite slot is precomputed in opp.
-__device__ __forceinline__ double opp_flag(const DV &v, int i, int j)
-{
-    if (v.nbcur[i] == v.nb)
-        return 0.5;
-    const int o = v.opp[BI(j, i)];
-    if (o < 0)
-        return 1.0;
-    return v.db[BI(o, i)] <= LPMX_EPS ? 1.0 : 0.5;
-}
-
-// One stretch pass of particle i (constitutive.c:495-515): dL, cs and the shell
-// sums, with plastic stretch from `xp` (slot plane) or, if plastic == nullptr, none.
-__device__ __forceinline__ void stretch_pass(const DV &v, int i, const double *xp, double &a0, double &a1, double &b0,
-                                             double &b1)
-{
-    a0 = 0, b0 = 0;
-    a1 = 0, b1 = 0;
-    for (int j = 0; j < v.nb0[i]; j++)
-    {
-        const int64_t b = BI(j, i);
-        const int q = v.nbr[b];
-        const double dis = dist_xyz(v.x, i, q);
-        double dl = dis - v.L0[b];
-        dl -= xp[b];
-        dl *= v.db[b];
-        v.dL[b] = dl;
-        if (v.nsign[b] == 0)
-            a0 += dl, b0 += v.Tv[b] * dl;
-        else
-            a1 += dl, b1 += v.Tv[b] * dl;
-        v.csx[b] = (v.x[3 * i] - v.x[3 * q]) / dis;
-        v.csy[b] = (v.x[3 * i + 1] - v.x[3 * q + 1]) / dis;
-        v.csz[b] = (v.x[3 * i + 2] - v.x[3 * q + 2]) / dis;
-    }
-}
+#include "bond_common.h"
 
 // ------------------------------------------------------------ elastic law -------
 __global__ void k_elastic_pass(DV v, int first, int count)
@@ -655,7 +611,7 @@ extern "C" int lpmx_j2_particle_bytes(double *bytes)
     return LPMX_OK;
 }
 
-extern "C" int lpmx_bond_force_range(int mode, int first, int count)
+static int bond_force_range(int mode, int t, int first, int count)
 {
     RETCHK(lpmx_require_model());
     if (first < 0 || count < 0 || first + count > g.n)
@@ -663,6 +619,13 @@ extern "C" int lpmx_bond_force_range(int mode, int first, int count)
     if (count == 0)
         return LPMX_OK;
     const dim3 G(nblocks(count)), B(LPMX_BLOCK);
+    if (mode == 3 || mode == 5)
+    {
+        // plmode 5 advances the slot-0 state of every particle in place: no subrange
+        if (first != 0 || count != g.n)
+            return lpmx_fail(LPMX_ERR_ARG, "constitutive mode %d evaluates every particle", mode);
+        return mode == 3 ? lpmx_law3(t) : lpmx_law5();
+    }
     if (mode == 0)
     {
         // the final force of a particle needs its partners' return maps too
@@ -698,9 +661,17 @@ extern "C" int lpmx_bond_force_range(int mode, int first, int count)
         hipLaunchKernelGGL(k_elastic_force, G, B, 0, g_stream, g, first, count);
     }
     else
-        return lpmx_fail(LPMX_ERR_ARG, "constitutive mode %d is not on this path (0, 1, 4 and 6 are)", mode);
+        return lpmx_fail(LPMX_ERR_ARG, "constitutive mode %d is not a law of the reference (0, 1, 3, 4, 5, 6 are)",
+                         mode);
     LAUNCHCHK();
     return LPMX_OK;
+}
+
+extern "C" int lpmx_bond_force_range(int mode, int first, int count) { return bond_force_range(mode, 1, first, count); }
+
+extern "C" int lpmx_bond_force_range_ex(int mode, int t, int first, int count)
+{
+    return bond_force_range(mode, t, first, count);
 }
 
 int lpmx_eval_j2(void)
@@ -713,11 +684,16 @@ int lpmx_eval_j2(void)
     return LPMX_OK;
 }
 
-extern "C" int lpmx_bond_force(int mode)
+extern "C" int lpmx_bond_force_ex(int mode, int t)
 {
-    RETCHK(lpmx_bond_force_range(mode, 0, g.n));
+    RETCHK(bond_force_range(mode, t, 0, g.n));
     RETCHK(lpmx_compute_stress());
     return lpmx_switch_state(2);
+}
+
+extern "C" int lpmx_bond_force(int mode)
+{
+    return lpmx_bond_force_ex(mode, 1);
 }
 
 // ------------------------------------------------------------ residual ----------
@@ -1349,11 +1325,23 @@ extern "C" int lpmx_damage(int plmode, int *broken)
         HIPCHK(hipMemcpyAsync(&b, d_cnt, sizeof(int), hipMemcpyDeviceToHost, g_stream));
         HIPCHK(hipStreamSynchronize(g_stream));
     }
+    else if (plmode == 5)
+        RETCHK(lpmx_damage_variant(LPMX_DMG_BWISE_LOCAL, &b));
     else if (plmode == 6)
         RETCHK(brittle(&b));
     if (broken)
         *broken = b;
     return LPMX_OK;
+}
+
+extern "C" int lpmx_damage_law(int law, int *count)
+{
+    RETCHK(lpmx_require_model());
+    if (law == LPMX_DMG_PWISE_NONLOCAL)
+        return lpmx_damage(0, count);
+    if (law == LPMX_DMG_BRITTLE)
+        return lpmx_damage(6, count);
+    return lpmx_damage_variant(law, count);
 }
 
 // ------------------------------------------------------------------- crack -------

@@ -26,7 +26,7 @@ struct DV
     double radius, cut1, cut2, volume;
     // parameters
     int plmode, nbreak;
-    double H, xi, dmgL, dthr, dA, crit0;
+    double H, xi, dmgL, dthr, dA, crit0, J2C; // J2C: the kinematic modulus of plmode 5
     // positions
     double *x, *x0, *xt;
     // per particle
@@ -48,8 +48,8 @@ struct DV
     int nslices;
     int64_t nslots, nnzb;
     int *sptr, *col, *tpos, *dslot, *conn;      // conn: [cm][n] planes
-    int64_t *rowptr;                            // row-major block offsets of J
-    double *K, *J, *Minv;
+    int64_t *rowptr;                            // row-major block offsets (conn order)
+    double *K, *Minv;
     float *K32, *Minv32;
     // static nonlocal-damage neighbourhoods (index order) and weights
     int *dptr, *didx;
@@ -93,6 +93,12 @@ int lpmx_hip_fail(hipError_t e, const char *what, const char *file, int line);
     } while (0)
 
 int lpmx_require_model(void);
+// laws.hip: plmode 3 (load indicator t) and plmode 5 force evaluations of every
+// particle, and the damage variants (codes of include/lpmx.h)
+int lpmx_law3(int t);
+int lpmx_law5(void);
+int lpmx_damage_variant(int law, int *count);
+void lpmx_laws_free(void);
 int lpmx_require_pattern(void);
 static inline int nblocks(int64_t n, int bs = LPMX_BLOCK) { return (int)((n + bs - 1) / bs); }
 
@@ -167,6 +173,8 @@ __device__ __forceinline__ double ordered_total(const double *parts, int np, dou
 // ------------------------------------------------------ host-side reductions -----
 // Deterministic device sums of up to 4 per-element values produced by a functor;
 // used for norms and small outputs outside the CG inner loop.
+#define LPMX_RED_PARTS 1024       // partials of the deterministic device sums
+int lpmx_dev_sum_into(const double *src, int64_t count, int sq, double *parts, double *out_host);
 int lpmx_dev_sum(const double *src, int64_t count, double *out_host);
 int lpmx_dev_sumsq(const double *src, int64_t count, double *out_host);
 int lpmx_dev_scan_int(const int *in, int *out_excl, int count, int *total_host);

@@ -105,6 +105,10 @@ int lpmx_run_steps_impl(const lpmx_solver *s, int nsteps, const int *nbd, const 
                 o.ir_passes += lpmx_last_ir_passes();
                 if (stt != 1)
                     o.status |= stt == 3 ? 4 : 2;
+                // plmode 5 advances slot 0 in place: every evaluation starts from the
+                // converged state, as the reference's switchStateV(0) at each iteration gives
+                if (g.plmode == 5)
+                    RETCHK(lpmx_switch_state(0));
                 RETCHK(lpmx_bond_force_range(g.plmode, 0, n));
                 RETCHK(lpmx_update_rr(&nres, nullptr, nullptr));
                 if (mixed)
@@ -114,6 +118,8 @@ int lpmx_run_steps_impl(const lpmx_solver *s, int nsteps, const int *nbd, const 
                     {
                         o.accel_reverts++;
                         RETCHK(lpmx_accel_revert());
+                        if (g.plmode == 5)
+                            RETCHK(lpmx_switch_state(0));
                         RETCHK(lpmx_bond_force_range(g.plmode, 0, n));
                         RETCHK(lpmx_update_rr(&nres, nullptr, nullptr));
                     }

@@ -140,13 +140,14 @@ static void free_all(void)
     for (void *p : g_allocs)
         hipFree(p);
     g_allocs.clear();
-    void *pat[] = {g.sptr, g.col, g.tpos, g.K, g.K32, g.J, g.Minv, g.Minv32, g.crit, g.ktype_t, g.ce_t, g.emod};
+    void *pat[] = {g.sptr, g.col, g.tpos, g.K, g.K32, g.Minv, g.Minv32, g.crit, g.ktype_t, g.ce_t, g.emod};
     for (void *p : pat)
         if (p)
             hipFree(p);
     lpmx_tangent_free();
     lpmx_solver_free();
     lpmx_damage_free();
+    lpmx_laws_free();
     lpmx_accel_free();
     lpmx_amg_free();
     lpmx_cp_free();
@@ -243,6 +244,7 @@ extern "C" int lpmx_set_params(const lpmx_params *p)
     g.plmode = p->plmode, g.H = p->J2_H, g.xi = p->J2_xi;
     g.dmgL = p->damage_L, g.dthr = p->damage_threshold, g.dA = p->damagec_A, g.crit0 = p->critical_bstrain;
     g.nbreak = p->nbreak;
+    g.J2C = p->J2_C;
     return LPMX_OK;
 }
 
@@ -341,12 +343,12 @@ __global__ void k_final_sum(const double *parts, int np, double *out)
         *out = s;
 }
 
-static int dev_sum_impl(const double *src, int64_t n, int sq, double *out_host)
+// Sum (sq = 0) or sum of squares (sq = 1) of src[0, n) in a fixed order: LPMX_RED_PARTS
+// grid-stride partials, then one work-group over them.  parts: LPMX_RED_PARTS + 8
+// device doubles of the caller's.
+int lpmx_dev_sum_into(const double *src, int64_t n, int sq, double *parts, double *out_host)
 {
-    const int np = 1024;
-    if (!g_red_buf)
-        HIPCHK(hipMalloc(&g_red_buf, sizeof(double) * (np + 8)));
-    double *parts = g_red_buf;
+    const int np = LPMX_RED_PARTS;
     hipLaunchKernelGGL(k_partial_sum, dim3(np), dim3(LPMX_BLOCK), 0, g_stream, src, n, sq, parts);
     LAUNCHCHK();
     hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(1024), 0, g_stream, parts, np, parts + np);
@@ -354,6 +356,13 @@ static int dev_sum_impl(const double *src, int64_t n, int sq, double *out_host)
     HIPCHK(hipMemcpyAsync(out_host, parts + np, sizeof(double), hipMemcpyDeviceToHost, g_stream));
     HIPCHK(hipStreamSynchronize(g_stream));
     return LPMX_OK;
+}
+
+static int dev_sum_impl(const double *src, int64_t n, int sq, double *out_host)
+{
+    if (!g_red_buf)
+        HIPCHK(hipMalloc(&g_red_buf, sizeof(double) * (LPMX_RED_PARTS + 8)));
+    return lpmx_dev_sum_into(src, n, sq, g_red_buf, out_host);
 }
 
 int lpmx_dev_sum(const double *src, int64_t n, double *out) { return dev_sum_impl(src, n, 0, out); }
@@ -458,6 +467,11 @@ static int field_desc(int f, FieldDesc *fd)
     case LPMX_F_DAMAGE_NONLOCAL0: p = g.dnl0; break;
     case LPMX_F_DAMAGE_NONLOCAL1: p = g.dnl1; break;
     case LPMX_F_DAMAGE_VISUAL: p = g.dvis; break;
+    case LPMX_F_J2_BETA_EQ0: p = g.beq0; break;
+    case LPMX_F_J2_BETA_EQ1: p = g.beq1; break;
+    case LPMX_F_J2_BETA_EQ2: p = g.beq2; break;
+    case LPMX_F_DAMAGE_LOCAL0: p = g.dloc0; break;
+    case LPMX_F_DAMAGE_LOCAL1: p = g.dloc1; break;
     case LPMX_F_SIGMAY: p = g.sigmay; break;
     case LPMX_F_PIN: p = g.Pin, w = 3; break;
     case LPMX_F_PEX: p = g.Pex, w = dim; break;
@@ -1021,7 +1035,7 @@ extern "C" int lpmx_connectivity_build(int64_t *len_K)
     HIPCHK(hipMemsetAsync(g.col, 0, sizeof(int) * nval, g_stream));
     hipLaunchKernelGGL(k_sell_cols, dim3(nblocks((int64_t)n * g.cm)), dim3(LPMX_BLOCK), 0, g_stream, g);
     LAUNCHCHK();
-    // row-major block offsets of the Jacobian rows
+    // row-major block offsets (the first block of each row in conn order) and nnzb
     const int per = 1024;
     const int nbk = (n + 1 + per - 1) / per;
     int64_t *bsum = nullptr;
@@ -1041,9 +1055,6 @@ extern "C" int lpmx_connectivity_build(int64_t *len_K)
     HIPCHK(hipStreamSynchronize(g_stream));
     hipFree(bsum);
     g.nnzb = acc;
-    if (g.J)
-        hipFree(g.J);
-    HIPCHK(hipMalloc(&g.J, sizeof(double) * acc * g.d2));
     // K_pointer[n][1]
     int *start = nullptr, total = 0;
     RETCHK(kptr_start(&start, &total));

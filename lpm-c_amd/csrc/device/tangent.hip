@@ -1,43 +1,62 @@
-// Tangent stiffness on the device: analytic elastic Jacobian rows, the reference's
-// symmetrisation into the sliced-ELL block matrix, displacement-BC elimination,
+// Tangent stiffness on the device: analytic elastic Jacobian rows symmetrised as the
+// reference does, straight into the sliced-ELL block matrix, displacement-BC elimination,
 // block-Jacobi inverse, and the reference's upper-CSR view of the matrix.
 #include <math.h>
 
 #include "common.h"
 
-static double *t_A = nullptr, *t_B = nullptr; // shell sums of the elastic law
-static int *t_kstart = nullptr;               // K_pointer[i][1] (0-based starts), n+1
-static double t_nd = 0.0;                     // diagonal value of the constrained DOFs
-static bool t_nd_valid = false;               // computed since the last tangent assembly
+static double *t_S = nullptr;   // per-particle shell sums of the elastic law, TS_NPLANE planes of n
+static int *t_kstart = nullptr; // K_pointer[i][1] (0-based starts), n+1
+static double t_nd = 0.0;       // diagonal value of the constrained DOFs
+static bool t_nd_valid = false; // computed since the last tangent assembly
 // version of the device K, bumped by every kernel sequence that writes it (assembly,
 // import, elimination): the block-Jacobi inverse is formed once per version
 static int t_kver = 0, t_bj_kver = -1;
 
-// Shell sums A_s = sum dL, B_s = sum Tv dL of every particle under the elastic law
-// (constitutive.c:241-261): dL = (|x_i - x_j| - L0 - dLp[0]) * damage_broken.
-__global__ void k_elastic_sums(DV v, double *A, double *B)
+// Planes of t_S: A_s = sum dL, B_s = sum Tv dL (s = 0, 1), V_s = sum db n and
+// W_s = sum db Tv n (3 components each) over the particle's shell-s bonds.
+#define TS_A 0
+#define TS_B 2
+#define TS_V 4
+#define TS_W 10
+#define TS_NPLANE 16
+
+// Shell sums of every particle under the elastic law (constitutive.c:241-261):
+// dL = (|x_i - x_j| - L0 - dLp[0]) * damage_broken, n = (x_i - x_j)/|x_i - x_j|.
+__global__ void k_tangent_sums(DV v, double *S)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= v.n)
         return;
-    double a0 = 0, a1 = 0, b0 = 0, b1 = 0;
+    const int64_t n = v.n;
+    double a[2] = {0, 0}, bb[2] = {0, 0}, V[2][3] = {{0, 0, 0}, {0, 0, 0}}, W[2][3] = {{0, 0, 0}, {0, 0, 0}};
     const double xi = v.x[3 * i], yi = v.x[3 * i + 1], zi = v.x[3 * i + 2];
-    for (int a = 0; a < v.nb0[i]; a++)
+    for (int e = 0; e < v.nb0[i]; e++)
     {
-        int64_t b = (int64_t)a * v.n + i;
-        int j = v.nbr[b];
-        double dx = xi - v.x[3 * j], dy = yi - v.x[3 * j + 1], dz = zi - v.x[3 * j + 2];
-        double dis = sqrt(dx * dx + dy * dy + dz * dz);
+        const int64_t b = (int64_t)e * n + i;
+        const int j = v.nbr[b], s = v.nsign[b];
+        const double dx = xi - v.x[3 * j], dy = yi - v.x[3 * j + 1], dz = zi - v.x[3 * j + 2];
+        const double dis = sqrt(dx * dx + dy * dy + dz * dz);
+        const double db = v.db[b], tv = v.Tv[b];
         double dl = dis - v.L0[b];
         dl -= v.dLp0[b];
-        dl *= v.db[b];
-        if (v.nsign[b] == 0)
-            a0 += dl, b0 += v.Tv[b] * dl;
-        else
-            a1 += dl, b1 += v.Tv[b] * dl;
+        dl *= db;
+        const double nv[3] = {dx / dis, dy / dis, dz / dis};
+        for (int q = 0; q < 2; q++)
+            if (s == q)
+            {
+                a[q] += dl, bb[q] += tv * dl;
+                for (int c = 0; c < 3; c++)
+                    V[q][c] += db * nv[c], W[q][c] += db * tv * nv[c];
+            }
     }
-    A[i] = a0, A[(int64_t)v.n + i] = a1;
-    B[i] = b0, B[(int64_t)v.n + i] = b1;
+    for (int q = 0; q < 2; q++)
+    {
+        S[(TS_A + q) * n + i] = a[q];
+        S[(TS_B + q) * n + i] = bb[q];
+        for (int c = 0; c < 3; c++)
+            S[(TS_V + 3 * q + c) * n + i] = V[q][c], S[(TS_W + 3 * q + c) * n + i] = W[q][c];
+    }
 }
 
 __device__ __forceinline__ int conn_find(const int *sc, int nc, int q)
@@ -57,189 +76,174 @@ __device__ __forceinline__ int conn_find(const int *sc, int nc, int q)
     return -1;
 }
 
-// Row i of the Jacobian J_im = dPin_i/dx_m of the elastic law, m in conn[i]; one
-// wavefront per row.  With n = n_ij, g_ik = db_ik n_ik, P = I - n n^T:
-//   J_ii += F P/d + 2 Kn db^2 n n^T,            J_ij -= the same
-//   J_ii += n (sum_k w_k g_ik)^T,  J_ik -= w_k n g_ik^T,  w_k = db (Tv_ik + Tv_ij)/2
-//   J_ij += n (sum_l w_l g_jl)^T,  J_il -= w_l n g_jl^T,  w_l = db (Tv_jl + Tv_ij)/2
-// (k over i's shell-s bonds, l over j's shell-s bonds).  Every LDS block entry has
-// one writer per phase and the phases run in bond order, so the sums are
-// deterministic.
+// Row i of the symmetrised tangent, written straight into the sliced-ELL K and K32:
+// K(i,m) = 0.5 (J_im + J_mi^T) for m != i, the upper triangle of J_ii mirrored for
+// m = i (stiffness.c:441-483), J = dPin/dx the Jacobian of the elastic law.  With
+// n = n_ij, g_ij = db_ij n_ij, F_ij = db_ij [2Kn dL_ij + (B_i + B_j)/2 + Tv_ij (A_i + A_j)/2]
+// (sums over the bond's shell s) and G_ij = F_ij/d (I - n n^T) + 2 Kn db_ij^2 n n^T,
+// the terms of bond (i,j) are
+//   J_ii += G_ij + db_ij/2 n (W_i + Tv_ij V_i)^T            (stretch and i's shell)
+//   J_ij += -G_ij + db_ij/2 n (W_j + Tv_ij V_j)^T - 1/2 (Tv_ij V_i + W_i) g_ij^T
+//   J_il -= db_ij/2 (Tv_jl + Tv_ij) n g_jl^T,   l in j's shell-s bonds (two hops)
+// with V = sum db n, W = sum db Tv n over the particle's shell-s bonds.  A two-hop
+// path i -> j -> l contributes to K(i,l) through J_il and, reversed, through J_li:
+// 0.5 (alpha + beta) n_ij n_jl^T with alpha = -db_ij (Tv_jl + Tv_ij) db_jl / 2 and
+// beta = -db_lj (Tv_ji + Tv_lj) db_ji / 2; for l = i it is J_ii's alpha term.
+// One 64-lane work-group per row.  Lane a handles bond a; every bond's off-diagonal
+// block lands on a distinct conn slot, and the two-hop rounds (bond a of i, lane t =
+// bond t of j) land on distinct slots within a round and are added in bond order:
+// the sums are deterministic.  Rows are mapped so that each XCD assembles a
+// contiguous range, so the partial cache lines of a slice meet in one L2.
 template <int D>
-__global__ __launch_bounds__(64) void k_jacobian(DV v, const double *A, const double *B)
+__global__ __launch_bounds__(64) void k_tangent_rows(DV v, const double *S)
 {
     constexpr int D2 = D * D;
-    __shared__ double J[LPMX_MAXCONN][D2];
-    __shared__ int sc[LPMX_MAXCONN];
-    const int i = blockIdx.x, lane = threadIdx.x;
-    const int n = v.n, nc = v.nb_conn[i];
+    __shared__ double acc[LPMX_MAXCONN][D2];
+    __shared__ double dg[LPMX_MAXNB][D2];
+    __shared__ double bn[LPMX_MAXNB][3], bx[LPMX_MAXNB][3], bdb[LPMX_MAXNB], btv[LPMX_MAXNB], bdbr[LPMX_MAXNB],
+        btvr[LPMX_MAXNB];
+    __shared__ int sc[LPMX_MAXCONN], bj[LPMX_MAXNB], bs[LPMX_MAXNB];
+    const int i = xcd_block(), lane = threadIdx.x;
+    const int64_t n = v.n;
+    const int nc = v.nb_conn[i], nbi = v.nb0[i];
     if (lane < nc)
         sc[lane] = v.conn[(int64_t)lane * n + i];
     for (int e = lane; e < LPMX_MAXCONN * D2; e += 64)
-        (&J[0][0])[e] = 0.0;
+        (&acc[0][0])[e] = 0.0;
     __syncthreads();
-    const int pi = v.dslot[i];
     const double xi[3] = {v.x[3 * i], v.x[3 * i + 1], v.x[3 * i + 2]};
-    const int nbi = v.nb0[i];
-    for (int a = 0; a < nbi; a++)
+    // ---- bonds of i, one per lane
+    if (lane < nbi)
     {
-        const int64_t b = (int64_t)a * n + i;
-        const double db = v.db[b];
-        if (db == 0.0)
-            continue;
+        const int64_t b = (int64_t)lane * n + i;
         const int j = v.nbr[b], s = v.nsign[b];
-        const double Kn = v.Kn[b], Tvij = v.Tv[b];
+        const double db = v.db[b], tv = v.Tv[b], kn = v.Kn[b];
+        const int64_t br = (int64_t)v.rev[b] * n + j;
+        const double dbr = v.db[br], tvr = v.Tv[br], knr = v.Kn[br];
         const double xj[3] = {v.x[3 * j], v.x[3 * j + 1], v.x[3 * j + 2]};
         double nv[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
-        const double dij = sqrt(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]);
+        const double d = sqrt(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]);
         for (int c = 0; c < 3; c++)
-            nv[c] /= dij;
-        double dl = (dij - v.L0[b] - v.dLp0[b]) * db;
-        const double F = db * (2.0 * Kn * dl + 0.5 * (B[(int64_t)s * n + i] + B[(int64_t)s * n + j]) +
-                               0.5 * Tvij * (A[(int64_t)s * n + i] + A[(int64_t)s * n + j]));
+            nv[c] /= d;
+        const double Ai = S[(TS_A + s) * n + i], Bi = S[(TS_B + s) * n + i];
+        const double Aj = S[(TS_A + s) * n + j], Bj = S[(TS_B + s) * n + j];
+        const double dl = (d - v.L0[b] - v.dLp0[b]) * db, dlr = (d - v.L0[br] - v.dLp0[br]) * dbr;
+        const double F = db * (2.0 * kn * dl + 0.5 * (Bi + Bj) + 0.5 * tv * (Ai + Aj));
+        const double Fr = dbr * (2.0 * knr * dlr + 0.5 * (Bj + Bi) + 0.5 * tvr * (Aj + Ai));
+        double Vi[3], Wi[3], Vj[3], Wj[3];
+        for (int c = 0; c < 3; c++)
+        {
+            Vi[c] = S[(TS_V + 3 * s + c) * n + i], Wi[c] = S[(TS_W + 3 * s + c) * n + i];
+            Vj[c] = S[(TS_V + 3 * s + c) * n + j], Wj[c] = S[(TS_W + 3 * s + c) * n + j];
+        }
+        const double fd = F / d, fdr = Fr / d, k2 = 2.0 * kn * db * db, k2r = 2.0 * knr * dbr * dbr;
+        const double w2 = 0.5 * db * dbr * (tvr + tv);
         const int pj = conn_find(sc, nc, j);
-        if (pj < 0 || pi < 0 || pi >= nc)
-            continue; // cannot happen for a conn built from these lists
-        // geometric and direct stretch terms
-        if (lane < D2)
-        {
-            const int r = lane / D, c = lane % D;
-            double gval = F * ((r == c ? 1.0 : 0.0) - nv[r] * nv[c]) / dij + 2.0 * Kn * db * db * nv[r] * nv[c];
-            J[pi][lane] += gval;
-            J[pj][lane] -= gval;
-        }
-        __syncthreads();
-        // i's own shell-s bonds
-        double u[3] = {0, 0, 0};
-        if (lane < nbi)
-        {
-            const int64_t be = (int64_t)lane * n + i;
-            const double dbe = v.db[be];
-            if (v.nsign[be] == s && dbe != 0.0)
+        for (int r = 0; r < D; r++)
+            for (int c = 0; c < D; c++)
             {
-                const int k = v.nbr[be];
-                double gk[3] = {xi[0] - v.x[3 * k], xi[1] - v.x[3 * k + 1], xi[2] - v.x[3 * k + 2]};
-                const double dik = sqrt(gk[0] * gk[0] + gk[1] * gk[1] + gk[2] * gk[2]);
-                for (int c = 0; c < 3; c++)
-                    gk[c] = dbe * gk[c] / dik;
-                const double w = 0.5 * db * (v.Tv[be] + Tvij);
-                const int pk = conn_find(sc, nc, k);
-                if (pk >= 0)
-                    for (int r = 0; r < D; r++)
-                    for (int c = 0; c < D; c++)
-                        J[pk][r * D + c] -= w * nv[r] * gk[c];
-                for (int c = 0; c < 3; c++)
-                    u[c] = w * gk[c];
+                const double nn = nv[r] * nv[c], del = r == c ? 1.0 : 0.0;
+                // J_ij of this bond, and J_ji's transpose (n_ji = -n)
+                const double jij = -(fd * (del - nn) + k2 * nn) + 0.5 * db * nv[r] * (Wj[c] + tv * Vj[c]) -
+                                   0.5 * (tv * Vi[r] + Wi[r]) * db * nv[c];
+                const double jjiT = -(fdr * (del - nn) + k2r * nn) - 0.5 * dbr * (Wi[r] + tvr * Vi[r]) * nv[c] +
+                                    0.5 * dbr * nv[r] * (tvr * Vj[c] + Wj[c]);
+                if (pj >= 0)
+                    acc[pj][r * D + c] = 0.5 * (jij + jjiT);
+                dg[lane][r * D + c] =
+                    fd * (del - nn) + k2 * nn + 0.5 * db * nv[r] * (Wi[c] + tv * Vi[c]) + w2 * nn;
             }
-        }
+        bj[lane] = j, bs[lane] = s;
         for (int c = 0; c < 3; c++)
-            u[c] = wave_sum(u[c]);
-        __syncthreads();
-        if (lane < D2)
-            J[pi][lane] += nv[lane / D] * u[lane % D];
-        __syncthreads();
-        // j's shell-s bonds
-        double q[3] = {0, 0, 0};
-        const int nbj = v.nb0[j];
-        if (lane < nbj)
-        {
-            const int64_t be = (int64_t)lane * n + j;
-            const double dbe = v.db[be];
-            if (v.nsign[be] == s && dbe != 0.0)
-            {
-                const int l = v.nbr[be];
-                double gl[3] = {xj[0] - v.x[3 * l], xj[1] - v.x[3 * l + 1], xj[2] - v.x[3 * l + 2]};
-                const double djl = sqrt(gl[0] * gl[0] + gl[1] * gl[1] + gl[2] * gl[2]);
-                for (int c = 0; c < 3; c++)
-                    gl[c] = dbe * gl[c] / djl;
-                const double w = 0.5 * db * (v.Tv[be] + Tvij);
-                const int pl = conn_find(sc, nc, l);
-                if (pl >= 0)
-                    for (int r = 0; r < D; r++)
-                    for (int c = 0; c < D; c++)
-                        J[pl][r * D + c] -= w * nv[r] * gl[c];
-                for (int c = 0; c < 3; c++)
-                    q[c] = w * gl[c];
-            }
-        }
-        for (int c = 0; c < 3; c++)
-            q[c] = wave_sum(q[c]);
-        __syncthreads();
-        if (lane < D2)
-            J[pj][lane] += nv[lane / D] * q[lane % D];
-        __syncthreads();
+            bn[lane][c] = nv[c], bx[lane][c] = xj[c];
+        bdb[lane] = db, btv[lane] = tv, bdbr[lane] = dbr, btvr[lane] = tvr;
     }
-    const int64_t base = v.rowptr[i] * D2;
-    for (int e = lane; e < nc * D2; e += 64)
-        v.J[base + e] = (&J[0][0])[e];
-}
-
-// K(i,m) = 0.5 (J_im + J_mi^T) for m != i; K(i,i) = the upper triangle of J_ii
-// mirrored (stiffness.c:441-483).  One 64-thread group per slice; lane = row.
-template <int D>
-__global__ __launch_bounds__(64) void k_symmetrize(DV v)
-{
-    constexpr int D2 = D * D;
-    const int s = blockIdx.x, l = threadIdx.x, i = s * LPMX_SLICE + l;
-    if (i >= v.n)
-        return;
-    const int w = v.sptr[s + 1] - v.sptr[s], nc = v.nb_conn[i];
-    const int64_t jrow = v.rowptr[i];
-    for (int k = 0; k < w; k++)
+    __syncthreads();
+    const int pii = v.dslot[i];
+    if (lane < D2)
     {
-        const int64_t slot = v.sptr[s] + k;
-        double out[D2];
-        if (k >= nc)
+        double t = 0.0;
+        for (int a = 0; a < nbi; a++)
+            t += dg[a][lane];
+        acc[pii][lane] += t;
+    }
+    __syncthreads();
+    // ---- two-hop paths i -> j -> l: groups of nb lanes take consecutive bonds of i
+    const int nbm = v.nb, ng = 64 / nbm, grp = lane / nbm, t = lane % nbm;
+    for (int a0 = 0; a0 < nbi; a0 += ng)
+    {
+        const int a = a0 + grp;
+        double gam = 0.0, njl[3] = {0, 0, 0};
+        int pl = -1;
+        if (grp < ng && a < nbi)
         {
-            for (int e = 0; e < D2; e++)
-                out[e] = 0.0;
-        }
-        else
-        {
-            const int m = v.col[slot * LPMX_SLICE + l];
-            const double *Jim = v.J + (jrow + k) * D2;
-            if (m == i)
+            const int j = bj[a], s = bs[a];
+            if (t < v.nb0[j])
             {
+                const int64_t bt = (int64_t)t * n + j;
+                const int l = v.nbr[bt];
+                if (v.nsign[bt] == s && l != i)
+                {
+                    const double dbjl = v.db[bt], tvjl = v.Tv[bt];
+                    const int64_t br = (int64_t)v.rev[bt] * n + l;
+                    const double dblj = v.db[br], tvlj = v.Tv[br];
+                    double u[3] = {bx[a][0] - v.x[3 * l], bx[a][1] - v.x[3 * l + 1], bx[a][2] - v.x[3 * l + 2]};
+                    const double djl = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+                    for (int c = 0; c < 3; c++)
+                        njl[c] = u[c] / djl;
+                    const double alpha = -0.5 * bdb[a] * (tvjl + btv[a]) * dbjl;
+                    const double beta = -0.5 * dblj * (btvr[a] + tvlj) * bdbr[a];
+                    gam = 0.5 * (alpha + beta);
+                    pl = conn_find(sc, nc, l);
+                }
+            }
+        }
+        // groups add in bond order; within a group the slots are distinct
+        for (int q = 0; q < ng; q++)
+        {
+            if (grp == q && pl >= 0)
+            {
+                const int a = a0 + q;
                 for (int r = 0; r < D; r++)
                     for (int c = 0; c < D; c++)
-                        out[r * D + c] = r <= c ? Jim[r * D + c] : Jim[c * D + r];
+                        acc[pl][r * D + c] += gam * bn[a][r] * njl[c];
             }
-            else
+            __syncthreads();
+        }
+    }
+    // ---- the row's blocks in slice order; padding slots hold zero blocks
+    const int si = i / LPMX_SLICE, li = i % LPMX_SLICE;
+    const int w = v.sptr[si + 1] - v.sptr[si];
+    for (int k = lane; k < w; k += 64)
+    {
+        const int64_t slot = v.sptr[si] + k;
+        for (int r = 0; r < D; r++)
+            for (int c = 0; c < D; c++)
             {
-                const int tp = v.tpos[slot * LPMX_SLICE + l];
-                const int kt = tp < 0 ? -1 : tp / LPMX_SLICE - v.sptr[m / LPMX_SLICE];
-                const double *Jmi = v.J + (v.rowptr[m] + (kt < 0 ? 0 : kt)) * D2;
-                for (int r = 0; r < D; r++)
-                    for (int c = 0; c < D; c++)
-                        out[r * D + c] = 0.5 * Jim[r * D + c] + (kt < 0 ? 0.0 : 0.5 * Jmi[c * D + r]);
+                double val = 0.0;
+                if (k < nc)
+                    val = (k == pii && r > c) ? acc[k][c * D + r] : acc[k][r * D + c];
+                const int64_t o = (slot * D2 + r * D + c) * LPMX_SLICE + li;
+                v.K[o] = val;
+                v.K32[o] = (float)val;
             }
-        }
-        for (int e = 0; e < D2; e++)
-        {
-            const int64_t o = (slot * D2 + e) * LPMX_SLICE + l;
-            v.K[o] = out[e];
-            v.K32[o] = (float)out[e];
-        }
     }
 }
 
 static int ensure_tangent_buffers(void)
 {
-    if (!t_A)
-    {
-        HIPCHK(hipMalloc(&t_A, sizeof(double) * 2 * (int64_t)g.n));
-        HIPCHK(hipMalloc(&t_B, sizeof(double) * 2 * (int64_t)g.n));
-    }
+    if (!t_S)
+        HIPCHK(hipMalloc(&t_S, sizeof(double) * TS_NPLANE * (int64_t)g.n));
     return LPMX_OK;
 }
 
 void lpmx_tangent_free(void)
 {
-    if (t_A)
-        hipFree(t_A), hipFree(t_B);
+    if (t_S)
+        hipFree(t_S);
     if (t_kstart)
         hipFree(t_kstart);
-    t_A = t_B = nullptr;
+    t_S = nullptr;
     t_kstart = nullptr;
     t_nd_valid = false;
 }
@@ -250,24 +254,19 @@ int lpmx_tangent_epoch(void) { return t_epoch; }
 extern "C" int lpmx_tangent(void)
 {
     RETCHK(lpmx_require_pattern());
+    if (g.nb > 64 || g.nb > LPMX_MAXNB || g.cm > LPMX_MAXCONN)
+        return lpmx_fail(LPMX_ERR_CAPACITY, "tangent rows handle at most %d bonds and %d conn entries", LPMX_MAXNB,
+                         LPMX_MAXCONN);
     RETCHK(ensure_tangent_buffers());
     t_nd_valid = false;
     t_epoch++;
     t_kver++;
-    hipLaunchKernelGGL(k_elastic_sums, dim3(nblocks(g.n)), dim3(LPMX_BLOCK), 0, g_stream, g, t_A, t_B);
+    hipLaunchKernelGGL(k_tangent_sums, dim3(nblocks(g.n)), dim3(LPMX_BLOCK), 0, g_stream, g, t_S);
     LAUNCHCHK();
     if (g.dim == 3)
-    {
-        hipLaunchKernelGGL(k_jacobian<3>, dim3(g.n), dim3(64), 0, g_stream, g, t_A, t_B);
-        LAUNCHCHK();
-        hipLaunchKernelGGL(k_symmetrize<3>, dim3(g.nslices), dim3(64), 0, g_stream, g);
-    }
+        hipLaunchKernelGGL(k_tangent_rows<3>, dim3(g.n), dim3(64), 0, g_stream, g, t_S);
     else
-    {
-        hipLaunchKernelGGL(k_jacobian<2>, dim3(g.n), dim3(64), 0, g_stream, g, t_A, t_B);
-        LAUNCHCHK();
-        hipLaunchKernelGGL(k_symmetrize<2>, dim3(g.nslices), dim3(64), 0, g_stream, g);
-    }
+        hipLaunchKernelGGL(k_tangent_rows<2>, dim3(g.n), dim3(64), 0, g_stream, g, t_S);
     LAUNCHCHK();
     return LPMX_OK;
 }

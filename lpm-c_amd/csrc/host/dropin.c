@@ -135,6 +135,10 @@ static int bind(int f, struct hb *b)
     case LPMX_F_CP_RSS: b->kind = K_D2, b->p = cp_RSS, b->width = nslipSys; break;
     case LPMX_F_CP_JACT: b->kind = K_I2, b->p = cp_Jact, b->width = nslipSys; break;
     case LPMX_F_CP_CAB: b->kind = K_D2, b->p = cp_Cab, b->width = nslipSys * nslipSys; break;
+    case LPMX_F_J2_BETA_EQ0: case LPMX_F_J2_BETA_EQ1: case LPMX_F_J2_BETA_EQ2:
+        b->kind = K_D2S, b->p = J2_beta_eq, b->slot = f - LPMX_F_J2_BETA_EQ0; break;
+    case LPMX_F_DAMAGE_LOCAL0: case LPMX_F_DAMAGE_LOCAL1:
+        b->kind = K_D2S, b->p = damage_local, b->slot = f - LPMX_F_DAMAGE_LOCAL0; break;
     default: return 0;
     }
     if (f >= LPMX_F_CP_GY0 && f <= LPMX_F_CP_CAB && nslipSys <= 0)
@@ -260,7 +264,7 @@ static void pull_list(const int *f, int k)
 /* ------------------------------------------------------------ device model --- */
 static void push_params(void)
 {
-    lpmx_params p = {plmode, J2_H, J2_xi, damage_L, damage_threshold, damagec_A, critical_bstrain, nbreak};
+    lpmx_params p = {plmode, J2_H, J2_xi, damage_L, damage_threshold, damagec_A, critical_bstrain, nbreak, J2_C};
     CHK("params", lpmx_set_params(&p));
 }
 
@@ -499,19 +503,6 @@ void solverPARDISO()
 }
 
 /* ------------------------------------------------------------- state and laws --- */
-static void host_slot_copy(int to, int from)
-{
-    /* fields the device laws of this path do not use: copied on the host */
-    for (int i = 0; i < nparticle; i++)
-    {
-        if (J2_beta_eq)
-            J2_beta_eq[i][to] = J2_beta_eq[i][from];
-        if (cp_A)
-            cp_A[i][to] = cp_A[i][from];
-        if (to != 2 && from != 2 && damage_local)
-            damage_local[i][to] = damage_local[i][from];
-    }
-}
 
 /* switchStateV (reference src/constitutive.c:10-85). */
 void switchStateV(int conv_flag)
@@ -520,39 +511,39 @@ void switchStateV(int conv_flag)
     PUSH(LPMX_F_DLP0, LPMX_F_DLP1, LPMX_F_DLP2, LPMX_F_DAMAGE_D0, LPMX_F_DAMAGE_D1, LPMX_F_J2_BETA0, LPMX_F_J2_BETA1,
          LPMX_F_J2_BETA2, LPMX_F_J2_ALPHA0, LPMX_F_J2_ALPHA1, LPMX_F_J2_ALPHA2, LPMX_F_DAMAGE_NONLOCAL0,
          LPMX_F_DAMAGE_NONLOCAL1, LPMX_F_CP_GY0, LPMX_F_CP_GY1, LPMX_F_CP_GY2, LPMX_F_CP_A_SINGLE0,
-         LPMX_F_CP_A_SINGLE1, LPMX_F_CP_A_SINGLE2, LPMX_F_CP_A0, LPMX_F_CP_A1, LPMX_F_CP_A2);
+         LPMX_F_CP_A_SINGLE1, LPMX_F_CP_A_SINGLE2, LPMX_F_CP_A0, LPMX_F_CP_A1, LPMX_F_CP_A2, LPMX_F_J2_BETA_EQ0,
+         LPMX_F_J2_BETA_EQ1, LPMX_F_J2_BETA_EQ2, LPMX_F_DAMAGE_LOCAL0, LPMX_F_DAMAGE_LOCAL1);
     CHK("switchStateV", lpmx_switch_state(conv_flag));
     if (conv_flag == 0)
     {
+        PULL(LPMX_F_J2_BETA_EQ0, LPMX_F_DAMAGE_LOCAL0);
         PULL(LPMX_F_DLP0, LPMX_F_DAMAGE_D0, LPMX_F_J2_BETA0, LPMX_F_J2_ALPHA0, LPMX_F_DAMAGE_NONLOCAL0, LPMX_F_CP_GY0,
              LPMX_F_CP_A_SINGLE0, LPMX_F_CP_A0);
-        host_slot_copy(0, 1);
     }
     else if (conv_flag == 1)
     {
+        PULL(LPMX_F_J2_BETA_EQ1, LPMX_F_DAMAGE_LOCAL1);
         PULL(LPMX_F_DLP1, LPMX_F_DAMAGE_D1, LPMX_F_J2_BETA1, LPMX_F_J2_ALPHA1, LPMX_F_DAMAGE_NONLOCAL1, LPMX_F_CP_GY1,
              LPMX_F_CP_A_SINGLE1, LPMX_F_CP_A1);
-        host_slot_copy(1, 0);
     }
     else
-    {
-        PULL(LPMX_F_DLP0, LPMX_F_J2_BETA0, LPMX_F_J2_ALPHA0, LPMX_F_CP_GY0, LPMX_F_CP_A_SINGLE0, LPMX_F_CP_A0);
-        host_slot_copy(0, 2);
-    }
+        PULL(LPMX_F_DLP0, LPMX_F_J2_BETA0, LPMX_F_J2_ALPHA0, LPMX_F_CP_GY0, LPMX_F_CP_A_SINGLE0, LPMX_F_CP_A0,
+             LPMX_F_J2_BETA_EQ0);
 }
 
 static void law_inputs(void)
 {
     PUSH(LPMX_F_XYZ, LPMX_F_XYZ_TEMP, LPMX_F_DLP0, LPMX_F_DLP2, LPMX_F_J2_BETA0, LPMX_F_J2_BETA2, LPMX_F_J2_ALPHA0,
          LPMX_F_J2_ALPHA2, LPMX_F_DAMAGE_BROKEN, LPMX_F_DAMAGE_W, LPMX_F_KN, LPMX_F_TV, LPMX_F_CSX, LPMX_F_CSY,
-         LPMX_F_CSZ, LPMX_F_F_TEMP, LPMX_F_SIGMAY, LPMX_F_NB, LPMX_F_PL_FLAG, LPMX_F_DL_AVE);
+         LPMX_F_CSZ, LPMX_F_F_TEMP, LPMX_F_SIGMAY, LPMX_F_NB, LPMX_F_PL_FLAG, LPMX_F_DL_AVE, LPMX_F_J2_BETA_EQ0,
+         LPMX_F_J2_BETA_EQ2, LPMX_F_DAMAGE_D0);
 }
 
 static void law_outputs(void)
 {
     PULL(LPMX_F_F, LPMX_F_PIN, LPMX_F_DL, LPMX_F_DL_AVE, LPMX_F_DDLP, LPMX_F_DDL, LPMX_F_CSX, LPMX_F_CSY, LPMX_F_CSZ,
          LPMX_F_DL_TOTAL, LPMX_F_TDL_TOTAL, LPMX_F_DDL_TOTAL, LPMX_F_TDDL_TOTAL, LPMX_F_DLP2, LPMX_F_J2_BETA2,
-         LPMX_F_J2_ALPHA2, LPMX_F_J2_DLAMBDA, LPMX_F_PL_FLAG);
+         LPMX_F_J2_ALPHA2, LPMX_F_J2_DLAMBDA, LPMX_F_PL_FLAG, LPMX_F_J2_BETA_EQ2);
 }
 
 /* crystal-plasticity state around a plmode 1 evaluation */
@@ -592,29 +583,23 @@ static void stress_outputs(void)
     PULL(LPMX_F_STRESS, LPMX_F_J2_STRESSEQ, LPMX_F_J2_STRESSM, LPMX_F_J2_TRIAXIALITY, LPMX_F_BOND_STRESS);
 }
 
-static void not_on_path(const char *what)
-{
-    fprintf(stderr, "%s is not part of this build (the device path covers plmode 0, 1, 4 and 6)\n", what);
-    exit(1);
-}
-
 /* computeBondForceGeneral (reference src/constitutive.c:88-146). */
 void computeBondForceGeneral(int pm, int t)
 {
-    (void)t;
-    if (pm != 0 && pm != 1 && pm != 4 && pm != 6)
-        not_on_path("computeBondForceGeneral for this plmode");
+    if (pm != 0 && pm != 1 && pm != 3 && pm != 4 && pm != 5 && pm != 6)
+        return; /* the reference computes no force for other values, and still forms the stress */
     need_model("computeBondForceGeneral");
+    push_params();
     law_inputs();
     if (pm == 1)
         cp_inputs();
-    CHK("computeBondForceGeneral", lpmx_bond_force(pm));
+    CHK("computeBondForceGeneral", lpmx_bond_force_ex(pm, t));
     law_outputs();
     if (pm == 1)
         cp_outputs();
     stress_outputs();
-    PULL(LPMX_F_DLP0, LPMX_F_J2_BETA0, LPMX_F_J2_ALPHA0, LPMX_F_CP_GY0, LPMX_F_CP_A_SINGLE0, LPMX_F_CP_A0);
-    host_slot_copy(0, 2);
+    PULL(LPMX_F_DLP0, LPMX_F_J2_BETA0, LPMX_F_J2_ALPHA0, LPMX_F_CP_GY0, LPMX_F_CP_A_SINGLE0, LPMX_F_CP_A0,
+         LPMX_F_J2_BETA_EQ0);
 }
 
 static void one_particle(int pm, int i)
@@ -628,10 +613,18 @@ static void one_particle(int pm, int i)
 void computeBondForceElastic(int i) { one_particle(6, i); }
 void computeBondForceJ2mixedLinear3D(int ii) { one_particle(0, ii); }
 void computeBondForceIncrementalUpdating(int ii) { one_particle(4, ii); }
+/* The per-particle entry points of plmode 3 and 5 evaluate every particle (each
+ * particle's final force needs its partners' updates); particle ii's force and state
+ * are the reference's. */
 void computeBondForceJ2nonlinearIso(int ii)
 {
     (void)ii;
-    not_on_path("computeBondForceJ2nonlinearIso (plmode 5)");
+    need_model("computeBondForceJ2nonlinearIso");
+    push_params();
+    law_inputs();
+    CHK("computeBondForceJ2nonlinearIso", lpmx_bond_force_range(5, 0, nparticle));
+    law_outputs();
+    PULL(LPMX_F_DLP0, LPMX_F_J2_BETA0, LPMX_F_J2_ALPHA0);
 }
 void computeBondForceCPMiehe(int ii)
 {
@@ -644,8 +637,11 @@ void computeBondForceCPMiehe(int ii)
 }
 void computeBondForceJ2energyReturnMap(int ii, int load_indicator)
 {
-    (void)ii, (void)load_indicator;
-    not_on_path("computeBondForceJ2energyReturnMap (plmode 3)");
+    (void)ii;
+    need_model("computeBondForceJ2energyReturnMap");
+    law_inputs();
+    CHK("computeBondForceJ2energyReturnMap", lpmx_bond_force_range_ex(3, load_indicator, 0, nparticle));
+    law_outputs();
 }
 
 /* computeCab (reference src/constitutive.c:1864-1917) on the device. */
@@ -731,7 +727,7 @@ int updateBrittleDamage(const char *dataName, int tstep, int nbreak_)
 {
     need_model("updateBrittleDamage");
     FILE *fp = bond_file(dataName, tstep);
-    lpmx_params p = {plmode, J2_H, J2_xi, damage_L, damage_threshold, damagec_A, critical_bstrain, nbreak_};
+    lpmx_params p = {plmode, J2_H, J2_xi, damage_L, damage_threshold, damagec_A, critical_bstrain, nbreak_, J2_C};
     CHK("params", lpmx_set_params(&p));
     if (ext_crit)
         CHK("critical stretch", lpmx_set_bond_critical(ext_crit));
@@ -754,25 +750,69 @@ int updateBrittleDamage(const char *dataName, int tstep, int nbreak_)
     return k;
 }
 
+/* The other ductile damage updates (reference src/constitutive.c:1529-1755) on the
+ * device, with the bond-file lines the reference writes as it breaks in index order:
+ * each newly broken entry "i neighbour" (bond-wise nonlocal), each newly broken pair
+ * once, from its lower-index end (bond-wise local), each newly failed particle "i"
+ * (point-wise local). */
+static int ductile_variant(const char *dataName, int tstep, int law, const char *what)
+{
+    need_model(what);
+    push_params();
+    FILE *fp = bond_file(dataName, tstep);
+    PUSH(LPMX_F_DAMAGE_NONLOCAL0, LPMX_F_DAMAGE_LOCAL0, LPMX_F_J2_DLAMBDA, LPMX_F_J2_TRIAXIALITY,
+         LPMX_F_DAMAGE_BROKEN, LPMX_F_DAMAGE_D0, LPMX_F_DAMAGE_W, LPMX_F_NB);
+    const long nbn = (long)nparticle * nneighbors;
+    unsigned char *was = malloc(nbn > 0 ? nbn : 1);
+    unsigned char *failed_before = malloc(nparticle > 0 ? nparticle : 1);
+    for (long i = 0; i < nparticle; i++)
+    {
+        for (long j = 0; j < nneighbors; j++)
+            was[i * nneighbors + j] = damage_broken[i][j] > EPS || damage_broken[i][j] < -EPS;
+        failed_before[i] = fabs(damage_local[i][0] - 1.0) <= EPS;
+    }
+    int k = 0;
+    CHK(what, lpmx_damage_law(law, &k));
+    PULL(LPMX_F_DAMAGE_NONLOCAL0, LPMX_F_DAMAGE_LOCAL0, LPMX_F_DAMAGE_BROKEN, LPMX_F_DAMAGE_D0, LPMX_F_DAMAGE_W,
+         LPMX_F_NB);
+    if (fp)
+    {
+        for (long i = 0; i < nparticle; i++)
+        {
+            if (law == LPMX_DMG_PWISE_LOCAL)
+            {
+                if (!failed_before[i] && damage_local[i][0] == 1.0)
+                    fprintf(fp, "%ld \n", i);
+                continue;
+            }
+            for (long j = 0; j < nb_initial[i]; j++)
+                if (was[i * nneighbors + j] && damage_broken[i][j] == 0.0 &&
+                    (law != LPMX_DMG_BWISE_LOCAL || i < neighbors[i][j]))
+                    fprintf(fp, "%ld %d \n", i, neighbors[i][j]);
+        }
+        fclose(fp);
+    }
+    free(was);
+    free(failed_before);
+    return k;
+}
+
+/* updateDuctileDamageBwiseLocal (reference src/constitutive.c:1607-1695). */
 int updateDuctileDamageBwiseLocal(const char *dataName, int tstep)
 {
-    (void)dataName, (void)tstep;
-    not_on_path("updateDuctileDamageBwiseLocal");
-    return 0;
+    return ductile_variant(dataName, tstep, LPMX_DMG_BWISE_LOCAL, "updateDuctileDamageBwiseLocal");
 }
 
+/* updateDuctileDamagePwiseLocal (reference src/constitutive.c:1698-1755). */
 int updateDuctileDamagePwiseLocal(const char *dataName, int tstep)
 {
-    (void)dataName, (void)tstep;
-    not_on_path("updateDuctileDamagePwiseLocal");
-    return 0;
+    return ductile_variant(dataName, tstep, LPMX_DMG_PWISE_LOCAL, "updateDuctileDamagePwiseLocal");
 }
 
+/* updateDuctileDamageBwiseNonlocal (reference src/constitutive.c:1529-1605). */
 int updateDuctileDamageBwiseNonlocal(const char *dataName, int tstep)
 {
-    (void)dataName, (void)tstep;
-    not_on_path("updateDuctileDamageBwiseNonlocal");
-    return 0;
+    return ductile_variant(dataName, tstep, LPMX_DMG_BWISE_NONLOCAL, "updateDuctileDamageBwiseNonlocal");
 }
 
 /* updateDamageGeneral (reference src/constitutive.c:149-164). */

@@ -124,6 +124,11 @@ typedef struct orc_model
     double *cp_Cab;      /* n*nslip*nslip */
     int *state_v;        /* n: return map done in the current sweep */
     int cp_fail;         /* singular slip Jacobians met */
+
+    /* plmode 5: kinematic modulus J2_C; 1 for the one-update-per-particle form in
+     * orc_bond_force_general (0: the reference's serial order) */
+    double J2_C;
+    int j2nl_jacobi;
 } orc_model;
 
 typedef struct orc_dispbc
@@ -271,6 +276,11 @@ double orc_reaction(const orc_model *m, char axis, int t1);
 double orc_mean_disp(const orc_model *m, char axis, int t1);
 
 /* ---- load-step loop ------------------------------------------------------------- */
+/* Reference-format files that orc_run_steps appends to (NULL: none): the broken-bond
+ * file of the damage update, and writeReaction / writeDisp of (axis, t1) after every
+ * Newton loop of a step; step numbers are the step index + 1 + step_offset. */
+void orc_set_output(const char *bond_file, const char *reaction_file, const char *disp_file, char axis, int t1,
+                    int step_offset);
 int orc_run_steps(orc_model *m, const orc_solver *s, int nsteps,
                   const int *nbd, const orc_dispbc *dbc, const int *nbf, const orc_forcebc *fbc,
                   int maxbc, char out_axis, int out_type, orc_stepout *out);

@@ -768,6 +768,18 @@ void orc_switch_state(orc_model *m, int flag)
  * particle order, then stress and switchStateV(2). */
 void orc_bond_force_general(orc_model *m, int mode)
 {
+    /* plmode 3 and 5 (with their own stress and switch): load indicator +1; plmode 5 in
+     * the reference's serial order, or in its one-update-per-particle form */
+    if (mode == 3)
+    {
+        orc_bond_force_j2energy(m, 1, 0);
+        return;
+    }
+    if (mode == 5)
+    {
+        orc_bond_force_j2nonlinear(m, m->J2_C, m->j2nl_jacobi);
+        return;
+    }
     if (mode == 1)
         memset(m->state_v, 0, sizeof(int) * m->n);
     for (int i = 0; i < m->n; i++)

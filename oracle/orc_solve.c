@@ -225,6 +225,10 @@ double orc_cg_baseline(const orc_model *m, int nthreads, int iters, double *rr_o
     return secs;
 }
 
+/* Broken-bond file of the damage update being run (NULL: none); the damage
+ * functions write their lines where the reference writes them. */
+static FILE *o_fp = NULL;
+
 /* updateDuctileDamagePwiseNonlocal (reference src/constitutive.c:1757-1862):
  * Gaussian-weighted average of dlambda*(1 + A*triaxiality) over all particles
  * within 3L in the initial configuration (index order), bond breakage when either
@@ -272,6 +276,8 @@ int orc_damage_pwise_nonlocal(orc_model *m)
                 {
                     m->dmg_broken[b] = 0.0;
                     m->dmgD[2 * b] = 1.0;
+                    if (o_fp)
+                        fprintf(o_fp, "%d %d \n", i, q);
                     k++;
                 }
         }
@@ -369,6 +375,8 @@ int orc_damage_bwise_nonlocal(orc_model *m)
             {
                 m->dmg_broken[b] = 0.0;
                 m->dmgD[2 * b + 0] = 1.0;
+                if (o_fp)
+                    fprintf(o_fp, "%d %d \n", i, q);
                 k++;
             }
         }
@@ -412,6 +420,8 @@ int orc_damage_pwise_local(orc_model *m, int order)
                     if (m->nbr[(int64_t)q * nbm + jj] == i)
                         m->dmg_broken[(int64_t)q * nbm + jj] = 0.0;
             }
+            if (o_fp)
+                fprintf(o_fp, "%d \n", i);
             k++;
         }
     }
@@ -464,6 +474,8 @@ int orc_damage_bwise_local(orc_model *m, int order)
                         m->dmgD[2 * ((int64_t)q * nbm + jj) + 0] = 1.0;
                         m->dmg_broken[(int64_t)q * nbm + jj] = 0.0;
                     }
+                if (o_fp)
+                    fprintf(o_fp, "%d %d \n", i, q);
                 k++;
             }
             if (m->dmg_broken[b] <= O_EPS)
@@ -523,6 +535,8 @@ int orc_damage_brittle(orc_model *m, int *bi, int *bj, int cap)
             m->dmg_broken[b] = 0.0;
             if (nb_out < cap)
                 bi[nb_out] = ci[t], bj[nb_out] = m->nbr[b], nb_out++;
+            if (o_fp)
+                fprintf(o_fp, "%d %d \n", (int)(b / m->nbmax), m->nbr[b]);
         }
     }
     else if (k > m->nbreak)
@@ -537,6 +551,8 @@ int orc_damage_brittle(orc_model *m, int *bi, int *bj, int cap)
             m->dmg_broken[b] = 0.0;
             if (nb_out < cap)
                 bi[nb_out] = ci[ord[t]], bj[nb_out] = m->nbr[b], nb_out++;
+            if (o_fp)
+                fprintf(o_fp, "%d %d \n", (int)(b / m->nbmax), m->nbr[b]);
         }
         free(ord);
     }
@@ -627,13 +643,65 @@ static double first_bc_diag(const orc_model *m)
     return -1.0;
 }
 
-static int damage_general(orc_model *m)
+/* Reference-format files written by orc_run_steps when set (NULL: none): the
+ * broken-bond file of updateBrittleDamage / updateDuctileDamagePwiseNonlocal, a
+ * "TIMESTEP" line per call and one "i neighbour" line per newly broken bond in the
+ * order the reference breaks them (constitutive.c:1440-1442, 1481, 1519, 1760-1763,
+ * 1838), and writeReaction / writeDisp of (axis, type) after every Newton loop of a
+ * step, re-solves after breakage included (data_handler.c:278-335, called at
+ * lpmc_project.c:494-501).  Step numbers are the step index + 1 + step_offset. */
+static char o_bond[1024], o_reac[1024], o_disp[1024];
+static char o_axis = 'z';
+static int o_type = 1, o_off = 0;
+
+void orc_set_output(const char *bond_file, const char *reaction_file, const char *disp_file, char axis, int t1,
+                    int step_offset)
 {
+    snprintf(o_bond, sizeof(o_bond), "%s", bond_file ? bond_file : "");
+    snprintf(o_reac, sizeof(o_reac), "%s", reaction_file ? reaction_file : "");
+    snprintf(o_disp, sizeof(o_disp), "%s", disp_file ? disp_file : "");
+    o_axis = axis, o_type = t1, o_off = step_offset;
+}
+
+static int damage_general(orc_model *m, int tstep)
+{
+    /* updateDamageGeneral (reference src/constitutive.c:149-164) */
+    o_fp = o_bond[0] ? fopen(o_bond, "a+") : NULL;
+    if (o_fp)
+        fprintf(o_fp, "TIMESTEP %d\n", tstep);
+    int k = 0;
     if (m->plmode == 0)
-        return orc_damage_pwise_nonlocal(m);
-    if (m->plmode == 6)
-        return orc_damage_brittle(m, NULL, NULL, 0);
-    return 0;
+        k = orc_damage_pwise_nonlocal(m);
+    else if (m->plmode == 5)
+        k = orc_damage_bwise_local(m, 0);
+    else if (m->plmode == 6)
+        k = orc_damage_brittle(m, NULL, NULL, 0);
+    if (o_fp)
+        fclose(o_fp);
+    o_fp = NULL;
+    return k;
+}
+
+static void write_outputs(const orc_model *m, int tstep)
+{
+    if (o_reac[0])
+    {
+        FILE *fp = fopen(o_reac, "a+");
+        if (fp)
+        {
+            fprintf(fp, "%d %8.8e \n", tstep, orc_reaction(m, o_axis, o_type));
+            fclose(fp);
+        }
+    }
+    if (o_disp[0])
+    {
+        FILE *fp = fopen(o_disp, "a+");
+        if (fp)
+        {
+            fprintf(fp, "%d %8.8e\n", tstep, orc_mean_disp(m, o_axis, o_type));
+            fclose(fp);
+        }
+    }
 }
 
 /* The load-step loop of the reference driver (src/lpmc_project.c:382-546) with the
@@ -806,9 +874,10 @@ int orc_run_steps(orc_model *m, const orc_solver *s, int nsteps,
             o.newton_iters += ni;
             o.res_norm = nres;
             o.tol_mult = tolm;
-            int broken = s->skip_damage ? 0 : damage_general(m);
+            int broken = s->skip_damage ? 0 : damage_general(m, st + 1 + o_off);
             orc_update_crack(m);
             orc_switch_state(m, 1);
+            write_outputs(m, st + 1 + o_off);
             o.broken += broken;
             o.reaction = orc_reaction(m, out_axis, out_type);
             o.disp = orc_mean_disp(m, out_axis, out_type);

@@ -45,7 +45,7 @@ class OrcModel(C.Structure):
         ("cp_eta", C.c_double), ("cp_p", C.c_double), ("dtime", C.c_double), ("cp_maxloop", C.c_int),
         ("cp_gy", _dp), ("cp_A_single", _dp), ("cp_A", _dp), ("cp_dA", _dp),
         ("cp_dgy", _dp), ("cp_dA_single", _dp), ("cp_RSS", _dp), ("cp_Jact", _ip), ("cp_Cab", _dp),
-        ("state_v", _ip), ("cp_fail", C.c_int),
+        ("state_v", _ip), ("cp_fail", C.c_int), ("J2_C", C.c_double), ("j2nl_jacobi", C.c_int),
     ]
 
 
@@ -144,6 +144,8 @@ def lib():
         L.orc_run_steps.argtypes = [P, C.POINTER(OrcSolver), C.c_int, _ip, C.POINTER(OrcDispBC), _ip,
                                     C.POINTER(OrcForceBC), C.c_int, C.c_char, C.c_int, C.POINTER(OrcStepOut)]
         L.orc_run_steps.restype = C.c_int
+        L.orc_set_output.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, C.c_char, C.c_int, C.c_int]
+        L.orc_set_output.restype = None
         _lib = L
     return _lib
 
@@ -185,7 +187,7 @@ class Model:
 
     def __init__(self, x, lattice, radius, types, Ce, box_z=1.0, plmode=0, sigmay=200.0,
                  J2_H=0.0, J2_xi=0.0, damage_L=0.5, damage_threshold=0.9, damagec_A=0.0,
-                 critical_bstrain=1e-2, nbreak=20, emod=None, crit=None, cp=None):
+                 critical_bstrain=1e-2, nbreak=20, emod=None, crit=None, cp=None, J2_C=0.0, j2nl_jacobi=0):
         lp = lattice_params(lattice, radius, box_z)
         n = x.shape[0]
         d = lp.dim
@@ -261,6 +263,7 @@ class Model:
         s.plmode, s.J2_H, s.J2_xi = plmode, J2_H, J2_xi
         s.damage_L, s.damage_threshold, s.damagec_A = damage_L, damage_threshold, damagec_A
         s.critical_bstrain, s.nbreak = critical_bstrain, nbreak
+        s.J2_C, s.j2nl_jacobi = J2_C, j2nl_jacobi
         if cp is not None:
             s.nslip = A["schmid"].shape[0]
             s.cp_tau0, s.cp_taus, s.cp_h0, s.cp_q = cp["tau0"], cp["taus"], cp["h0"], cp["q"]
@@ -329,6 +332,12 @@ def run_steps(model, steps, solver, out_axis="z", out_type=1):
     rc = lib().orc_run_steps(model.p, C.byref(solver), len(steps), ptr(nbd), dbc, ptr(nbf), fbc, maxbc,
                              out_axis.encode(), out_type, out)
     return rc, list(out)
+
+
+def set_output(bond=None, reaction=None, disp=None, axis="z", t1=1, step_offset=0):
+    """Reference-format files orc_run_steps appends to (None: off)."""
+    enc = lambda p: None if p is None else str(p).encode()
+    lib().orc_set_output(enc(bond), enc(reaction), enc(disp), axis.encode(), t1, step_offset)
 
 
 def adjoint_run(model, steps, solver, target, out_axis="z", out_type=1):

@@ -11,7 +11,7 @@ import subprocess
 import numpy as np
 import pytest
 
-from lpm_harness import ROOT, TOL_FP64, P, assert_rel, get, oracle_model, push, sc_pair, tol_for
+from lpm_harness import ROOT, TOL_CP, TOL_FP64, P, assert_rel, get, oracle_model, push, sc_pair, tol_for
 
 pytestmark = pytest.mark.gpu
 
@@ -313,11 +313,12 @@ def test_accelerated_newton_same_solution(oracle, prec):
     k = 6
     _, n, x, t, m = sc_pair(oracle, k)
     steps = P.sc_uniaxial_steps(k - 1.0, np.array([0.5e-3, 1e-3, 1.5e-3, 2e-3]))
+    # Newton to 1e-12: at 1e-11 the plastic steps fix the displacements to ~1.5e-10 only
     s = P.solver(precision=prec, rtol2=1e-24 if prec == P.LPMX_FP64 else 1e-16, atol2=0.0, maxit=100 * 3 * n,
-                 newton_tol=1e-11, newton_maxit=600, accel_depth=5)
+                 newton_tol=1e-12, newton_maxit=600, accel_depth=5)
     rc, out = P.run_steps(steps, s, out_axis="z", out_type=1)
     assert rc == 0, (P.lib().lpmx_last_error(), [(o.status, o.newton_iters, o.cg_iters) for o in out])
-    so = oracle.solver(rtol2=1e-24, atol2=0.0, maxit=100 * 3 * n, newton_tol=1e-11, newton_maxit=600, tangent=1,
+    so = oracle.solver(rtol2=1e-26, atol2=0.0, maxit=100 * 3 * n, newton_tol=1e-12, newton_maxit=1200, tangent=1,
                        fixed_diag=1)
     rco, outo = oracle.run_steps(m, steps, so, out_axis="z", out_type=1)
     assert rco == 0
@@ -477,7 +478,7 @@ def test_crystal_plasticity_load_steps_match_oracle(oracle, prec):
     r, ro = np.array([o.reaction for o in out]), np.array([o.reaction for o in outo])
     assert_rel("reaction", r, ro, tol_for(prec))
     u, uo = get("XYZ", (n, 3)) - x, m.x - x
-    assert_rel("displacement", u, uo, tol_for(prec))
+    assert_rel("displacement", u, uo, TOL_CP)
 
 
 def test_j2_vjp_matches_oracle(oracle):
@@ -791,4 +792,4 @@ def test_vcycle_newton_matches_oracle(oracle, case, prec):
     r, ro = np.array([o.reaction for o in out]), np.array([o.reaction for o in outo])
     assert_rel("reaction", r, ro, tol_for(prec))
     u, uo = get("XYZ", (n, 3)) - x, m.x - x
-    assert_rel("displacement", u, uo, tol_for(prec))
+    assert_rel("displacement", u, uo, TOL_CP if case == "cp" else tol_for(prec))
