@@ -185,6 +185,15 @@ int lpmx_compute_stress(void);
 int lpmx_switch_state(int flag);
 /* updateRR: residual and reaction on the device; norms returned (NULL allowed). */
 int lpmx_update_rr(double *norm_residual, double *norm_reaction, int *nreaction);
+/* lpmx_update_rr without any allocation on any call (no counterpart in the reference:
+ * for callers that cannot allocate device memory, such as code under HIP graph
+ * capture or a tight serving loop).  Same kernels and results, bit for bit; its
+ * intermediates live in one static device array of the library (16 MiB per GPU,
+ * held until the process ends), so models of at most LPMX_NOALLOC_MAX_PARTICLES
+ * particles (larger ones fail with LPMX_ERR_ARG), and calls that overlap on one GPU
+ * race. */
+#define LPMX_NOALLOC_MAX_PARTICLES 1048576
+int lpmx_update_rr_noalloc(double *norm_residual, double *norm_reaction, int *nreaction);
 int lpmx_get_reaction_force(double *host, int count);
 
 typedef struct lpmx_dispbc { int type; char flag; double step; } lpmx_dispbc;

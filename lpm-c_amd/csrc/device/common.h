@@ -93,6 +93,9 @@ int lpmx_hip_fail(hipError_t e, const char *what, const char *file, int line);
     } while (0)
 
 int lpmx_require_model(void);
+// updateRR's kernels (bondforce.hip), shared by lpmx_update_rr and its allocation-free form
+__global__ void k_update_rr(DV v, double *sq_res, double *sq_rea);
+__global__ void k_count_bc(const int *dispBC, int m, int *cnt);
 // laws.hip: plmode 3 (load indicator t) and plmode 5 force evaluations of every
 // particle, and the damage variants (codes of include/lpmx.h)
 int lpmx_law3(int t);
