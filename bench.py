@@ -41,8 +41,14 @@ smoother SpMV k_slice_mv, the CG SpMV k_cg_spmv), timed with HIP events on the
 library's stream; achieved = algorithmic bytes per launch / mean launch time.
 `rooflines` lists all profiled kernels.
 
-cpu_baseline: the oracle's threaded CG (reference solverCG algorithm) on the same 1M
-matrix, a bounded number of iterations on the host cores, in DOF-iters/s.
+schedule: each form also runs the rest of the 50-step schedule after the timed steps;
+the line gives the whole schedule's time with and without the model set-up (the AMG
+set-up runs inside step 1's first solve, so it is in the step times).
+
+cpu_baseline: the oracle's load step (the reference's algorithm and tolerances) on
+the SC 20^3 lattice of the same workload, as many steps as fit the --cpu-seconds
+budget, SpMV threaded over the host cores, in load steps/s; cpu_cg_baseline: the
+oracle's threaded CG on the same 1M matrix, in DOF-iters/s.
 """
 import argparse
 import ctypes as C
@@ -84,9 +90,15 @@ def log(msg):
 
 
 def product():
-    from lpm_harness import P  # loads lpm-c_amd/liblpm.so; fails loudly without it
+    """The product's Python module (lpm-c_amd/__init__.py, loaded by path because of the
+    package directory's name); it loads lpm-c_amd/liblpm.so and fails loudly without it."""
+    import importlib.util
 
-    return P
+    spec = importlib.util.spec_from_file_location("lpm_c_amd", os.path.join(ROOT, "lpm-c_amd", "__init__.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    mod.lib()
+    return mod
 
 
 def schedule(P):
@@ -94,15 +106,17 @@ def schedule(P):
     return P.sc_uniaxial_steps(EDGE - 1.0, strains)
 
 
-def run_form(P, prec, warmup, steps, profile, accel_depth, newton_tol, newton_maxit, solver):
+def run_form(P, prec, warmup, steps, profile, accel_depth, newton_tol, newton_maxit, solver, rest=False):
     L = P.lib()
     cfg = SOLVERS[solver]
     accel_depth = cfg["accel_depth"] if accel_depth is None else accel_depth
     newton_maxit = cfg["newton_maxit"] if newton_maxit is None else newton_maxit
-    t0 = time.time()
+    t0 = time.perf_counter()
     emod = P.uniform_modulus(EDGE ** 3, 0)
     n, x, _, _ = P.build_sc_model(BOX, emod=emod)
-    log(f"form {prec}: model of {n} particles built in {time.time() - t0:.1f} s")
+    sync()
+    t_model = time.perf_counter() - t0
+    log(f"form {prec}: model of {n} particles built in {t_model:.1f} s")
     sched = schedule(P)
     amg = cfg["precond"] > 0
     # mixed form: with block-Jacobi the FP32 residual is replaced by the FP64 one after
@@ -110,11 +124,13 @@ def run_form(P, prec, warmup, steps, profile, accel_depth, newton_tol, newton_ma
     # iterations, so its FP32 CG runs to the outer threshold in one segment
     s = P.solver(precision=prec, rtol2=RTOL2, atol2=ATOL2, maxit=20000, newton_tol=newton_tol, newton_maxit=newton_maxit,
                  inner_rtol2=RTOL2 if amg else 1e-2, ir_maxit=100, accel_depth=accel_depth, precond=cfg["precond"])
+    tw = time.perf_counter()
     if warmup:
         rc, wout = P.run_steps(sched[:warmup], s)
         P.check(rc, "warmup steps")
         log(f"form {prec}: warmup {warmup} steps, Newton {[o.newton_iters for o in wout]}")
     sync()
+    t_warm = time.perf_counter() - tw
     if profile:
         L.lpmx_profile_spmv(1)
         L.lpmx_profile_force(1)
@@ -154,8 +170,18 @@ def run_form(P, prec, warmup, steps, profile, accel_depth, newton_tol, newton_ma
         log(f"form {prec}: {L.lpmx_last_error().decode()} "
             f"{[(o.status, o.newton_iters, o.cg_iters) for o in out]}")
     u = P.field("XYZ", 3 * n).reshape(n, 3) - x
-    return dict(n=n, wall=wall, out=out, rc=rc, prof=prof, amg=amg_info, u=u, solver=solver,
-                accel_depth=accel_depth, newton_maxit=newton_maxit)
+    res = dict(n=n, wall=wall, out=out, rc=rc, prof=prof, amg=amg_info, u=u, solver=solver,
+               accel_depth=accel_depth, newton_maxit=newton_maxit, t_model=t_model, t_warm=t_warm)
+    if rest:
+        # the rest of the 50-step schedule, for the whole-schedule figure (not in `value`)
+        tr = time.perf_counter()
+        rrc, rout = P.run_steps(sched[warmup + steps:], s)
+        sync()
+        res["t_rest"] = time.perf_counter() - tr
+        res["rest_rc"] = rrc
+        res["rest_newton"] = [o.newton_iters for o in rout]
+        log(f"form {prec}: steps {warmup + steps + 1}..{len(sched)} in {res['t_rest']:.2f} s, rc {rrc}")
+    return res
 
 
 def sync():
@@ -234,6 +260,42 @@ def form_rooflines(r, form):
     time over the timed steps) first."""
     ks = sorted(r["prof"], key=lambda k: -r["prof"][k][0])
     return [dict(roofline(r["prof"][k], form, k), total_ms=r["prof"][k][0]) for k in ks]
+
+
+CPU_EDGE, CPU_BOX = 20, 19.035  # SC 20^3 = 8,000 particles
+
+
+def cpu_load_steps(P, budget_s, cores):
+    """The oracle's load step (the reference's algorithm: elastic tangent per step,
+    modified Newton to TOLITER = 1e-4 of max(|R0|, |reaction|) with MAXITER = 100,
+    unpreconditioned CG to |r|^2 <= 1e-8 |r0|^2 + 1e-12, the J2 law and the nonlocal
+    damage) on the SC 20^3 lattice of the same material and modulus field, tension to
+    2% in 50 steps, threaded SpMV on `cores` host threads: steps 1.. until the budget
+    is spent, in load steps/s."""
+    import oracle_py as O
+
+    x = O.create_lattice(2, 0.5, [0, CPU_BOX, 0, CPU_BOX, 0, CPU_BOX])
+    n = len(x)
+    c = O.elastic_constants(200e3, 0.3)
+    m = O.Model(x, 2, 0.5, P.sc_types(x), [c] * 5, plmode=0, sigmay=200.0, J2_H=2000.0,
+                emod=P.uniform_modulus(n, 0))
+    m.setup(cells=True)
+    strains = np.arange(1, NSTEPS_SCHEDULE + 1) * (STRAIN_END / NSTEPS_SCHEDULE)
+    steps = P.sc_uniaxial_steps(CPU_EDGE - 1.0, strains)
+    s = O.solver(rtol2=RTOL2, atol2=ATOL2, maxit=3 * n, newton_tol=1e-4, newton_maxit=100, tangent=1,
+                 nthreads=cores)
+    t0 = time.perf_counter()
+    done, iters = 0, []
+    while done < len(steps) and time.perf_counter() - t0 < budget_s:
+        rc, o = O.run_steps(m, steps[done:done + 1], s)
+        iters.append(o[0].newton_iters)
+        done += 1
+    t = time.perf_counter() - t0
+    return {"value": done / t, "unit": "load steps/s", "cores": cores, "kind": "port",
+            "sample": f"load steps 1-{done} of the same tension schedule on the SC {CPU_EDGE}^3 = {n}-particle "
+                      f"lattice (the oracle's restatement of the reference load step at the reference's "
+                      f"tolerances, TOLITER 1e-4, unpreconditioned CG, SpMV on {cores} threads), {t:.1f} s; "
+                      f"Newton iterations {iters}"}
 
 
 def cpu_baseline(P, budget_s):
@@ -443,16 +505,19 @@ def main():
     for f in forms:
         prec = P.LPMX_FP64 if f == "fp64" else P.LPMX_MIXED
         res[f] = run_form(P, prec, a.warmup, a.steps, profile=True, accel_depth=a.accel_depth,
-                          newton_tol=a.newton_tol, newton_maxit=a.newton_maxit, solver=a.solver)
+                          newton_tol=a.newton_tol, newton_maxit=a.newton_maxit, solver=a.solver, rest=True)
         log(f"{f}: {a.steps} steps in {res[f]['wall']:.2f} s")
     main_form = "mixed" if "mixed" in res else forms[0]
     r = res[main_form]
     n = r["n"]
     sm = {f: summarize(res[f], n) for f in res}
     cfg = SOLVERS[a.solver]
+    # a timed step that hit the Newton or CG cap voids the rate
+    converged = all(res[f]["rc"] == 0 for f in res)
     line = {
         "metric": METRIC,
-        "value": sm[main_form]["load_steps_per_s"],
+        "value": sm[main_form]["load_steps_per_s"] if converged else None,
+        "converged": converged,
         "unit": "load steps/s",
         "n_gpus": 1,
         "steps": a.steps,
@@ -473,6 +538,18 @@ def main():
         "forms": sm,
     }
     line.update(compare_forms(res, sm))
+    # the whole 50-step schedule of each form: model set-up, AMG set-up (inside the
+    # first solve of step 1), every load step
+    line["schedule"] = {}
+    for f, rf in res.items():
+        t_steps = rf["t_warm"] + rf["wall"] + rf.get("t_rest", 0.0)
+        amg_s = rf["amg"]["setup_s"] if rf.get("amg") else 0.0
+        line["schedule"][f] = {
+            "steps": NSTEPS_SCHEDULE, "converged": rf["rc"] == 0 and rf.get("rest_rc", 0) == 0,
+            "model_setup_s": rf["t_model"], "amg_setup_s": amg_s, "steps_s": t_steps,
+            "load_steps_per_s_all_steps": NSTEPS_SCHEDULE / t_steps,
+            "load_steps_per_s_with_setup": NSTEPS_SCHEDULE / (t_steps + rf["t_model"]),
+            "newton_iters_rest": rf.get("rest_newton")}
     rl = form_rooflines(r, main_form) if r["prof"] else []
     line["roofline"] = rl[0] if rl else None
     line["rooflines"] = rl
@@ -495,7 +572,9 @@ def main():
         line["comparison"] = cmp_
     if not a.no_cpu:
         t0 = time.time()
-        line["cpu_baseline"] = cpu_baseline(P, a.cpu_seconds)
+        cg = cpu_baseline(P, a.cpu_seconds)
+        line["cpu_baseline"] = cpu_load_steps(P, a.cpu_seconds, cg["cores"] if cg else 1)
+        line["cpu_cg_baseline"] = cg
         log(f"cpu baseline in {time.time() - t0:.1f} s")
     else:
         line["cpu_baseline"] = None

@@ -27,11 +27,16 @@
 #define AA_MAX 32
 
 // dC_j = dU_j + dF_j (the iterate and correction differences, summed: the mix uses
-// only their sum) and dF_j (the Gram matrix and right-hand side use the corrections).
-struct AaPtrs
+// only their sum) and dF_j (the Gram matrix and right-hand side use the corrections),
+// stored in FP64, or in FP32 for the mixed-precision Newton loop (the iterate, the
+// corrections and the dot products stay FP64).
+template <typename T>
+struct AaPtrsT
 {
-    double *dC[AA_MAX], *dF[AA_MAX];
+    T *dC[AA_MAX], *dF[AA_MAX];
 };
+typedef AaPtrsT<double> AaPtrs;
+typedef AaPtrsT<float> AaPtrs32;
 
 struct AaGamma
 {
@@ -43,9 +48,10 @@ struct AaGamma
 // xs = dim for a DOF vector).
 struct AaHist
 {
-    int n = 0, dim = 0, depth = 0, cnt = 0, next = 0, have_prev = 0;
+    int n = 0, dim = 0, depth = 0, cnt = 0, next = 0, have_prev = 0, f32 = 0;
     double *u_prev = nullptr, *f_prev = nullptr, *parts = nullptr, *out = nullptr;
     AaPtrs P{};
+    AaPtrs32 P32{};
     double G[AA_MAX][AA_MAX];
 };
 
@@ -59,10 +65,10 @@ static void aa_release(AaHist *a)
 {
     for (int j = 0; j < AA_MAX; j++)
     {
-        if (a->P.dC[j])
-            hipFree(a->P.dC[j]);
-        if (a->P.dF[j])
-            hipFree(a->P.dF[j]);
+        void *ps[] = {a->P.dC[j], a->P.dF[j], a->P32.dC[j], a->P32.dF[j]};
+        for (void *p : ps)
+            if (p)
+                hipFree(p);
     }
     void *ps[] = {a->u_prev, a->f_prev, a->parts, a->out};
     for (void *p : ps)
@@ -75,9 +81,10 @@ void lpmx_accel_free(void) { aa_release(&A); }
 
 // u = x - d (the iterate the update started from); with a previous iterate the new
 // differences go to slot s; then (u, d) become the previous iterate.
+template <typename T>
 __global__ void k_aa_push(int n, int dim, int xs, const double *__restrict__ x, const double *__restrict__ d,
-                          double *__restrict__ u_prev, double *__restrict__ f_prev, double *__restrict__ dC,
-                          double *__restrict__ dF, int have_prev)
+                          double *__restrict__ u_prev, double *__restrict__ f_prev, T *__restrict__ dC,
+                          T *__restrict__ dF, int have_prev)
 {
     const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (t >= (int64_t)n * dim)
@@ -88,8 +95,8 @@ __global__ void k_aa_push(int n, int dim, int xs, const double *__restrict__ x, 
     if (have_prev)
     {
         const double du = u - u_prev[t], df = d[t] - f_prev[t];
-        dC[t] = du + df;
-        dF[t] = df;
+        dC[t] = (T)(du + df);
+        dF[t] = (T)df;
     }
     u_prev[t] = u;
     f_prev[t] = d[t];
@@ -99,8 +106,9 @@ __global__ void k_aa_push(int n, int dim, int xs, const double *__restrict__ x, 
 // Every slot is read: the host points the slots from cnt on at slot s (cached after
 // its first read), so the AA_MAX loads of an element are issued together.  With the
 // loads under `j < cnt` each one was waited for before the next was issued.
-__global__ __launch_bounds__(256) void k_aa_dots(int64_t m, AaPtrs P, int cnt, int s, const double *__restrict__ d,
-                                                 double *parts)
+template <typename T>
+__global__ __launch_bounds__(256) void k_aa_dots(int64_t m, AaPtrsT<T> P, int cnt, int s,
+                                                 const double *__restrict__ d, double *parts)
 {
     double g[AA_MAX], h[AA_MAX];
 #pragma unroll
@@ -108,11 +116,11 @@ __global__ __launch_bounds__(256) void k_aa_dots(int64_t m, AaPtrs P, int cnt, i
         g[j] = 0.0, h[j] = 0.0;
     for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < m; t += (int64_t)gridDim.x * blockDim.x)
     {
-        const double fs = P.dF[s][t], dt = d[t];
+        const double fs = (double)P.dF[s][t], dt = d[t];
         double f[AA_MAX];
 #pragma unroll
         for (int j = 0; j < AA_MAX; j++)
-            f[j] = P.dF[j][t];
+            f[j] = (double)P.dF[j][t];
 #pragma unroll
         for (int j = 0; j < AA_MAX; j++)
         {
@@ -159,7 +167,8 @@ __global__ __launch_bounds__(256) void k_aa_dots_total(const double *parts, int 
 }
 
 // x -= sum_j gamma_j dC_j
-__global__ void k_aa_mix(int n, int dim, int xs, double *__restrict__ x, AaPtrs P, int cnt, AaGamma gam)
+template <typename T>
+__global__ void k_aa_mix(int n, int dim, int xs, double *__restrict__ x, AaPtrsT<T> P, int cnt, AaGamma gam)
 {
     const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (t >= (int64_t)n * dim)
@@ -168,7 +177,7 @@ __global__ void k_aa_mix(int n, int dim, int xs, double *__restrict__ x, AaPtrs 
 #pragma unroll
     for (int j = 0; j < AA_MAX; j++)
         if (j < cnt)
-            s += gam.g[j] * P.dC[j][t];
+            s += gam.g[j] * (double)P.dC[j][t];
     const int64_t i = t / dim;
     const int c = (int)(t - i * dim);
     x[xs * i + c] -= s;
@@ -188,7 +197,7 @@ __global__ void k_aa_plain(int n, int dim, int xs, double *__restrict__ x, const
 
 // Start a new history (a new load step, or a re-solve after bond breakage: the
 // tangent, and with it the map, has changed).
-static int aa_begin(AaHist *a, int depth)
+static int aa_begin(AaHist *a, int depth, int f32 = 0)
 {
     if (depth < 0 || depth > AA_MAX)
         return lpmx_fail(LPMX_ERR_ARG, "acceleration depth %d outside 0..%d", depth, AA_MAX);
@@ -197,11 +206,22 @@ static int aa_begin(AaHist *a, int depth)
     const int64_t m = (int64_t)g.n * g.dim;
     for (int j = 0; j < depth; j++)
     {
-        if (!a->P.dC[j])
-            HIPCHK(hipMalloc(&a->P.dC[j], sizeof(double) * m));
-        if (!a->P.dF[j])
-            HIPCHK(hipMalloc(&a->P.dF[j], sizeof(double) * m));
+        if (f32)
+        {
+            if (!a->P32.dC[j])
+                HIPCHK(hipMalloc(&a->P32.dC[j], sizeof(float) * m));
+            if (!a->P32.dF[j])
+                HIPCHK(hipMalloc(&a->P32.dF[j], sizeof(float) * m));
+        }
+        else
+        {
+            if (!a->P.dC[j])
+                HIPCHK(hipMalloc(&a->P.dC[j], sizeof(double) * m));
+            if (!a->P.dF[j])
+                HIPCHK(hipMalloc(&a->P.dF[j], sizeof(double) * m));
+        }
     }
+    a->f32 = f32;
     if (depth > 0 && !a->u_prev)
     {
         HIPCHK(hipMalloc(&a->u_prev, sizeof(double) * m));
@@ -284,8 +304,12 @@ static int aa_step(AaHist *a, double *x, int xs, const double *d, int *mixed)
     const int64_t m = (int64_t)n * dim;
     const int nb = nblocks(m);
     const int s = a->next;
-    hipLaunchKernelGGL(k_aa_push, dim3(nb), dim3(LPMX_BLOCK), 0, g_stream, n, dim, xs, x, d, a->u_prev, a->f_prev,
-                       a->P.dC[s], a->P.dF[s], a->have_prev);
+    if (a->f32)
+        hipLaunchKernelGGL(k_aa_push<float>, dim3(nb), dim3(LPMX_BLOCK), 0, g_stream, n, dim, xs, x, d, a->u_prev,
+                           a->f_prev, a->P32.dC[s], a->P32.dF[s], a->have_prev);
+    else
+        hipLaunchKernelGGL(k_aa_push<double>, dim3(nb), dim3(LPMX_BLOCK), 0, g_stream, n, dim, xs, x, d, a->u_prev,
+                           a->f_prev, a->P.dC[s], a->P.dF[s], a->have_prev);
     LAUNCHCHK();
     if (!a->have_prev)
     {
@@ -296,10 +320,20 @@ static int aa_step(AaHist *a, double *x, int xs, const double *d, int *mixed)
         a->cnt++;
     a->next = (a->next + 1) % a->depth;
     const int cnt = a->cnt;
-    AaPtrs Q = a->P; // the unused slots read slot s
-    for (int j = cnt; j < AA_MAX; j++)
-        Q.dF[j] = a->P.dF[s];
-    hipLaunchKernelGGL(k_aa_dots, dim3(AA_GRID), dim3(256), 0, g_stream, m, Q, cnt, s, d, a->parts);
+    if (a->f32)
+    {
+        AaPtrs32 Q = a->P32; // the unused slots read slot s
+        for (int j = cnt; j < AA_MAX; j++)
+            Q.dF[j] = a->P32.dF[s];
+        hipLaunchKernelGGL(k_aa_dots<float>, dim3(AA_GRID), dim3(256), 0, g_stream, m, Q, cnt, s, d, a->parts);
+    }
+    else
+    {
+        AaPtrs Q = a->P; // the unused slots read slot s
+        for (int j = cnt; j < AA_MAX; j++)
+            Q.dF[j] = a->P.dF[s];
+        hipLaunchKernelGGL(k_aa_dots<double>, dim3(AA_GRID), dim3(256), 0, g_stream, m, Q, cnt, s, d, a->parts);
+    }
     hipLaunchKernelGGL(k_aa_dots_total, dim3(2 * AA_MAX), dim3(256), 0, g_stream, a->parts, AA_GRID, cnt, a->out);
     LAUNCHCHK();
     double o[2 * AA_MAX];
@@ -316,14 +350,18 @@ static int aa_step(AaHist *a, double *x, int xs, const double *d, int *mixed)
         aa_forget(a);
         return LPMX_OK;
     }
-    hipLaunchKernelGGL(k_aa_mix, dim3(nb), dim3(LPMX_BLOCK), 0, g_stream, n, dim, xs, x, a->P, cnt, gam);
+    if (a->f32)
+        hipLaunchKernelGGL(k_aa_mix<float>, dim3(nb), dim3(LPMX_BLOCK), 0, g_stream, n, dim, xs, x, a->P32, cnt, gam);
+    else
+        hipLaunchKernelGGL(k_aa_mix<double>, dim3(nb), dim3(LPMX_BLOCK), 0, g_stream, n, dim, xs, x, a->P, cnt, gam);
     LAUNCHCHK();
     *mixed = 1;
     return LPMX_OK;
 }
 
 // --------------------------------------------------------- the Newton loop's -----
-int lpmx_accel_begin(int depth) { return aa_begin(&A, depth); }
+// precision LPMX_MIXED keeps the differences in FP32
+int lpmx_accel_begin(int depth, int precision) { return aa_begin(&A, depth, precision == LPMX_MIXED); }
 void lpmx_accel_forget(void) { aa_forget(&A); }
 // Revert xyz to the plain iterate of the last solve and forget the history.
 int lpmx_accel_revert(void) { return aa_revert(&A, g.x, 3); }

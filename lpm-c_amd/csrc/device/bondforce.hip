@@ -219,8 +219,7 @@ __global__ __launch_bounds__(256) void k_j2_particle_u(
             double dl = d - l0;
             dl -= dLp0[b];
             dl *= db[b];
-            dL[b] = dl;
-            rdl[j] = dl;
+            rdl[j] = dl; // dL of the first pass: the second pass below stores the final one
             if (nsign[b] == 0)
                 a0 += dl, b0 += tv * dl;
             else

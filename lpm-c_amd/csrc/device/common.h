@@ -93,6 +93,8 @@ int lpmx_hip_fail(hipError_t e, const char *what, const char *file, int line);
     } while (0)
 
 int lpmx_require_model(void);
+// selects and checks the gfx950 device and creates the library stream (idempotent)
+int lpmx_device_init(void);
 // updateRR's kernels (bondforce.hip), shared by lpmx_update_rr and its allocation-free form
 __global__ void k_update_rr(DV v, double *sq_res, double *sq_rea);
 __global__ void k_count_bc(const int *dispBC, int m, int *cnt);
@@ -209,7 +211,7 @@ int lpmx_vjp_device(const VjpIO &io);
 int lpmx_cp_particles(void);
 
 // Anderson acceleration of the Newton loop (accel.hip)
-int lpmx_accel_begin(int depth);
+int lpmx_accel_begin(int depth, int precision);
 int lpmx_accel_step(int *mixed);
 int lpmx_accel_revert(void);
 void lpmx_accel_forget(void);

@@ -74,7 +74,7 @@ int lpmx_run_steps_impl(const lpmx_solver *s, int nsteps, const int *nbd, const 
             RETCHK(lpmx_update_rr(&nres, &nrea, nullptr));
             const double tolm = nres > nrea ? nres : nrea;
             int ni = 0;
-            RETCHK(lpmx_accel_begin(s->accel_depth));
+            RETCHK(lpmx_accel_begin(s->accel_depth, s->precision));
             double nprev = nres;
             double ring[RING];
             int since = 0;

@@ -86,15 +86,19 @@ def test_nonlinear_j2_law_matches_its_oracle(oracle):
     eps = 6e-3 * np.array([[1.0, 0.3, 0.0], [0.3, -0.6, 0.2], [0.0, 0.2, -0.4]])
     x = x0 + x0 @ eps.T + 2e-4 * rng.standard_normal(x0.shape)
     set_state(m, x, lp, alpha, beq, 30.0 * beta)
-    P.check(L.lpmx_bond_force_range(5, 0, n), "plmode 5")
+    # computeBondForceGeneral(5): the law, the stress, then switchStateV(2), which
+    # restores slot 0 from slot 2 (the law advanced slot 0 in place and writes no slot 2)
+    P.check(L.lpmx_bond_force_ex(5, 1), "plmode 5")
     oracle.lib().orc_bond_force_j2nonlinear(m.p, C_kin, 1)
     dlam = get("J2_DLAMBDA", (n,))
     assert 0 < (dlam > 0).sum() < n, "some particles yield, some do not"
     # the hardening law calls exp: its values agree to rounding, the bisection's
     # decisions (and so every multiplier) exactly
     np.testing.assert_array_equal(dlam, m.dlambda)
-    for name, ref in (("F", m.F), ("DL", m.dL), ("PIN", m.Pin.reshape(n, 3)), ("DLP0", m.dLp[:, :, 0]),
-                      ("J2_ALPHA0", m.alpha[:, 0]), ("J2_BETA0", m.beta[:, :, 0])):
+    assert np.abs(get("DDLP", (n, 18))).max() > 0, "plastic stretch increments"
+    for name, ref in (("F", m.F), ("DL", m.dL), ("PIN", m.Pin.reshape(n, 3)), ("DDLP", m.ddLp),
+                      ("STRESS", m.stress), ("DLP0", m.dLp[:, :, 0]), ("J2_ALPHA0", m.alpha[:, 0]),
+                      ("J2_BETA0", m.beta[:, :, 0])):
         assert_rel(name, get(name, ref.shape), ref, 1e-13)
 
 

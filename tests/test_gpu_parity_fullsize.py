@@ -162,3 +162,28 @@ def test_fcc_256k_single_slip_return_map(oracle):
     g0 = np.median(ref)
     assert np.abs(ref - g0).max() <= 1e-8 * g0
     assert np.abs(slip - g0).max() <= 1e-8 * g0, (slip.min(), slip.max(), g0)
+
+
+@pytest.mark.skip(reason="does not converge yet: on the 256,000-particle FCC crystal the first 0.05% [001] step "
+                         "(eight tied slip systems activate) brings the residual from 1.5e5 to 0.53 in 100 Newton "
+                         "iterations with one AMG V-cycle each, then stalls between 2e-2 and 1e-1 for 300 more "
+                         "(0.075 with a PCG solve per correction after 100); a 1e-10 tolerance needs 1.5e-5.  The "
+                         "elastic tangent of the reference contracts the slip modes by only ~1 - h0/mu per "
+                         "iteration, and the return map's slip Newton stops at TOLITER, which leaves a residual "
+                         "floor near 1e-12 of the reaction on small crystals")
+def test_fcc_256k_001_tension_converges():
+    """FCC 40^3 cells (256,000 particles), crystal plasticity with CRSS 10 MPa and
+    hardening 100 MPa, tension along [001] to 3% in 60 steps: the first steps past
+    first slip converge with the fused loop (one V-cycle per iteration, Anderson 32)
+    in both precision forms, with the eight tied systems active."""
+    n, x, t, c = P.build_fcc_cp_model(55.907, CP)
+    assert n == 256_000
+    height = float(x[:, 2].max() - x[:, 2].min())
+    steps = P.sc_uniaxial_steps(height, np.arange(1, 61) * (0.03 / 60))
+    for prec in (P.LPMX_MIXED, P.LPMX_FP64):
+        P.build_fcc_cp_model(55.907, CP)
+        s = P.solver(precision=prec, rtol2=1e-8, atol2=0.0, maxit=300 * n, newton_tol=1e-10, newton_maxit=1000,
+                     accel_depth=32, precond=2)
+        rc, out = P.run_steps(steps[:3], s, out_axis="z", out_type=1)
+        assert rc == 0, [(o.status, o.newton_iters, o.res_norm / o.tol_mult) for o in out]
+        assert get("CP_JACT", (n, 24)).sum(axis=1).max() >= 8
