@@ -72,6 +72,27 @@ int lpmx_set_materials(int ntype, const int *type, const double *Ce, const doubl
 /* Optional per-bond critical stretch [n][nneighbors] of the brittle law (NULL: global). */
 int lpmx_set_bond_critical(const double *crit);
 
+/* createCuboid's candidate grid (reference src/initialization.c:6-560): rows of points
+ * along x, row r = (layer k - 1) * rows + (j - 1), each candidate rotated by R
+ * (row-major, p = R t) and kept when inside the closed box (x, y; z too in 3D). */
+typedef struct lpmx_lattice_spec
+{
+    int lattice, dim;          /* 0 square, 1 hexagon (2D); 2 SC, 3 FCC, 4 BCC (3D) */
+    int pfr, rows, layers;     /* particles_first_row, rows, layers (1 in 2D) */
+    double hx, hy, hz, radius; /* spacings and the particle radius */
+    double bt[6];              /* candidate range: -a, a per axis, a the box diagonal */
+    double box[6];             /* the closed box that keeps a point */
+    double R[9];               /* R_matrix */
+} lpmx_lattice_spec;
+/* The kept points of createCuboid's lattice, generated on the device (needs no model):
+ * *count = their number; with xyz != NULL and cap >= *count also their positions
+ * [count][3] in the reference's generation order (z = 0 in 2D). */
+int lpmx_lattice_points(const lpmx_lattice_spec *spec, double *xyz, int cap, int *count);
+/* computeStrain (reference src/lpm_basic.c:127-249) on the device from the model's dL:
+ * strain [n][6] (xx yy zz yz xz xy), in and out -- components the reference leaves
+ * alone keep their values. */
+int lpmx_compute_strain(double *strain);
+
 /* ---- field transfer (reference layouts) ------------------------------------------ */
 enum lpmx_field
 {

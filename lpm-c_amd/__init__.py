@@ -48,6 +48,13 @@ class Params(C.Structure):
                 ("nbreak", C.c_int), ("J2_C", C.c_double)]
 
 
+class LatticeSpec(C.Structure):
+    """lpmx_lattice_spec: createCuboid's candidate grid."""
+    _fields_ = [("lattice", C.c_int), ("dim", C.c_int), ("pfr", C.c_int), ("rows", C.c_int), ("layers", C.c_int),
+                ("hx", C.c_double), ("hy", C.c_double), ("hz", C.c_double), ("radius", C.c_double),
+                ("bt", C.c_double * 6), ("box", C.c_double * 6), ("R", C.c_double * 9)]
+
+
 class DispBC(C.Structure):
     _fields_ = [("type", C.c_int), ("flag", C.c_char), ("step", C.c_double)]
 
@@ -133,6 +140,7 @@ def lib():
         "lpmx_bench_spmv": [C.c_int, C.c_int, _dp, _dp],
         "lpmx_spmv_bytes": [C.c_int, _dp],
         "lpmx_compute_cab": [],
+        "lpmx_lattice_points": [C.POINTER(LatticeSpec), _dp, C.c_int, _ip], "lpmx_compute_strain": [_dp],
         "lpmx_vjp": [_dp] * 11,
         "lpmx_history_forward": [C.POINTER(Solver), C.c_int, C.c_int, _ip, C.POINTER(DispBC), _ip,
                                  C.POINTER(ForceBC), C.c_int, C.c_char, C.c_int, _dp, C.POINTER(StepStats)],

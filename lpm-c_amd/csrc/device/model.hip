@@ -71,6 +71,8 @@ static int init_device(void)
     return LPMX_OK;
 }
 
+int lpmx_device_init(void) { return init_device(); }
+
 unsigned *lpmx_counters(void) { return g_counters; }
 
 double *lpmx_scratch(int64_t bytes)

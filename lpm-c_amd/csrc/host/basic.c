@@ -297,111 +297,6 @@ double cblas_dnrm2(const int n, const double *x, const int incx)
 void mkl_free_buffers(void) {}
 
 /* ------------------------------------------------ output-only post-processing --- */
-/* Small dense solve with partial pivoting (the role LAPACKE_dgesv plays in
- * computeStrain); returns 0 or the 1-based index of a zero pivot. */
-static int gauss_solve(int n, double *A, double *b)
-{
-    for (int k = 0; k < n; k++)
-    {
-        int p = k;
-        for (int i = k + 1; i < n; i++)
-            if (fabs(A[i * n + k]) > fabs(A[p * n + k]))
-                p = i;
-        if (A[p * n + k] == 0.0)
-            return k + 1;
-        if (p != k)
-        {
-            for (int j = 0; j < n; j++)
-            {
-                double t = A[k * n + j];
-                A[k * n + j] = A[p * n + j];
-                A[p * n + j] = t;
-            }
-            double t = b[k];
-            b[k] = b[p];
-            b[p] = t;
-        }
-        for (int i = k + 1; i < n; i++)
-        {
-            double f = A[i * n + k] / A[k * n + k];
-            for (int j = k; j < n; j++)
-                A[i * n + j] -= f * A[k * n + j];
-            b[i] -= f * b[k];
-        }
-    }
-    for (int k = n - 1; k >= 0; k--)
-    {
-        double s = b[k];
-        for (int j = k + 1; j < n; j++)
-            s -= A[k * n + j] * b[j];
-        b[k] = s / A[k * n + k];
-    }
-    return 0;
-}
-
-/* computeStrain (reference src/lpm_basic.c:127-249): weighted least-squares strain
- * from the bond stretches, weights 0.1 (first shell) and 0.9 (second shell).
- * Output only, on the host copy of dL. */
-void computeStrain()
-{
-    const int d = dim, ns = 3 * (dim - 1);
-    for (int i = 0; i < nparticle; i++)
-    {
-        double r4[3][3][3][3] = {{{{0.}}}}, r2[3][3] = {{0.}};
-        for (int j = 0; j < nb_initial[i]; j++)
-        {
-            double w = nsign[i][j] == 0 ? 0.1 : 0.9, e[3];
-            int q = neighbors[i][j];
-            for (int k = 0; k < 3; k++)
-                e[k] = (xyz_initial[q][k] - xyz_initial[i][k]) / distance_initial[i][j];
-            for (int k = 0; k < d; k++)
-                for (int n = 0; n < d; n++)
-                {
-                    for (int m = 0; m < d; m++)
-                        for (int l = 0; l < d; l++)
-                            r4[k][n][m][l] += w * e[k] * e[n] * e[m] * e[l];
-                    r2[k][n] += w * dL[i][j] / distance_initial[i][j] * e[k] * e[n];
-                }
-        }
-        double A[36], b[6];
-        int ii = 0;
-        for (int j = 0; j < d; j++)
-            for (int k = 0; k < d; k++)
-                for (int m = 0; m < d; m++)
-                    for (int l = 0; l < d; l++)
-                        if (m <= l && j <= k)
-                            A[ii++] = r4[m][l][j][k];
-        ii = 0;
-        for (int j = 0; j < d; j++)
-            for (int k = 0; k < d; k++)
-                if (j <= k)
-                    b[ii++] = r2[j][k];
-        int info = gauss_solve(ns, A, b);
-        if (info > 0)
-        {
-            if (i > 0)
-                for (int k = 0; k < ns; k++)
-                    strain_tensor[i][k] = strain_tensor[i - 1][k];
-            continue;
-        }
-        if (d == 2)
-        {
-            strain_tensor[i][0] = b[0];
-            strain_tensor[i][5] = b[1];
-            strain_tensor[i][1] = b[2];
-        }
-        else
-        {
-            strain_tensor[i][0] = b[0];
-            strain_tensor[i][5] = b[1];
-            strain_tensor[i][4] = b[2];
-            strain_tensor[i][1] = b[3];
-            strain_tensor[i][3] = b[4];
-            strain_tensor[i][2] = b[5];
-        }
-    }
-}
-
 /* computeBondForce / computeBondStretch (reference src/lpm_basic.c:8-50): bond
  * values re-ordered by the lattice's reference bond vectors.  Output only. */
 static void reorder_bonds(double **dst, int stretch)

@@ -673,6 +673,23 @@ void computedL()
     PULL(LPMX_F_DISTANCE, LPMX_F_DL, LPMX_F_DL_TOTAL, LPMX_F_TDL_TOTAL, LPMX_F_CSX, LPMX_F_CSY, LPMX_F_CSZ);
 }
 
+/* computeStrain (reference src/lpm_basic.c:127-249): the weighted least-squares
+ * strain of every particle, on the device from the bond stretches dL. */
+void computeStrain()
+{
+    need_model("computeStrain");
+    PUSH(LPMX_F_DL);
+    double *s = malloc(sizeof(double) * 6 * (size_t)nparticle);
+    for (int i = 0; i < nparticle; i++)
+        for (int c = 0; c < 6; c++)
+            s[6 * i + c] = strain_tensor[i][c];
+    CHK("computeStrain", lpmx_compute_strain(s));
+    for (int i = 0; i < nparticle; i++)
+        for (int c = 0; c < 6; c++)
+            strain_tensor[i][c] = s[6 * i + c];
+    free(s);
+}
+
 /* updateRR (reference src/stiffness.c:519-534). */
 void updateRR()
 {

@@ -1,10 +1,10 @@
 /*
- * One-time host setup of the drop-in API: lattice generation, geometry cuts, crack
- * definition and the allocation of every per-particle array
- * (reference src/initialization.c).  The particle positions must come out bit for
- * bit as the reference's, since every later list and force depends on them, so the
- * lattice rules (candidate grid over [-a, a], a the box diagonal, generation order,
- * closed box test) are kept; the candidate grid is streamed twice instead of stored.
+ * Setup of the drop-in API: the lattice description of createCuboid, whose points
+ * the device generates (lpmx_lattice_points), geometry cuts, crack definition and
+ * the allocation of every per-particle array (reference src/initialization.c).  The
+ * particle positions must come out bit for bit as the reference's, since every later
+ * list and force depends on them, so the lattice rules (candidate grid over [-a, a],
+ * a the box diagonal, generation order, closed box test) are kept.
  */
 #include <math.h>
 #include <stdio.h>
@@ -60,95 +60,6 @@ static void rotate(const double *t, double *p)
 {
     for (int r = 0; r < 3; r++)
         p[r] = R_matrix[3 * r] * t[0] + R_matrix[3 * r + 1] * t[1] + R_matrix[3 * r + 2] * t[2];
-}
-
-/* One candidate: rotate, keep it if inside the closed box; pass 1 writes it. */
-struct gen
-{
-    int pass, n;
-};
-
-static void candidate(struct gen *G, double x, double y, double z)
-{
-    double t[3] = {x, y, z}, p[3];
-    rotate(t, p);
-    if (!(p[0] >= box[0] && p[0] <= box[1] && p[1] >= box[2] && p[1] <= box[3]))
-        return;
-    if (dim == 3 && !(p[2] >= box[4] && p[2] <= box[5]))
-        return;
-    if (G->pass)
-    {
-        xyz[G->n][0] = p[0];
-        xyz[G->n][1] = p[1];
-        xyz[G->n][2] = dim == 3 ? p[2] : 0.0;
-    }
-    G->n++;
-}
-
-static void lattice_points(struct gen *G, const double *bt)
-{
-    if (lattice == 0)
-    {
-        for (int j = 1; j <= rows; j++)
-            for (int i = 1; i <= particles_first_row; i++)
-                candidate(G, bt[0] + hx * (i - 1) + radius, bt[2] + hy * (j - 1) + radius, 0.0);
-    }
-    else if (lattice == 1)
-    {
-        for (int j = 1; j <= rows; j++)
-        {
-            double y = bt[2] + hy * (j - 1);
-            if (j % 2 == 1)
-                for (int i = 1; i <= particles_first_row; i++)
-                    candidate(G, bt[0] + hx * (i - 1), y, 0.0);
-            else
-                for (int i = 1; i <= particles_first_row - 1; i++)
-                    candidate(G, bt[0] + hx * (2 * i - 1) / 2.0, y, 0.0);
-        }
-    }
-    else if (lattice == 2)
-    {
-        for (int k = 1; k <= layers; k++)
-            for (int j = 1; j <= rows; j++)
-                for (int i = 1; i <= particles_first_row; i++)
-                    candidate(G, box[0] + hx * (i - 1), bt[2] + hy * (j - 1), bt[4] + hz * (k - 1));
-    }
-    else if (lattice == 3)
-    {
-        for (int k = 1; k <= layers; k++)
-        {
-            double z = bt[4] + hz / 2.0 * (k - 1);
-            for (int j = 1; j <= rows; j++)
-            {
-                double y = bt[2] + hy / 2.0 * (j - 1);
-                if ((k % 2 == 1) == (j % 2 == 1))
-                    for (int i = 1; i <= particles_first_row; i++)
-                        candidate(G, bt[0] + hx * (i - 1), y, z);
-                else
-                    for (int i = 1; i <= particles_first_row - 1; i++)
-                        candidate(G, bt[0] + hx * (i - 1) + hx / 2.0, y, z);
-            }
-        }
-    }
-    else
-    {
-        for (int k = 1; k <= layers; k++)
-        {
-            double z = bt[4] + hz / 2.0 * (k - 1);
-            if (k % 2 == 1)
-            {
-                for (int j = 1; j <= rows; j++)
-                    for (int i = 1; i <= particles_first_row; i++)
-                        candidate(G, bt[0] + hx * (i - 1), bt[2] + hy * (j - 1), z);
-            }
-            else
-            {
-                for (int j = 1; j <= rows - 1; j++)
-                    for (int i = 1; i <= particles_first_row - 1; i++)
-                        candidate(G, bt[0] + hx * (i - 1) + hx / 2.0, bt[2] + hy * (j - 1) + hy / 2.0, z);
-            }
-        }
-    }
 }
 
 /* createCuboid (reference src/initialization.c:6-560). */
@@ -208,12 +119,27 @@ void createCuboid()
         particle_volume = 32.0 * sqrt(3.0) / 9.0 * pow(radius, 3);
         break;
     }
-    struct gen G = {0, 0};
-    lattice_points(&G, bt);
-    nparticle = G.n;
+    lpmx_lattice_spec sp;
+    memset(&sp, 0, sizeof(sp));
+    sp.lattice = lattice, sp.dim = dim, sp.pfr = particles_first_row, sp.rows = rows;
+    sp.layers = lattice >= 2 ? layers : 1;
+    sp.hx = hx, sp.hy = hy, sp.hz = hz, sp.radius = radius;
+    for (int c = 0; c < 6; c++)
+        sp.bt[c] = bt[c], sp.box[c] = box[c];
+    memcpy(sp.R, R_matrix, sizeof(sp.R));
+    int count = 0, rc = lpmx_lattice_points(&sp, NULL, 0, &count);
+    if (rc != LPMX_OK)
+        lpm_dropin_fatal("createCuboid", rc);
+    nparticle = count;
     xyz = allocDouble2D(nparticle, NDIM, 0.0);
-    G.pass = 1, G.n = 0;
-    lattice_points(&G, bt);
+    double *pts = malloc(sizeof(double) * 3 * (size_t)(count > 0 ? count : 1));
+    rc = lpmx_lattice_points(&sp, pts, count, &count);
+    if (rc != LPMX_OK)
+        lpm_dropin_fatal("createCuboid", rc);
+    for (int i = 0; i < nparticle; i++)
+        for (int c = 0; c < 3; c++)
+            xyz[i][c] = pts[3 * i + c];
+    free(pts);
     if (lattice == 2)
     {
         /* the 18 reference bond directions of the SC lattice, rotated */

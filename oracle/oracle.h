@@ -41,6 +41,9 @@
  *   orc_run_steps           Anderson-accelerated and plain loops reach one equilibrium
  *   orc_vjp_j2              central differences of orc_eval_j2 along random directions
  *   orc_adjoint_run         central differences of the whole forward history
+ *   orc_compute_strain      exact recovery of a strain from stretches linear in it;
+ *                           weighted least squares by numpy lstsq; the stored
+ *                           right-hand side of a collinear chain's singular system
  * MKL's internal iterate sequence (dcg) is closed source: parity of solverCG is
  * checked on converged solutions only ("parity unpinned" for iterates).
  */
@@ -237,6 +240,7 @@ void orc_bond_force_j2energy(orc_model *m, int t, int jacobi);
  * it, as in the reference. */
 void orc_bond_force_j2nonlinear(orc_model *m, double C, int jacobi);
 void orc_compute_stress(orc_model *m);
+void orc_compute_strain(const orc_model *m, double *strain);
 void orc_switch_state(orc_model *m, int flag);
 void orc_bond_force_general(orc_model *m, int mode);
 void orc_internal_force(orc_model *m, const double *x, double *pin);

@@ -707,6 +707,110 @@ void orc_compute_stress(orc_model *m)
     }
 }
 
+/* Gaussian elimination with partial pivoting (the first largest pivot in a column)
+ * on the dense n x n row-major system A x = b: the algorithm of LAPACK's dgesv, which
+ * computeStrain calls.  Returns 0, or the 1-based column of an exactly zero pivot,
+ * in which case b is left as given (dgesv does not solve a singular system). */
+static int orc_gauss(int n, double *A, double *b)
+{
+    double b0[6];
+    for (int k = 0; k < n; k++)
+        b0[k] = b[k];
+    for (int k = 0; k < n; k++)
+    {
+        int p = k;
+        for (int i = k + 1; i < n; i++)
+            if (fabs(A[i * n + k]) > fabs(A[p * n + k]))
+                p = i;
+        if (A[p * n + k] == 0.0)
+        {
+            for (int c = 0; c < n; c++)
+                b[c] = b0[c];
+            return k + 1;
+        }
+        if (p != k)
+        {
+            for (int j = 0; j < n; j++)
+            {
+                double t = A[k * n + j];
+                A[k * n + j] = A[p * n + j];
+                A[p * n + j] = t;
+            }
+            double t = b[k];
+            b[k] = b[p];
+            b[p] = t;
+        }
+        for (int i = k + 1; i < n; i++)
+        {
+            double f = A[i * n + k] / A[k * n + k];
+            for (int j = k; j < n; j++)
+                A[i * n + j] -= f * A[k * n + j];
+            b[i] -= f * b[k];
+        }
+    }
+    for (int k = n - 1; k >= 0; k--)
+    {
+        double s = b[k];
+        for (int j = k + 1; j < n; j++)
+            s -= A[k * n + j] * b[j];
+        b[k] = s / A[k * n + k];
+    }
+    return 0;
+}
+
+/* computeStrain (reference src/lpm_basic.c:127-249): per particle, the fourth- and
+ * second-order bond sums with weights 0.1 (first shell) and 0.9 (second shell), the
+ * normal equations of the weighted least-squares strain in the reference's packing
+ * (:196-217), a solve, and the reference's component order (:236-246).  A singular
+ * system stores its unsolved right-hand side, except at particle 0, which keeps its
+ * previous strain (:228-234 followed by :236-246).  strain: [n][6], in and out. */
+void orc_compute_strain(const orc_model *m, double *strain)
+{
+    const int d = m->dim, ns = 3 * (d - 1);
+    for (int i = 0; i < m->n; i++)
+    {
+        double r4[3][3][3][3] = {{{{0.0}}}}, r2[3][3] = {{0.0}};
+        for (int j = 0; j < m->nb_initial[i]; j++)
+        {
+            int64_t b = B(m, i, j);
+            const int s = m->nsign[b];
+            if (s != 0 && s != 1)
+                continue;
+            const double w = s == 0 ? 0.1 : 0.9, L = m->L0[b];
+            const double *xq = m->x0 + 3 * m->nbr[b], *xi = m->x0 + 3 * i;
+            for (int k = 0; k < d; k++)
+                for (int n = 0; n < d; n++)
+                    for (int a = 0; a < d; a++)
+                        for (int l = 0; l < d; l++)
+                            r4[k][n][a][l] += w * (xq[k] - xi[k]) / L * (xq[n] - xi[n]) / L * (xq[a] - xi[a]) / L *
+                                              (xq[l] - xi[l]) / L;
+            for (int k = 0; k < d; k++)
+                for (int n = 0; n < d; n++)
+                    r2[k][n] += w * m->dL[b] / L * (xq[k] - xi[k]) / L * (xq[n] - xi[n]) / L;
+        }
+        double A[36], rhs[6];
+        int ii = 0;
+        for (int j = 0; j < d; j++)
+            for (int k = 0; k < d; k++)
+                for (int a = 0; a < d; a++)
+                    for (int l = 0; l < d; l++)
+                        if (a <= l && j <= k)
+                            A[ii++] = r4[a][l][j][k];
+        ii = 0;
+        for (int j = 0; j < d; j++)
+            for (int k = 0; k < d; k++)
+                if (j <= k)
+                    rhs[ii++] = r2[j][k];
+        if (orc_gauss(ns, A, rhs) > 0 && i == 0)
+            continue;
+        double *st = strain + 6 * i;
+        if (d == 2)
+            st[0] = rhs[0], st[5] = rhs[1], st[1] = rhs[2];
+        else
+            st[0] = rhs[0], st[5] = rhs[1], st[4] = rhs[2], st[1] = rhs[3], st[3] = rhs[4], st[2] = rhs[5];
+    }
+}
+
 /* switchStateV (reference src/constitutive.c:10-85); crystal-plasticity slots are
  * not part of the oracle's laws. */
 void orc_switch_state(orc_model *m, int flag)

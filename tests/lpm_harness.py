@@ -34,6 +34,45 @@ def sc_pair(O, k, E=200e3, nu=0.3, sigmay=200.0, H=2000.0, emod=None):
     return L, n, x, t, m
 
 
+def host_lattice(O, lattice, radius, box, x=None):
+    """The host state that createCuboid and initMatrices leave (lattice globals,
+    positions, per-particle arrays), with the lattice description from the oracle's
+    restatement of createCuboid and its positions (or the given ones, x [n][3]): for
+    CPU tests of host code, since the product generates the points on the device, and
+    for models on positions no lattice rule makes."""
+    import ctypes as C
+    L = P.lib()
+    lp = O.lattice_params(lattice, radius, box[5] - box[4])
+    if x is None:
+        x = O.create_lattice(lattice, radius, box)
+    n = x.shape[0]
+    P.gint("lattice").value = lattice
+    P.gint("dim").value = lp.dim
+    P.gdouble("radius").value = radius
+    b = (C.c_double * 6).in_dll(L, "box")
+    for k in range(6):
+        b[k] = box[k]
+    P.gdouble("box_x").value = box[1] - box[0]
+    P.gdouble("box_y").value = box[3] - box[2]
+    P.gdouble("box_z").value = box[5] - box[4]
+    P.gint("nneighbors").value = lp.nbmax
+    P.gint("nneighbors1").value = lp.nb1max
+    P.gint("nneighbors2").value = lp.nb2max
+    P.gdouble("particle_volume").value = lp.volume
+    P.gdouble("neighbor1_cutoff").value = lp.cut1
+    P.gdouble("neighbor2_cutoff").value = lp.cut2
+    P.gint("nneighbors_AFEM").value = lp.nafem
+    P.gint("nparticle").value = n
+    P.set_gptr("xyz", L.allocDouble2D(n, 3, 0.0), C.c_double, 2)
+    xyz = P.gptr("xyz", 2)
+    for i in range(n):
+        for c in range(3):
+            xyz[i][c] = x[i, c]
+    L.initMatrices()
+    L.copyDouble2D(P.gptr("xyz_initial", 2), xyz, n, 3)
+    return n, x
+
+
 def push(name, arr):
     a = np.ascontiguousarray(arr)
     P.check(P.lib().lpmx_put(P.F[name], a.ctypes.data, a.size), name)

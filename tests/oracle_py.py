@@ -100,6 +100,8 @@ def lib():
             getattr(L, f).restype = None
         L.orc_eval_j2.argtypes = [P]
         L.orc_eval_j2.restype = None
+        L.orc_compute_strain.argtypes = [P, _dp]
+        L.orc_compute_strain.restype = None
         L.orc_vjp_j2.argtypes = [P, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
         L.orc_vjp_j2.restype = None
         L.orc_adjoint_run.argtypes = [P, C.POINTER(OrcSolver), C.c_int, _ip, C.POINTER(OrcDispBC), _ip,
@@ -304,6 +306,12 @@ class Model:
 
     def call(self, name, *args):
         return getattr(lib(), name)(self.p, *args)
+
+    def compute_strain(self, strain=None):
+        """computeStrain from the model's dL; strain [n][6] in and out (zeros if None)."""
+        st = np.zeros((self.n, 6)) if strain is None else np.array(strain, dtype=np.float64, order="C")
+        lib().orc_compute_strain(self.p, ptr(st))
+        return st
 
 
 def bc_tables(steps):

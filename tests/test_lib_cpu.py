@@ -1,6 +1,6 @@
 """CPU checks of liblpm.so: it loads, exports every entry point its headers declare,
-its host setup reproduces the reference lattice rules bit for bit, and its device
-entry points fail loudly when no GPU is present."""
+its host writers produce the reference's layouts, and its device entry points
+(lattice generation included) fail loudly when no GPU is present."""
 import ctypes as C
 import os
 import re
@@ -10,7 +10,7 @@ import sys
 import numpy as np
 import pytest
 
-from lpm_harness import P, ROOT
+from lpm_harness import P, ROOT, host_lattice
 
 HEADERS = ["lpm.h", "boundary.h", "constitutive.h", "data_handler.h", "initialization.h", "lpm_basic.h",
            "neighbor.h", "solver.h", "stiffness.h", "lpmx.h", "lpm_ext.h"]
@@ -39,21 +39,6 @@ def test_library_exports_every_declared_function():
 def test_globals_are_exported():
     for g in ("nparticle", "xyz", "K_global", "dLp", "J2_beta", "damage_D", "box", "R_matrix", "pbc"):
         C.c_int.in_dll(P.lib(), g)
-
-
-@pytest.mark.parametrize("lat,box", [(2, [0, 9.151, 0, 9.151, 0, 9.151]), (1, [0, 20.0, 0, 17.0, 0, 1.0]),
-                                     (3, [0, 8.0, 0, 8.0, 0, 8.0]), (0, [0, 10.0, 0, 8.0, 0, 1.0]),
-                                     (4, [0, 6.0, 0, 6.0, 0, 6.0])])
-def test_create_cuboid_matches_oracle(oracle, lat, box):
-    n = P.make_lattice(lat, 0.5, box)
-    x = P.positions()
-    xo = oracle.create_lattice(lat, 0.5, box)
-    assert n == xo.shape[0] > 0
-    np.testing.assert_array_equal(x, xo)
-    lp = oracle.lattice_params(lat, 0.5, box[5] - box[4])
-    assert P.gint("nneighbors").value == lp.nbmax
-    assert P.gdouble("particle_volume").value == lp.volume
-    assert P.gdouble("neighbor2_cutoff").value == lp.cut2
 
 
 def test_sc_box_sizes_give_the_configured_counts(oracle):
@@ -88,6 +73,9 @@ def test_device_calls_fail_loudly_without_gpu():
     x = np.zeros((8, 3))
     assert P.lib().lpmx_model_create(C.byref(desc), P.ptr(x)) == -5
     assert b"device" in P.lib().lpmx_last_error()
+    spec = P.LatticeSpec(lattice=2, dim=3, pfr=2, rows=2, layers=2, hx=1.0, hy=1.0, hz=1.0, radius=0.5)
+    cnt = C.c_int(-1)
+    assert P.lib().lpmx_lattice_points(C.byref(spec), None, 0, C.byref(cnt)) == -5
     # the drop-in entry point exits with status 1, as the reference's fatal paths do
     code = ("import sys; sys.path.insert(0, %r); from lpm_harness import P; "
             "P.make_lattice(2, 0.5, [0, 5.125, 0, 5.125, 0, 5.125]); P.lib().searchNormalNeighbor()"
@@ -115,44 +103,12 @@ def test_reference_drivers_compile_and_link_unchanged(tmp_path, driver):
     assert out.exists()
 
 
-def test_compute_strain_recovers_an_affine_strain(oracle):
-    """computeStrain (host, output only) fits the strain to the bond stretches by
-    weighted least squares.  Under a small affine strain the stretch of a bond is
-    e.eps.e to first order, so every particle's fit returns eps, with engineering
-    shears, in the reference's order s11 s22 s33 s23 s13 s12."""
-    L = P.sc_box_for(6)
-    n = P.make_lattice(2, 0.5, [0, L, 0, L, 0, L])
-    x = P.positions()
-    m = oracle.Model(x, 2, 0.5, np.zeros(n, np.int32), [oracle.elastic_constants(200e3, 0.3)])
-    m.setup()
-    nb = P.gint("nneighbors").value
-    rows = {name: P.gptr(name, 2, ct) for name, ct in (("neighbors", C.c_int), ("nsign", C.c_int),
-                                                         ("distance_initial", C.c_double), ("dL", C.c_double))}
-    nbi = P.gptr("nb_initial", 1, C.c_int)
-    eps = 1e-6 * np.array([[1.0, 0.3, -0.2], [0.3, -0.5, 0.4], [-0.2, 0.4, 0.8]])
-    xd = x + x @ eps.T
-    for i in range(n):
-        nbi[i] = int(m.nb_initial[i])
-        for j in range(nb):
-            rows["neighbors"][i][j] = int(m.nbr[i, j])
-            rows["nsign"][i][j] = int(m.nsign[i, j])
-            if m.nbr[i, j] >= 0:
-                q = m.nbr[i, j]
-                rows["distance_initial"][i][j] = m.L0[i, j]
-                rows["dL"][i][j] = np.linalg.norm(xd[q] - xd[i]) - m.L0[i, j]
-    P.lib().computeStrain()
-    st = P.read2d("strain_tensor", n, 6)
-    want = np.array([eps[0, 0], eps[1, 1], eps[2, 2], 2 * eps[1, 2], 2 * eps[0, 2], 2 * eps[0, 1]])
-    np.testing.assert_allclose(st, np.broadcast_to(want, st.shape), rtol=0, atol=1e-4 * np.abs(want).max())
-
-
-def test_write_disp_reports_the_mean_displacement(tmp_path):
+def test_write_disp_reports_the_mean_displacement(oracle, tmp_path):
     """writeDisp (host, src/data_handler.c:278-306) appends "step mean-displacement" of
     one particle type along an axis: under a uniform translation every type reports
     the translation, a line per call."""
     L = P.sc_box_for(5)
-    n = P.make_lattice(2, 0.5, [0, L, 0, L, 0, L])
-    x = P.positions()
+    n, x = host_lattice(oracle, 2, 0.5, [0, L, 0, L, 0, L])
     t = P.sc_types(x)
     P.set_types(t, 5)
     u = np.array([0.125, -0.5, 0.25])
@@ -172,13 +128,12 @@ def test_write_disp_reports_the_mean_displacement(tmp_path):
     assert got == [(3, 0.25), (4, 0.125), (5, -0.5)]
 
 
-def test_write_dump_lammps_layout(tmp_path):
+def test_write_dump_lammps_layout(oracle, tmp_path):
     """writeDump (host, src/data_handler.c:42-84) writes a LAMMPS dump: the timestep,
     the particle count, the box, the column header, then one row per particle with
     its id, type, position and displacement."""
     L = P.sc_box_for(4)
-    n = P.make_lattice(2, 0.5, [0, L, 0, L, 0, L])
-    x = P.positions()
+    n, x = host_lattice(oracle, 2, 0.5, [0, L, 0, L, 0, L])
     P.set_types(P.sc_types(x), 5)
     u = np.array([0.0, 0.0, 0.03125])
     xyz = P.gptr("xyz", 2)

@@ -422,3 +422,97 @@ def test_converged_steps_are_equilibria_of_the_law(oracle):
         assert abs(o.reaction - rz[1]) <= 1e-12 * abs(rz[1])
         assert np.sign(rz[1]) == -np.sign(e), (st, rz[1])
     assert m.alpha[:, 1].max() > 0, "the path is plastic"
+
+
+# ---------------------------------------------------------------- computeStrain --
+def packed_to_reference(x, dim):
+    """Least-squares unknowns in the reference's packing (j <= k, row-major) to its
+    strain components xx yy zz yz xz xy."""
+    s = np.zeros(6)
+    if dim == 2:
+        s[0], s[5], s[1] = x
+    else:
+        s[0], s[5], s[4], s[1], s[3], s[2] = x
+    return s
+
+
+def bond_rows(m, i):
+    """Unit bond vectors (initial configuration) and weights of particle i's bonds."""
+    d = m.dim
+    e, w, b = [], [], []
+    for j in range(m.nb_initial[i]):
+        q = m.nbr[i, j]
+        u = (m.x0[q] - m.x0[i])[:d] / m.L0[i, j]
+        e.append(u)
+        w.append(0.1 if m.nsign[i, j] == 0 else 0.9)
+        b.append(j)
+    return np.array(e), np.array(w), b
+
+
+def test_compute_strain_recovers_a_stretch_field_linear_in_the_strain(oracle):
+    """computeStrain (reference src/lpm_basic.c:127-249) fits a strain to the bond
+    stretches: when every bond's stretch is exactly e.eps.e (e its unit vector) the fit
+    returns eps itself, with engineering shears, in the order xx yy zz yz xz xy, at
+    every particle whose system is regular (3D SC and 2D hexagonal lattices)."""
+    rng = np.random.default_rng(11)
+    for m, d in ((sc(oracle, 5.2)[0], 3), (hexm(oracle, 8.0, 7.0)[0], 2)):
+        a = rng.standard_normal((d, d)) * 1e-3
+        eps = 0.5 * (a + a.T)
+        for i in range(m.n):
+            e, _, bs = bond_rows(m, i)
+            for k, j in enumerate(bs):
+                m.dL[i, j] = m.L0[i, j] * (e[k] @ eps @ e[k])
+        st = oracle_strain = m.compute_strain()
+        if d == 3:
+            want = np.array([eps[0, 0], eps[1, 1], eps[2, 2], 2 * eps[1, 2], 2 * eps[0, 2], 2 * eps[0, 1]])
+        else:
+            want = np.array([eps[0, 0], eps[1, 1], 0.0, 0.0, 0.0, 2 * eps[0, 1]])
+        err = np.abs(oracle_strain - want).max() / np.abs(want).max()
+        assert err < 1e-10, (d, err)
+        assert st.shape == (m.n, 6)
+
+
+def test_compute_strain_is_the_weighted_least_squares_fit(oracle):
+    """For arbitrary stretches the result is the minimiser of
+    sum_b w_b (dL_b / L_b - sum_{m<=l} x_ml e_m e_l)^2, w = 0.1 on the first shell and
+    0.9 on the second, as numpy's least-squares solver finds it."""
+    rng = np.random.default_rng(12)
+    for m in (sc(oracle, 5.2)[0], hexm(oracle, 8.0, 7.0)[0]):
+        d = m.dim
+        live = m.nbr >= 0
+        m.dL[live] = 1e-3 * rng.standard_normal(int(live.sum())) * m.L0[live]
+        st = m.compute_strain()
+        pairs = [(a, b) for a in range(d) for b in range(d) if a <= b]
+        for i in range(m.n):
+            e, w, bs = bond_rows(m, i)
+            F = np.array([[u[a] * u[b] for a, b in pairs] for u in e])
+            y = np.array([m.dL[i, j] / m.L0[i, j] for j in bs])
+            sw = np.sqrt(w)
+            x, *_ = np.linalg.lstsq(sw[:, None] * F, sw * y, rcond=None)
+            np.testing.assert_allclose(st[i], packed_to_reference(x, d), rtol=0, atol=1e-9 * np.abs(x).max())
+
+
+def test_compute_strain_singular_system_stores_its_right_hand_side(oracle):
+    """A particle whose bonds all lie on one line has a singular system: dgesv leaves
+    the right-hand side unsolved and the reference stores it as the strain (its copy
+    of the previous particle's strain is overwritten), except at particle 0, which
+    keeps what it had.  A chain of three particles along x, placed first, ahead of a
+    cubic block."""
+    blk = oracle.create_lattice(2, 0.5, [0, 4.2, 0, 4.2, 0, 4.2])
+    chain = np.array([[100.0, 100.0, 100.0], [101.0, 100.0, 100.0], [102.0, 100.0, 100.0]])
+    x = np.vstack([chain, blk])
+    m = oracle.Model(x, 2, 0.5, np.zeros(x.shape[0], np.int32), [oracle.elastic_constants(E0, NU0)])
+    m.setup()
+    assert list(m.nb_initial[:3]) == [1, 2, 1]
+    rng = np.random.default_rng(13)
+    live = m.nbr >= 0
+    m.dL[live] = 1e-3 * rng.standard_normal(int(live.sum()))
+    init = np.full((m.n, 6), 7.0)
+    st = m.compute_strain(init)
+    np.testing.assert_array_equal(st[0], init[0])
+    for i in (1, 2):
+        rhs = sum(0.1 * m.dL[i, j] / m.L0[i, j] * ((m.x0[m.nbr[i, j], 0] - m.x0[i, 0]) / m.L0[i, j]) ** 2
+                  for j in range(m.nb_initial[i]))
+        np.testing.assert_allclose(st[i], [rhs, 0, 0, 0, 0, 0], rtol=1e-15, atol=0)
+    # the block's particles are regular and fitted
+    assert np.all(np.abs(st[3:] - 7.0) > 1.0)
