@@ -125,7 +125,13 @@ int main(int argc, char **argv)
             return 4;
         }
         computeStrain();
+        double smax = 0.0;
+        for (int p = 0; p < nparticle; p++)
+            for (int j = 0; j < nb_initial[p]; j++)
+                if (dL[p][j] / distance_initial[p][j] > smax)
+                    smax = dL[p][j] / distance_initial[p][j];
         int broke = updateDamageGeneral(bondFile, i + 1, plmode);
+        printf("step %d: Newton %d, largest bond stretch %.6e, candidates %d\n", i + 1, ni, smax, broke);
         updateCrack();
         switchStateV(1);
         writeDisp(dispFile, 'x', 1, i + 1);
