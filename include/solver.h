@@ -8,7 +8,7 @@ extern "C" {
 #endif
 
 void solverPARDISO();
-/* K_global * disp = residual, then xyz += disp (reference src/solver.c:329-411). */
+/* K_global * disp = residual, then xyz += disp (reference src/solver.c:188-270). */
 void solverCG();
 
 #ifdef __cplusplus

@@ -473,7 +473,7 @@ __global__ void k_add_f32(int n, double *x, const float *e)
         x[t] += (double)e[t];
 }
 
-// xyz += disp (solver.c:403-408), disp is dim-strided.
+// xyz += disp (solver.c:262-268), disp is dim-strided.
 __global__ void k_apply_disp(int n, int dim, double *xyz, const double *disp)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;

@@ -484,7 +484,7 @@ static void solve(double rtol2, double atol2, int announce)
     PULL(LPMX_F_DISP, LPMX_F_XYZ);
 }
 
-/* solverCG (reference src/solver.c:329-411): the dcg residual test with the
+/* solverCG (reference src/solver.c:188-270): the dcg residual test with the
  * reference's dpar[0] = 1e-8, dpar[1] = 1e-12 on squared norms, cap dim*nparticle;
  * block-Jacobi PCG on the device instead of unpreconditioned CG. */
 void solverCG()

@@ -19,7 +19,7 @@ static inline int64_t B(const orc_model *m, int i, int j) { return (int64_t)i * 
 
 /* y = K x for the symmetric matrix stored as the reference's 1-based upper CSR
  * (the operation of mkl_sparse_d_mv with SPARSE_MATRIX_TYPE_SYMMETRIC/UPPER,
- * src/solver.c:336-347,384). */
+ * src/solver.c:195-206,243). */
 void orc_spmv_upper(const orc_model *m, const double *x, double *y)
 {
     const int nr = m->dim * m->n;
@@ -77,7 +77,7 @@ static void spmv_upper_par(const orc_model *m, const double *x, double *y, doubl
 #endif
 }
 
-/* solverCG (reference src/solver.c:329-411): unpreconditioned conjugate gradients
+/* solverCG (reference src/solver.c:188-270): unpreconditioned conjugate gradients
  * from disp = 0, stopping when |r|^2 <= rtol2*|r0|^2 + atol2 (MKL dcg's residual
  * test with dpar[0]=rtol2, dpar[1]=atol2 on squared norms) or after maxit
  * iterations; then xyz += disp, also on failure.  Returns the iteration count,
@@ -134,7 +134,7 @@ int orc_solve_cg(orc_model *m, const orc_solver *s, double *final_rr)
     return conv ? it : -it;
 }
 
-/* CPU timing baseline of the reference's linear solve (solverCG, src/solver.c:329-411:
+/* CPU timing baseline of the reference's linear solve (solverCG, src/solver.c:188-270:
  * unpreconditioned CG, the rate-setting part of a load step) on host cores.  The
  * symmetric upper CSR is expanded once to full CSR (untimed), as a threaded
  * symmetric SpMV would otherwise need per-thread scatter buffers; then `iters` CG

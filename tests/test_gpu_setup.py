@@ -94,3 +94,39 @@ def test_compute_strain_matches_oracle(oracle, kind):
     if kind == "chain":
         assert np.all(got[0] == 7.0)            # a singular particle 0 keeps its strain
         assert got[1, 0] != 7.0 and np.all(got[1, 1:] == 0.0)  # the others store the rhs
+
+
+def test_solver_pardiso_gives_the_direct_solution(oracle):
+    """solverPARDISO (reference src/solver.c:3-92) factorises the constrained tangent
+    (symmetric positive definite, mtype 2) and solves K disp = residual, then moves
+    the particles by disp.  The drop-in's answer must be the direct solution: against
+    numpy's dense solve of the same upper-CSR system to 1e-10 relative, with the
+    positions moved by exactly disp; with the extension solver settings left at their
+    defaults, as a reference driver leaves them."""
+    from lpm_harness import sc_pair
+    L = P.lib()
+    L.lpm_ext_set_solver(None)
+    _, n, x, t, m = sc_pair(oracle, 6)
+    L.calcStiffness3DFiniteDifference(6)
+    dbc = np.ones(3 * n, np.int32)
+    dbc.reshape(n, 3)[t == 2, :] = 0
+    dbc.reshape(n, 3)[t == 3, :] = 0
+    dbc.reshape(n, 3)[t == 1, 2] = 0
+    P.write1d("dispBC_index", dbc)
+    rng = np.random.default_rng(7)
+    P.write1d("residual", rng.standard_normal(3 * n) * dbc)
+    L.setDispBC_stiffnessUpdate3D()
+    K = P.read1d("K_global", m.s.nK)
+    r = P.read1d("residual", 3 * n)
+    oracle.lib().orc_tangent_analytic(m.p)
+    m.dispBC[:] = dbc
+    oracle.lib().orc_bc_stiffness_update(m.p)
+    m.K[:] = K  # the device's eliminated tangent, in the oracle's upper-CSR pattern
+    want = np.linalg.solve(oracle.upper_csr_dense(m), r)
+    x0 = P.positions()
+    L.solverPARDISO()
+    d = P.read1d("disp", 3 * n)
+    err = np.abs(d - want).max() / np.abs(want).max()
+    print(f"solverPARDISO: {3 * n} DOF, max relative error {err:.2e}", flush=True)
+    assert err <= 1e-10
+    np.testing.assert_array_equal(P.positions(), x0 + d.reshape(n, 3))
