@@ -476,6 +476,8 @@ def main():
                     help="a second configuration run on the same steps, both forms (the CG form of the "
                          "mixed-vs-FP64 comparison); none to skip")
     ap.add_argument("--workload", default="loadsteps", choices=["loadsteps", "adjoint"])
+    ap.add_argument("--no-schedule", action="store_true",
+                    help="skip the rest of the 50-step schedule after the timed steps (no whole-schedule figures)")
     ap.add_argument("--ckpt-every", type=int, default=5, help="adjoint workload: checkpoint spacing")
     ap.add_argument("--adjoint-tol", type=float, default=1e-8, help="adjoint workload: per-step solve tolerance")
     ap.add_argument("--adjoint-fd", default="",
@@ -505,7 +507,8 @@ def main():
     for f in forms:
         prec = P.LPMX_FP64 if f == "fp64" else P.LPMX_MIXED
         res[f] = run_form(P, prec, a.warmup, a.steps, profile=True, accel_depth=a.accel_depth,
-                          newton_tol=a.newton_tol, newton_maxit=a.newton_maxit, solver=a.solver, rest=True)
+                          newton_tol=a.newton_tol, newton_maxit=a.newton_maxit, solver=a.solver,
+                          rest=not a.no_schedule)
         log(f"{f}: {a.steps} steps in {res[f]['wall']:.2f} s")
     main_form = "mixed" if "mixed" in res else forms[0]
     r = res[main_form]
@@ -542,6 +545,8 @@ def main():
     # first solve of step 1), every load step
     line["schedule"] = {}
     for f, rf in res.items():
+        if a.no_schedule:
+            break
         t_steps = rf["t_warm"] + rf["wall"] + rf.get("t_rest", 0.0)
         amg_s = rf["amg"]["setup_s"] if rf.get("amg") else 0.0
         line["schedule"][f] = {

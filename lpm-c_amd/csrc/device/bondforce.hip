@@ -175,15 +175,21 @@ __global__ void k_j2_particle(DV v, int first, int count)
 // separate restrict-qualified argument: the loads of all bonds can then be issued
 // ahead of the stores (the generic kernel's pointers may alias, which serialises the
 // dependent index and position loads of one bond after another).  Same arithmetic in
-// the same order; the second stretch pass keeps the distances of the first in
-// registers, runs in the plastic-stretch loop and does not rewrite the unchanged
-// direction cosines.  The direction cosines, elastic stretches and shell bits of the
-// first pass stay in registers for the two later loops (234 VGPRs, two waves per
-// SIMD): 42% fewer bytes read than with them read back, and at 1M as fast in a
-// partly plastic step and 8% faster when every particle yields.  Keeping L0 and Tv
-// as well needs more than 256 VGPRs, one wave per SIMD, and is 28% slower.
+// the same order, so the same bits.  The kernel is bound by memory latency, so it is
+// laid out for occupancy (125 VGPRs, four waves per SIMD):
+// - registers keep across the three bond loops only each bond's neighbour index,
+//   elastic stretch and shell bit; the distance and direction cosines are recomputed
+//   from the neighbour's position in the two later loops (the same expressions, so
+//   the same values; the positions come back from cache), which halves the
+//   registers against keeping them (234 VGPRs, two waves per SIMD: 41% slower);
+// - L0 is read from HBM once and kept in LDS for the later loops, one column per
+//   thread (no thread reads another's column, so no barrier), 18 KiB per work-group;
+// - damage factors that are exactly 1.0 or +0.0 are kept as bits, so the later loops
+//   and the opposite-bond factor do not read them again;
+// - read-once streams are loaded and write-once streams stored nontemporally.
+#define J2_WG 128
 template <int NBM>
-__global__ __launch_bounds__(256) void k_j2_particle_u(
+__global__ __launch_bounds__(J2_WG) void k_j2_particle_u(
     int n, int nbmax, double volume, double H, double xi, const int *__restrict__ nb0_,
     const int *__restrict__ nbcur, const int *__restrict__ nbr, const int *__restrict__ nsign,
     const int *__restrict__ opp, const double *__restrict__ X, const double *__restrict__ L0,
@@ -195,44 +201,49 @@ __global__ __launch_bounds__(256) void k_j2_particle_u(
     double *__restrict__ beta2, double *__restrict__ alpha2, double *__restrict__ dlam_, int *__restrict__ pl_flag)
 {
     static_assert(NBM <= 32, "the shell bits of a particle's bonds share one 32-bit word");
+    __shared__ double sL0[NBM][J2_WG];
+    const int tl = threadIdx.x;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n)
         return;
     const int nb0 = nb0_[i];
     const double xi0 = X[3 * i], xi1 = X[3 * i + 1], xi2 = X[3 * i + 2];
-    // per bond, kept in registers for the later loops: distance, direction cosines,
-    // elastic stretch and the shell bit
-    double dis[NBM], rcx[NBM], rcy[NBM], rcz[NBM], rdl[NBM];
-    unsigned sbit = 0;
-    // first stretch pass (constitutive.c:495-515)
+    int qs[NBM];
+    double rdl[NBM];
+    unsigned sbit = 0, one = 0, zero = 0;
     double a0 = 0, a1 = 0, b0 = 0, b1 = 0;
 #pragma unroll
     for (int j = 0; j < NBM; j++)
         if (j < nb0)
         {
             const int64_t b = (int64_t)j * n + i;
-            const int q = nbr[b];
+            const int q = __builtin_nontemporal_load(&nbr[b]);
+            qs[j] = q;
             const double dx = xi0 - X[3 * q], dy = xi1 - X[3 * q + 1], dz = xi2 - X[3 * q + 2];
             const double d = sqrt(dx * dx + dy * dy + dz * dz);
-            dis[j] = d;
-            const double l0 = L0[b], tv = Tv[b];
+            const double l0 = __builtin_nontemporal_load(&L0[b]), tv = Tv[b],
+                         dbv = __builtin_nontemporal_load(&db[b]);
+            sL0[j][tl] = l0;
+            const unsigned long long u = (unsigned long long)__double_as_longlong(dbv);
+            if (u == 0x3FF0000000000000ULL)
+                one |= 1u << j;
+            else if (u == 0ULL)
+                zero |= 1u << j;
             double dl = d - l0;
             dl -= dLp0[b];
-            dl *= db[b];
-            rdl[j] = dl; // dL of the first pass: the second pass below stores the final one
-            if (nsign[b] == 0)
+            dl *= dbv;
+            rdl[j] = dl;
+            if (__builtin_nontemporal_load(&nsign[b]) == 0)
                 a0 += dl, b0 += tv * dl;
             else
             {
                 a1 += dl, b1 += tv * dl;
                 sbit |= 1u << j;
             }
-            rcx[j] = dx / d, rcy[j] = dy / d, rcz[j] = dz / d;
-            csx[b] = rcx[j];
-            csy[b] = rcy[j];
-            csz[b] = rcz[j];
+            __builtin_nontemporal_store(dx / d, &csx[b]);
+            __builtin_nontemporal_store(dy / d, &csy[b]);
+            __builtin_nontemporal_store(dz / d, &csz[b]);
         }
-    // trial stress from the particle's own sums (constitutive.c:518-568)
     const bool full = nbcur[i] == nbmax;
     double sl[6] = {0, 0, 0, 0, 0, 0}, dep[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll
@@ -240,16 +251,31 @@ __global__ __launch_bounds__(256) void k_j2_particle_u(
         if (j < nb0)
         {
             const int64_t b = (int64_t)j * n + i;
+            const int q = qs[j];
+            const double dx = xi0 - X[3 * q], dy = xi1 - X[3 * q + 1], dz = xi2 - X[3 * q + 2];
+            const double d = sqrt(dx * dx + dy * dy + dz * dz);
             const bool s1 = (sbit >> j) & 1u;
-            double Fij = 2.0 * Kn[b] * rdl[j] + (s1 ? b1 : b0) + Tv[b] * (s1 ? a1 : a0);
-            Fij *= dw[b];
+            double Fij = 2.0 * __builtin_nontemporal_load(&Kn[b]) * rdl[j] + (s1 ? b1 : b0) + Tv[b] * (s1 ? a1 : a0);
+            Fij *= __builtin_nontemporal_load(&dw[b]);
             double of = 0.5;
             if (!full)
             {
                 const int o = opp[b];
-                of = o < 0 ? 1.0 : (db[(int64_t)o * n + i] <= LPMX_EPS ? 1.0 : 0.5);
+                if (o < 0)
+                    of = 1.0;
+                else
+                {
+                    double dbo;
+                    if ((one >> o) & 1u)
+                        dbo = 1.0;
+                    else if ((zero >> o) & 1u)
+                        dbo = 0.0;
+                    else
+                        dbo = db[(int64_t)o * n + i];
+                    of = dbo <= LPMX_EPS ? 1.0 : 0.5;
+                }
             }
-            const double cx = rcx[j], cy = rcy[j], cz = rcz[j], l0 = L0[b];
+            const double cx = dx / d, cy = dy / d, cz = dz / d, l0 = sL0[j][tl];
             sl[0] += of / volume * l0 * Fij * cx * cx;
             sl[1] += of / volume * l0 * Fij * cy * cy;
             sl[2] += of / volume * l0 * Fij * cz * cz;
@@ -257,7 +283,6 @@ __global__ __launch_bounds__(256) void k_j2_particle_u(
             sl[4] += of / volume * l0 * Fij * cx * cz;
             sl[5] += of / volume * l0 * Fij * cx * cy;
         }
-    // radial return with linear mixed hardening (constitutive.c:571-608)
     const double tmp = 1.0 / 3.0 * (sl[0] + sl[1] + sl[2]);
     for (int c = 0; c < 3; c++)
         sl[c] -= tmp;
@@ -290,36 +315,42 @@ __global__ __launch_bounds__(256) void k_j2_particle_u(
             dep[c] = dlam * 1.5 * sl[c] / seq;
             beta[c] += 2. / 3. * xi * H * dep[c];
         }
-    // plastic bond stretch over every list slot (constitutive.c:611-621) and, per
-    // bond, the second stretch pass with it (constitutive.c:625-645): each bond's
-    // plastic stretch is final before its stretch uses it, so one loop serves both
-    // (the shell sums still run in slot order)
     a0 = 0, a1 = 0, b0 = 0, b1 = 0;
 #pragma unroll
     for (int j = 0; j < NBM; j++)
         if (j < nbmax)
         {
             const int64_t b = (int64_t)j * n + i;
-            const double dbb = db[b];
+            double dbb;
+            if (j < nb0 && ((one >> j) & 1u))
+                dbb = 1.0;
+            else if (j < nb0 && ((zero >> j) & 1u))
+                dbb = 0.0;
+            else
+                dbb = db[b];
             double x_ = dLp0[b];
             if (j < nb0)
             {
-                const double cx = rcx[j], cy = rcy[j], cz = rcz[j], l0 = L0[b];
+                const int q = qs[j];
+                const double dx = xi0 - X[3 * q], dy = xi1 - X[3 * q + 1], dz = xi2 - X[3 * q + 2];
+                const double dd = sqrt(dx * dx + dy * dy + dz * dz);
+                const double cx = dx / dd, cy = dy / dd, cz = dz / dd, l0 = sL0[j][tl];
                 double d = l0 * (dep[0] * cx * cx + dep[1] * cy * cy + dep[2] * cz * cz + 2 * dep[3] * cy * cz +
                                  2 * dep[4] * cx * cz + 2 * dep[5] * cx * cy);
                 d *= dbb;
-                ddLp[b] = d;
+                __builtin_nontemporal_store(d, &ddLp[b]);
                 x_ += d;
-                double dl = dis[j] - l0;
+                double dl = dd - l0;
                 dl -= x_;
                 dl *= dbb;
-                dL[b] = dl;
+                __builtin_nontemporal_store(dl, &dL[b]);
+                const double tv = Tv[b];
                 if (((sbit >> j) & 1u) == 0)
-                    a0 += dl, b0 += Tv[b] * dl;
+                    a0 += dl, b0 += tv * dl;
                 else
-                    a1 += dl, b1 += Tv[b] * dl;
+                    a1 += dl, b1 += tv * dl;
             }
-            dLp2[b] = dbb * x_;
+            __builtin_nontemporal_store(dbb * x_, &dLp2[b]);
         }
     A[i] = a0, A[(int64_t)n + i] = a1;
     Bt[i] = b0, Bt[(int64_t)n + i] = b1;
@@ -335,7 +366,7 @@ static void launch_j2_particles(void)
 {
     const dim3 G(nblocks(g.n)), B(LPMX_BLOCK);
     if (g.nb <= 18)
-        hipLaunchKernelGGL(k_j2_particle_u<18>, G, B, 0, g_stream, g.n, g.nb, g.volume, g.H, g.xi, g.nb0, g.nbcur,
+        hipLaunchKernelGGL(k_j2_particle_u<18>, dim3(nblocks(g.n, J2_WG)), dim3(J2_WG), 0, g_stream, g.n, g.nb, g.volume, g.H, g.xi, g.nb0, g.nbcur,
                            g.nbr, g.nsign, g.opp, g.x, g.L0, g.dLp0, g.db, g.dw, g.Kn, g.Tv, g.beta0, g.alpha0,
                            g.sigmay, g.c44, g.dL, g.csx, g.csy, g.csz, g.ddLp, g.dLp2, g.A, g.Bt, g.beta2, g.alpha2,
                            g.dlam, g.pl_flag);
@@ -362,12 +393,14 @@ __global__ void k_j2_force(DV v, int first, int count)
         double f = 2.0 * v.Kn[b] * dla + 0.5 * (v.Bt[(int64_t)s * v.n + i] + v.Bt[(int64_t)s * v.n + q]) +
                    0.5 * v.Tv[b] * (v.A[(int64_t)s * v.n + i] + v.A[(int64_t)s * v.n + q]);
         f *= v.dw[b];
-        v.F[b] = f;
+        __builtin_nontemporal_store(f, &v.F[b]);
         p0 += v.csx[b] * f;
         p1 += v.csy[b] * f;
         p2 += v.csz[b] * f;
     }
-    v.Pin[3 * i] = p0, v.Pin[3 * i + 1] = p1, v.Pin[3 * i + 2] = p2;
+    __builtin_nontemporal_store(p0, &v.Pin[3 * i]);
+    __builtin_nontemporal_store(p1, &v.Pin[3 * i + 1]);
+    __builtin_nontemporal_store(p2, &v.Pin[3 * i + 2]);
 }
 
 // ------------------------------------------------------------ predictor ---------
