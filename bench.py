@@ -35,6 +35,16 @@ ratio, and, with --adjoint-fd, central differences (h = 1e-6 relative) of J alon
 any of E, H, the yield stress and a random direction of the modulus field (each the
 logarithmic derivative d J / d log theta, compared with the adjoint's).
 
+`--workload brittle` runs the 2D elastic-brittle shear: hexagonal 1000 x 1000 =
+1,000,000 particles, E 70 GPa, nu 0.25 (plane strain), per-bond critical stretch from
+Weibull(shape 10, scale 1e-3, seed 1), bottom edge held, top edge sheared by 7.5e-6
+of the height per step over 200 steps; after every breakage pass (at most nbreak = 20
+bonds) the tangent is rebuilt and the step solved again.  The FP64 form runs as many
+steps as fit --brittle-seconds after step 1 (which builds the AMG hierarchy); the
+mixed form then runs the same steps.  The line gives both rates, the broken-bond
+counts, whether the two forms broke the same bonds, and the mixed form's
+displacement and reaction error against the FP64 form.
+
 roofline: the dominant kernel of the main form (largest total time over the timed
 steps among the profiled ones: the J2 return map k_j2_particle, the finest level's
 smoother SpMV k_slice_mv, the CG SpMV k_cg_spmv), timed with HIP events on the
@@ -347,6 +357,93 @@ def copy_probe():
 ADJ_EDGE, ADJ_BOX, ADJ_STEPS, ADJ_STRAIN = 60, 59.3, 30, 0.015
 
 
+HEX_BOX = [0.0, 999.6, 0.0, 865.77, 0.0, 1.0]  # 1000 x 1000 hexagonal lattice, radius 0.5
+HEX_STEPS = 200
+HEX_SHEAR_PER_STEP = 7.5e-6  # the reference example's 1.5e-4 mm per step on a 20 mm plate
+
+
+def brittle_workload(P, a):
+    """The 2D elastic-brittle shear workload (see the module docstring)."""
+    L = P.lib()
+    forms = [f.strip() for f in a.forms.split(",") if f.strip()]
+    res, last = {}, None
+    for f in forms:
+        prec = P.LPMX_FP64 if f == "fp64" else P.LPMX_MIXED
+        t0 = time.perf_counter()
+        n, x, _, _ = P.build_hex_model(HEX_BOX, critical=1e-3, nbreak=20)
+        nb = P.gint("nneighbors").value
+        crit = P.weibull_bond_critical(P.field("NEIGHBORS", n * nb).reshape(n, nb), 1, shape=10.0, scale=1e-3)
+        n, x, _, _ = P.build_hex_model(HEX_BOX, critical=1e-3, nbreak=20, crit=crit)
+        sync()
+        t_model = time.perf_counter() - t0
+        height = float(x[:, 1].max() - x[:, 1].min())
+        steps = P.hex_shear_steps(height, np.arange(1, HEX_STEPS + 1) * HEX_SHEAR_PER_STEP)
+        s = P.solver(precision=prec, rtol2=RTOL2, atol2=ATOL2, maxit=20000, newton_tol=a.newton_tol,
+                     newton_maxit=a.newton_maxit or 4000, inner_rtol2=RTOL2, ir_maxit=100,
+                     accel_depth=a.accel_depth or 32, precond=2)
+        tw = time.perf_counter()
+        rc, o = P.run_steps(steps[:1], s, out_axis="x", out_type=1)  # step 1: AMG set-up inside
+        P.check(rc, "brittle step 1")
+        t_first = time.perf_counter() - tw
+        log(f"brittle {f}: model of {n} particles in {t_model:.1f} s, step 1 (with AMG set-up) in {t_first:.1f} s")
+        stats, times = [], []
+        limit = last if last is not None else HEX_STEPS
+        tb = time.perf_counter()
+        for k in range(1, limit):
+            ts = time.perf_counter()
+            rc, o = P.run_steps(steps[k:k + 1], s, out_axis="x", out_type=1)
+            sync()
+            times.append(time.perf_counter() - ts)
+            stats.append(o[0])
+            if rc != 0:
+                log(f"brittle {f}: step {k + 1} stopped with status {rc}")
+                break
+            if o[0].broken or k % 20 == 0:
+                log(f"brittle {f}: step {k + 1}: Newton {o[0].newton_iters}, re-solves {o[0].resolves}, "
+                    f"broken {o[0].broken}, {times[-1]:.2f} s")
+            if last is None and time.perf_counter() - tb > a.brittle_seconds:
+                break
+        last = len(stats) + 1
+        dmg = P.field("DAMAGE_BROKEN", n * nb)
+        res[f] = dict(n=n, rc=rc, stats=stats, times=times, t_model=t_model, t_first=t_first,
+                      x=P.field("XYZ", 3 * n).copy(), broken=int((dmg == 0.0).sum()) // 2,
+                      broken_set=np.flatnonzero(dmg == 0.0))
+    main = "mixed" if "mixed" in res else forms[0]
+    r = res[main]
+    nst = len(r["times"])
+    line = {
+        "metric": "load steps/s, 1M-particle 2D hexagonal elastic-brittle shear; mixed vs FP64",
+        "value": nst / sum(r["times"]) if r["rc"] == 0 and nst else None,
+        "unit": "load steps/s", "n_gpus": 1, "steps": nst, "warmup": 1,
+        "ms_per_step": 1e3 * sum(r["times"]) / max(nst, 1), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32+f64" if main == "mixed" else "f64", "data": "synthetic",
+        "config": {"workload": f"hexagonal lattice 1000 x 1000 = {r['n']} particles ({2 * r['n']} DOF), elastic-brittle "
+                               f"(E 70 GPa, nu 0.25, plane strain), Weibull critical stretch (shape 10, scale 1e-3, "
+                               f"seed 1), bottom edge held, top edge sheared {HEX_SHEAR_PER_STEP:g} per step over "
+                               f"{HEX_STEPS} steps, nbreak 20 with a tangent rebuild and re-solve after every break; "
+                               f"timed load steps 2..{nst + 1}",
+                   "form": main, "solver": "one AMG V-cycle per Newton iteration, Anderson(32)",
+                   "newton_tol": a.newton_tol},
+        "forms": {f: {"load_steps_per_s": len(v["times"]) / sum(v["times"]) if v["times"] else None,
+                      "steps_done": len(v["times"]) + 1, "status": v["rc"],
+                      "newton_iters": [o.newton_iters for o in v["stats"]],
+                      "resolves": [o.resolves for o in v["stats"]], "broken_per_step": [o.broken for o in v["stats"]],
+                      "broken_bonds": v["broken"], "model_setup_s": v["t_model"], "step1_with_amg_setup_s": v["t_first"]}
+                  for f, v in res.items()},
+    }
+    if "fp64" in res and "mixed" in res and len(res["fp64"]["stats"]) == len(res["mixed"]["stats"]):
+        a64, a32 = res["fp64"], res["mixed"]
+        x0 = P.field("XYZ_INITIAL", 3 * a64["n"])
+        rel = float(np.abs(a32["x"] - a64["x"]).max() / max(np.abs(a64["x"] - x0).max(), 1e-300))
+        r64 = np.array([o.reaction for o in a64["stats"]])
+        r32 = np.array([o.reaction for o in a32["stats"]])
+        line["mixed_speedup_vs_fp64"] = line["forms"]["mixed"]["load_steps_per_s"] / line["forms"]["fp64"]["load_steps_per_s"]
+        line["mixed_disp_rel_err"] = rel
+        line["mixed_reaction_rel_err"] = float(np.abs(r32 - r64).max() / max(np.abs(r64).max(), 1e-300))
+        line["broken_sets_identical"] = bool(np.array_equal(a64["broken_set"], a32["broken_set"]))
+    print(json.dumps(line), flush=True)
+
+
 def adjoint_workload(P, a):
     """The history-gradient workload (see the module docstring)."""
     os.environ.setdefault("LPMX_PROGRESS", "1")
@@ -475,7 +572,9 @@ def main():
     ap.add_argument("--compare", default="amg-cg", choices=sorted(SOLVERS) + ["none"],
                     help="a second configuration run on the same steps, both forms (the CG form of the "
                          "mixed-vs-FP64 comparison); none to skip")
-    ap.add_argument("--workload", default="loadsteps", choices=["loadsteps", "adjoint"])
+    ap.add_argument("--workload", default="loadsteps", choices=["loadsteps", "adjoint", "brittle"])
+    ap.add_argument("--brittle-seconds", type=float, default=60.0,
+                    help="brittle workload: wall-clock budget of the first form's timed steps")
     ap.add_argument("--no-schedule", action="store_true",
                     help="skip the rest of the 50-step schedule after the timed steps (no whole-schedule figures)")
     ap.add_argument("--ckpt-every", type=int, default=5, help="adjoint workload: checkpoint spacing")
@@ -498,6 +597,9 @@ def main():
         return
     if a.workload == "adjoint":
         adjoint_workload(product(), a)
+        return
+    if a.workload == "brittle":
+        brittle_workload(product(), a)
         return
     if a.warmup + a.steps > NSTEPS_SCHEDULE or a.steps < 1 or a.warmup < 0:
         sys.exit(f"--warmup + --steps must lie in 1..{NSTEPS_SCHEDULE}")
