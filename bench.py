@@ -378,11 +378,14 @@ def brittle_workload(P, a):
         t_model = time.perf_counter() - t0
         height = float(x[:, 1].max() - x[:, 1].min())
         steps = P.hex_shear_steps(height, np.arange(1, HEX_STEPS + 1) * HEX_SHEAR_PER_STEP)
-        s = P.solver(precision=prec, rtol2=RTOL2, atol2=ATOL2, maxit=20000, newton_tol=a.newton_tol,
+        s = P.solver(precision=prec, rtol2=RTOL2, atol2=ATOL2, maxit=20000, newton_tol=a.brittle_newton_tol,
                      newton_maxit=a.newton_maxit or 4000, inner_rtol2=RTOL2, ir_maxit=100,
                      accel_depth=a.accel_depth or 32, precond=2)
         tw = time.perf_counter()
         rc, o = P.run_steps(steps[:1], s, out_axis="x", out_type=1)  # step 1: AMG set-up inside
+        if rc != 0:
+            log(f"brittle {f}: step 1: status {o[0].status}, Newton {o[0].newton_iters}, |R| {o[0].res_norm:.3e} "
+                f"of {o[0].tol_mult:.3e}")
         P.check(rc, "brittle step 1")
         t_first = time.perf_counter() - tw
         log(f"brittle {f}: model of {n} particles in {t_model:.1f} s, step 1 (with AMG set-up) in {t_first:.1f} s")
@@ -396,7 +399,8 @@ def brittle_workload(P, a):
             times.append(time.perf_counter() - ts)
             stats.append(o[0])
             if rc != 0:
-                log(f"brittle {f}: step {k + 1} stopped with status {rc}")
+                log(f"brittle {f}: step {k + 1} stopped with status {rc} (step status {o[0].status}, Newton "
+                    f"{o[0].newton_iters}, re-solves {o[0].resolves}, |R| {o[0].res_norm:.3e} of {o[0].tol_mult:.3e})")
                 break
             if o[0].broken or k % 20 == 0:
                 log(f"brittle {f}: step {k + 1}: Newton {o[0].newton_iters}, re-solves {o[0].resolves}, "
@@ -423,7 +427,7 @@ def brittle_workload(P, a):
                                f"{HEX_STEPS} steps, nbreak 20 with a tangent rebuild and re-solve after every break; "
                                f"timed load steps 2..{nst + 1}",
                    "form": main, "solver": "one AMG V-cycle per Newton iteration, Anderson(32)",
-                   "newton_tol": a.newton_tol},
+                   "newton_tol": a.brittle_newton_tol},
         "forms": {f: {"load_steps_per_s": len(v["times"]) / sum(v["times"]) if v["times"] else None,
                       "steps_done": len(v["times"]) + 1, "status": v["rc"],
                       "newton_iters": [o.newton_iters for o in v["stats"]],
@@ -575,6 +579,10 @@ def main():
     ap.add_argument("--workload", default="loadsteps", choices=["loadsteps", "adjoint", "brittle"])
     ap.add_argument("--brittle-seconds", type=float, default=60.0,
                     help="brittle workload: wall-clock budget of the first form's timed steps")
+    ap.add_argument("--brittle-newton-tol", type=float, default=1e-7,
+                    help="brittle workload: Newton test |R| <= tol * max(|R0|, |reaction|) (the reference's TOLITER "
+                         "is 1e-4); at this size, where coordinates reach 1000, the FP64 residual floor is about "
+                         "3.6e-6, and a re-solve after a break starts from max(|R0|, |reaction|) of a few hundred")
     ap.add_argument("--no-schedule", action="store_true",
                     help="skip the rest of the 50-step schedule after the timed steps (no whole-schedule figures)")
     ap.add_argument("--ckpt-every", type=int, default=5, help="adjoint workload: checkpoint spacing")
