@@ -154,3 +154,51 @@ def test_write_dump_lammps_layout(oracle, tmp_path):
     for i, r in enumerate(rows):
         assert int(r[0]) == i and int(r[1]) == t[i]
         np.testing.assert_allclose([float(v) for v in r[2:8]], list(x[i] + u) + list(u), rtol=1e-4, atol=1e-4)
+
+
+def test_write_dump_crystal_plasticity_columns(oracle, tmp_path):
+    """writeDump with plmode 1 (host, src/data_handler.c:59-70): the slip header and,
+    per particle, the stress, the accumulated slips of systems 1, 4, 10, 19, 9, 14,
+    17 and 22 (cp_A_single, current slot), pl_flag and the total slip cp_A."""
+    L = P.lib()
+    Lb = P.sc_box_for(4)
+    n, x = host_lattice(oracle, 2, 0.5, [0, Lb, 0, Lb, 0, Lb])
+    P.set_types(P.sc_types(x), 5)
+    a3 = L.allocDouble3D
+    a3.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double]
+    a3.restype = C.c_void_p
+    a2 = L.allocDouble2D
+    P.set_gptr("cp_A_single", C.cast(a3(n, 24, 3, 0.0), C.POINTER(C.POINTER(C.POINTER(C.c_double)))),
+               C.c_double, 3)
+    P.set_gptr("cp_A", a2(n, 3, 0.0), C.c_double, 2)
+    cas = P.gptr("cp_A_single", 3)
+    ca = P.gptr("cp_A", 2)
+    st = P.gptr("stress_tensor", 2)
+    pl = P.gptr("pl_flag", 1, C.c_int)
+    for i in range(n):
+        for k in range(24):
+            cas[i][k][0] = 1e-3 * (i + 1) + 1e-5 * k
+            cas[i][k][1] = -1.0  # other slots are not written
+        ca[i][0] = 0.5 * (i + 1)
+        for c in range(6):
+            st[i][c] = 10.0 * c + i
+        pl[i] = i % 2
+    P.gint("plmode").value = 1
+    try:
+        f = str(tmp_path / "dump_cp.txt")
+        fn = L.writeDump
+        fn.argtypes = [C.c_char_p, C.c_int, C.c_char]
+        fn.restype = None
+        fn(f.encode(), 3, b"s")
+    finally:
+        P.gint("plmode").value = 0
+    lines = open(f).read().splitlines()
+    assert lines[8] == ("ITEM: ATOMS id type x y z dx dy dz s11 s22 s33 s23 s13 s12 gam11 gam12 gam13 gam14 "
+                        "gam21 gam22 gam23 gam24 pl gam_total")
+    rows = [ln.split() for ln in lines[9:]]
+    assert len(rows) == n and all(len(r) == 24 for r in rows)
+    sys_cols = [1, 4, 10, 19, 9, 14, 17, 22]
+    for i, r in enumerate(rows):
+        assert [float(v) for v in r[8:14]] == [float("%.4e" % (10.0 * c + i)) for c in range(6)]
+        assert [float(v) for v in r[14:22]] == [float("%.4e" % (1e-3 * (i + 1) + 1e-5 * k)) for k in sys_cols]
+        assert int(r[22]) == i % 2 and float(r[23]) == float("%.4e" % (0.5 * (i + 1)))
