@@ -63,6 +63,11 @@ schedule: each form also runs the rest of the 50-step schedule after the timed s
 the line gives the whole schedule's time with and without the model set-up (the AMG
 set-up runs inside step 1's first solve, so it is in the step times).
 
+adjoint: after the load steps the default run also runs the gradient workload below
+(mixed form, Newton 1e-10) and summarises it: the gradient's cost as a multiple of
+the forward's, both times, the adjoint iterations and the history's bytes
+(`--no-adjoint` skips it).
+
 cpu_baseline: the oracle's load step (the reference's algorithm and tolerances) on
 the SC 20^3 lattice of the same workload, as many steps as fit the --cpu-seconds
 budget, SpMV threaded over the host cores, in load steps/s; cpu_cg_baseline: the
@@ -540,9 +545,10 @@ def cp_workload(P, a):
     print(json.dumps(line), flush=True)
 
 
-def adjoint_workload(P, a):
-    """The history-gradient workload (see the module docstring)."""
-    os.environ.setdefault("LPMX_PROGRESS", "1")
+def adjoint_workload(P, a, emit=True):
+    """The history-gradient workload (see the module docstring); returns its line."""
+    if emit:
+        os.environ.setdefault("LPMX_PROGRESS", "1")
     E0, NU, SY0, H0 = 200e3, 0.3, 200.0, 2000.0
     strains = np.arange(1, ADJ_STEPS + 1) * (ADJ_STRAIN / ADJ_STEPS)
     steps = P.sc_uniaxial_steps(ADJ_EDGE - 1.0, strains)
@@ -615,7 +621,9 @@ def adjoint_workload(P, a):
         "dJ_demod_norm": float(np.linalg.norm(ge)),
         "plastic_particles": int((P.field("PL_FLAG", n0) > 0).sum()),
     }
-    print(json.dumps(line), flush=True)
+    if emit:
+        print(json.dumps(line), flush=True)
+    return line
     if a.adjoint_fd:
         h = 1e-6
         rng = np.random.default_rng(3)
@@ -680,6 +688,8 @@ def main():
                     help="brittle workload: Newton test |R| <= tol * max(|R0|, |reaction|) (the reference's TOLITER "
                          "is 1e-4); at this size, where coordinates reach 1000, the FP64 residual floor is about "
                          "3.6e-6, and a re-solve after a break starts from max(|R0|, |reaction|) of a few hundred")
+    ap.add_argument("--no-adjoint", action="store_true",
+                    help="skip the gradient run summarised under `adjoint` in the default line")
     ap.add_argument("--no-schedule", action="store_true",
                     help="skip the rest of the 50-step schedule after the timed steps (no whole-schedule figures)")
     ap.add_argument("--ckpt-every", type=int, default=5, help="adjoint workload: checkpoint spacing")
@@ -772,6 +782,18 @@ def main():
         line["rooflines_fp64"] = form_rooflines(res["fp64"], "fp64")
     if r.get("amg"):
         line["amg"] = {f: res[f]["amg"] for f in res}
+    if not a.no_adjoint:
+        # the gradient workload (SC 60^3, 30 steps, mixed form, Newton 1e-10), summarised
+        aa = argparse.Namespace(**vars(a))
+        aa.forms, aa.newton_tol, aa.adjoint_target = "mixed", 1e-10, ""
+        try:
+            al = adjoint_workload(P, aa, emit=False)
+            line["adjoint"] = {"gradient_cost_x_forward": al["value"], "t_forward_s": al["t_forward_s"],
+                               "t_gradient_s": al["t_gradient_s"], "adjoint_iters": al["adjoint_iters"],
+                               "history_bytes": al["history_bytes"], "workload": al["config"]["workload"],
+                               "newton_tol": 1e-10, "ckpt_every": al["config"]["ckpt_every"]}
+        except Exception as e:  # a failed gradient run is reported, not fatal to the line
+            line["adjoint"] = {"error": str(e)}
     if a.compare != "none" and a.compare != a.solver and len(forms) == 2:
         cres = {}
         for f in forms:
