@@ -172,18 +172,24 @@ HB transpose(const HB &A)
         T.ptr[A.idx[k] + 1]++;
     for (int i = 0; i < T.nr; i++)
         T.ptr[i + 1] += T.ptr[i];
-    std::vector<int64_t> pos(T.ptr.begin(), T.ptr.end() - 1);
+    // the place of every block in T (ascending rows of A: the columns of T ascend),
+    // then the copies in parallel
+    std::vector<int64_t> pos(T.ptr.begin(), T.ptr.end() - 1), dst(A.nnzb());
+    for (int64_t k = 0; k < A.nnzb(); k++)
+        dst[k] = pos[A.idx[k]]++;
     T.idx.resize(A.nnzb());
     T.val.resize(A.nnzb() * bs);
-    for (int i = 0; i < A.nr; i++) // ascending rows: columns of T ascend
-        for (int64_t k = A.ptr[i]; k < A.ptr[i + 1]; k++)
-        {
-            const int64_t d = pos[A.idx[k]]++;
-            T.idx[d] = i;
-            for (int r = 0; r < A.br; r++)
-                for (int c = 0; c < A.bc; c++)
-                    T.val[d * bs + c * A.br + r] = A.val[k * bs + r * A.bc + c];
-        }
+    parallel_for(A.nr, [&](int a, int b, int) {
+        for (int i = a; i < b; i++)
+            for (int64_t k = A.ptr[i]; k < A.ptr[i + 1]; k++)
+            {
+                const int64_t d = dst[k];
+                T.idx[d] = i;
+                for (int r = 0; r < A.br; r++)
+                    for (int c = 0; c < A.bc; c++)
+                        T.val[d * bs + c * A.br + r] = A.val[k * bs + r * A.bc + c];
+            }
+    });
     return T;
 }
 
@@ -595,8 +601,11 @@ static int upload(T *dst, const std::vector<T> &src)
 static int upload_vals(const std::vector<double> &val, Mat &B)
 {
     std::vector<float> v32(val.size());
-    for (size_t q = 0; q < val.size(); q++)
-        v32[q] = (float)val[q];
+    const int64_t nv = (int64_t)val.size(), chunk = 1 << 16;
+    parallel_for((int)((nv + chunk - 1) / chunk), [&](int a, int b, int) {
+        for (int64_t q = (int64_t)a * chunk; q < std::min(nv, (int64_t)b * chunk); q++)
+            v32[q] = (float)val[q];
+    });
     RETCHK(amg_alloc(&B.val, (int64_t)val.size()));
     RETCHK(amg_alloc(&B.val32, (int64_t)v32.size()));
     RETCHK(upload(B.val, val));
@@ -643,20 +652,22 @@ static int upload_mat(const HB &H, Mat &B)
     B.nslots = sptr[B.nslices];
     std::vector<int> col((size_t)B.nslots * 64, 0);
     std::vector<double> val((size_t)B.nslots * bs * 64, 0.0);
-    for (int i = 0; i < H.nr; i++)
-    {
-        const int s = i / 64, l = i % 64;
-        int k = 0;
-        for (int64_t q = H.ptr[i]; q < H.ptr[i + 1]; q++, k++)
+    parallel_for(H.nr, [&](int a, int b, int) { // rows write disjoint lanes
+        for (int i = a; i < b; i++)
         {
-            const int64_t slot = sptr[s] + k;
-            col[slot * 64 + l] = H.idx[q];
-            for (int e = 0; e < bs; e++)
-                val[(slot * bs + e) * 64 + l] = H.val[q * bs + e];
+            const int s = i / 64, l = i % 64;
+            int k = 0;
+            for (int64_t q = H.ptr[i]; q < H.ptr[i + 1]; q++, k++)
+            {
+                const int64_t slot = sptr[s] + k;
+                col[slot * 64 + l] = H.idx[q];
+                for (int e = 0; e < bs; e++)
+                    val[(slot * bs + e) * 64 + l] = H.val[q * bs + e];
+            }
+            for (; sptr[s] + k < sptr[s + 1]; k++) // padding: a zero block at a valid column
+                col[(int64_t)(sptr[s] + k) * 64 + l] = H.nr == H.nc ? i : 0;
         }
-        for (; sptr[s] + k < sptr[s + 1]; k++) // padding: a zero block at a valid column
-            col[(int64_t)(sptr[s] + k) * 64 + l] = H.nr == H.nc ? i : 0;
-    }
+    });
     RETCHK(amg_alloc(&B.sptr, B.nslices + 1));
     RETCHK(amg_alloc(&B.col, (int64_t)col.size()));
     RETCHK(upload(B.sptr, sptr));
@@ -816,19 +827,32 @@ int lpmx_amg_build(void)
     // bond graph (ascending, self excluded)
     std::vector<int64_t> aptr(n + 1, 0);
     std::vector<int> adj;
-    for (int i = 0; i < n; i++)
     {
-        std::vector<int> nb;
-        for (int j = 0; j < nb0[i]; j++)
-        {
-            const int q = nbr[(size_t)j * n + i];
-            if (q >= 0 && q != i)
-                nb.push_back(q);
-        }
-        std::sort(nb.begin(), nb.end());
-        nb.erase(std::unique(nb.begin(), nb.end()), nb.end());
-        adj.insert(adj.end(), nb.begin(), nb.end());
-        aptr[i + 1] = (int64_t)adj.size();
+        // per row the sorted distinct partners (at most nb), then the prefix and the fill
+        const int w = g.nb;
+        std::vector<int> rows((size_t)n * w), cnt(n);
+        parallel_for(n, [&](int a, int b, int) {
+            for (int i = a; i < b; i++)
+            {
+                int *o = &rows[(size_t)i * w];
+                int m = 0;
+                for (int j = 0; j < nb0[i]; j++)
+                {
+                    const int q = nbr[(size_t)j * n + i];
+                    if (q >= 0 && q != i)
+                        o[m++] = q;
+                }
+                std::sort(o, o + m);
+                cnt[i] = (int)(std::unique(o, o + m) - o);
+            }
+        });
+        for (int i = 0; i < n; i++)
+            aptr[i + 1] = aptr[i] + cnt[i];
+        adj.resize(aptr[n]);
+        parallel_for(n, [&](int a, int b, int) {
+            for (int i = a; i < b; i++)
+                std::copy(&rows[(size_t)i * w], &rows[(size_t)i * w] + cnt[i], adj.begin() + aptr[i]);
+        });
     }
     int na = 0;
     phase("fine matrix copied");
@@ -871,7 +895,7 @@ int lpmx_amg_build(void)
         }
     }
     // levels
-    std::vector<HB> As, Ps;
+    std::vector<HB> As, Ps, Rs;
     std::vector<std::vector<double>> Dinvs;
     std::vector<double> lm;
     As.push_back(std::move(A));
@@ -882,6 +906,7 @@ int lpmx_amg_build(void)
         std::vector<double> Dinv = diag_inverses(Al);
         const double lmax = lambda_max(Al, Dinv, 20);
         lm.push_back(lmax);
+        phase("diagonal inverses and spectral bound");
         if ((int64_t)Al.nr * Al.br <= dense_max || lev == max_levels - 1)
         {
             Dinvs.push_back(std::move(Dinv));
@@ -896,12 +921,16 @@ int lpmx_amg_build(void)
         }
         std::vector<double> Bc;
         HB P0 = tentative(agg, na, Al.br, nm, B, Bc);
+        phase("aggregates and tentative prolongator");
         HB P = smooth_prolongator(Al, Dinv, P0, 4.0 / (3.0 * lmax));
+        phase("smoothed prolongator");
         HB R = transpose(P);
         HB AP = spgemm(Al, P);
+        phase("A P");
         HB Ac = spgemm(R, AP);
         Dinvs.push_back(std::move(Dinv));
         Ps.push_back(std::move(P));
+        Rs.push_back(std::move(R));
         As.push_back(std::move(Ac));
         B.swap(Bc);
         quotient_graph(agg, na, aptr, adj);
@@ -911,6 +940,7 @@ int lpmx_amg_build(void)
     }
     // upload: levels 0 .. nlev-1 smoothed, level nlev dense
     AM.nlev = (int)As.size() - 1;
+    phase("levels built");
     AM.L.resize(AM.nlev + 1);
     for (int lev = 0; lev <= AM.nlev; lev++)
     {
@@ -937,9 +967,10 @@ int lpmx_amg_build(void)
             RETCHK(h2d(v.Minv32, mp32.data(), sizeof(float) * mp32.size()));
         }
         RETCHK(upload_mat(Ps[lev], v.P));
-        RETCHK(upload_mat(transpose(Ps[lev]), v.R));
+        RETCHK(upload_mat(Rs[lev], v.R));
         RETCHK(alloc_vectors(v));
     }
+    phase("levels uploaded");
     // dense coarsest level
     {
         const HB &Ac = As[AM.nlev];
