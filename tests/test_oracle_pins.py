@@ -516,3 +516,39 @@ def test_compute_strain_singular_system_stores_its_right_hand_side(oracle):
         np.testing.assert_allclose(st[i], [rhs, 0, 0, 0, 0, 0], rtol=1e-15, atol=0)
     # the block's particles are regular and fitted
     assert np.all(np.abs(st[3:] - 7.0) > 1.0)
+
+
+# ------------------------------------------------------------------- calcKnTv --
+def test_kntv_endpoint_average_over_types_and_modulus_field(oracle):
+    """calcKnTv for SC (reference src/stiffness.c:178-203): per type the bond constants
+    r (C11 - C12 - C44), r C44 and r (C12 - C44) / 18, and per bond the mean of its two
+    endpoints' values (the first-shell constant for a first-shell bond, the second-shell
+    one otherwise; Tv likewise), each endpoint's value scaled by its modulus factor
+    first (the configurations' per-particle modulus).  Three types at random and a
+    factor field in [0.95, 1.05]; every bond's Kn and Tv to round-off, Kn and Tv
+    symmetric in (i, j)."""
+    x = oracle.create_lattice(2, 0.5, [0, 5.2, 0, 5.2, 0, 5.2])
+    n = x.shape[0]
+    rng = np.random.default_rng(21)
+    types = rng.integers(0, 3, n).astype(np.int32)
+    Ce = [oracle.elastic_constants(E, nu) for E, nu in ((200e3, 0.3), (120e3, 0.25), (70e3, 0.33))]
+    em = 1.0 + 0.05 * rng.uniform(-1, 1, n)
+    m = oracle.Model(x, 2, 0.5, types, Ce, emod=em)
+    m.setup()
+    r = 0.5
+    kt = np.array([[r * (c11 - c12 - c44), r * c44, r * (c12 - c44) / 18.0] for c11, c12, c44 in Ce])
+    live = m.nbr >= 0
+    i_idx = np.broadcast_to(np.arange(n)[:, None], m.nbr.shape)[live]
+    j_idx = m.nbr[live]
+    s = m.nsign[live]
+    kn_i = em[i_idx] * kt[types[i_idx], s]
+    kn_j = em[j_idx] * kt[types[j_idx], s]
+    tv_i = em[i_idx] * kt[types[i_idx], 2]
+    tv_j = em[j_idx] * kt[types[j_idx], 2]
+    np.testing.assert_allclose(m.Kn[live], 0.5 * (kn_i + kn_j), rtol=1e-15, atol=0)
+    np.testing.assert_allclose(m.Tv[live], 0.5 * (tv_i + tv_j), rtol=1e-14, atol=0)
+    # symmetry: bond (i, j) and its reverse carry the same constants
+    for i, j in zip(i_idx[:500], j_idx[:500]):
+        k = int(np.where(m.nbr[j] == i)[0][0])
+        row = int(np.where(m.nbr[i] == j)[0][0])
+        assert m.Kn[i, row] == m.Kn[j, k] and m.Tv[i, row] == m.Tv[j, k]
