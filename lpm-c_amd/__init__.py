@@ -290,8 +290,8 @@ def set_types(types, ntype):
     """type[] (allocated through allocInt1D, as the reference driver does) and ntype."""
     n = gint("nparticle").value
     p = lib().allocInt1D(n, 0)
-    for k, t in enumerate(np.asarray(types, np.int32)):
-        p[k] = int(t)
+    tv = np.ascontiguousarray(types, np.int32)
+    C.memmove(p, tv.ctypes.data, 4 * n)
     set_gptr("type", p, C.c_int, 1)
     gint("ntype").value = ntype
 
@@ -517,7 +517,7 @@ def build_sc_model(L, E=200e3, nu=0.3, sigmay=200.0, H=2000.0, emod=None, damage
     n = make_lattice(2, 0.5, [0.0, L, 0.0, L, 0.0, L])
     Lb.searchNormalNeighbor()
     Lb.searchAFEMNeighbor()
-    x = positions()
+    x = field("XYZ", 3 * n).reshape(n, 3)  # the device model's copy of xyz, made by the search
     t = sc_types(x)
     set_types(t, 5)
     c = elastic_constants(E, nu)
@@ -588,7 +588,7 @@ def build_hex_model(box, E=70e3, nu=0.25, critical=1e-3, nbreak=20, crit=None):
     n = make_lattice(1, 0.5, box)
     Lb.searchNormalNeighbor()
     Lb.searchAFEMNeighbor()
-    x = positions()
+    x = field("XYZ", 3 * n).reshape(n, 3)  # the device model's copy of xyz, made by the search
     t = hex_types(x)
     set_types(t, 3)
     c = elastic_constants(E, nu)
@@ -621,7 +621,7 @@ def build_fcc_cp_model(L, cp, E=70e3, nu=0.3):
     n = make_lattice(3, 0.5, [0.0, L, 0.0, L, 0.0, L])
     Lb.searchNormalNeighbor()
     Lb.searchAFEMNeighbor()
-    x = positions()
+    x = field("XYZ", 3 * n).reshape(n, 3)  # the device model's copy of xyz, made by the search
     t = sc_types(x)
     set_types(t, 5)
     c = elastic_constants(E, nu)
