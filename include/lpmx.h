@@ -364,9 +364,15 @@ void lpmx_profile_amg_read(double *total_ms, int64_t *launches);
 /* Algorithmic HBM bytes of one such launch: nnzb*(dim^2*s_val + 4) + 4*(nslices+1)
  * + 3*n*dim*s_vec (x read once, b or y read, y written; s = 8 FP64, 4 mixed). */
 int lpmx_amg_smoother_bytes(int precision, double *bytes);
-/* The current AMG hierarchy: levels (0 when none), host set-up time of its last
- * build, device memory it holds. */
+/* The current AMG hierarchy: levels (0 when none), wall time of its last build,
+ * device memory it holds. */
 int lpmx_amg_info(int *levels, double *setup_seconds, int64_t *device_bytes);
+/* No counterpart in the reference; for tests of the AMG set-up.  The stored blocks of
+ * level `level`'s operator (which 0, levels >= 1), prolongator (1) or restriction (2)
+ * as block COO: rows[k], cols[k], blocks vals[k*br*bc ...] row-major (a sliced-ELL
+ * level includes its zero padding blocks).  dims receives nr, nc, br, bc and the
+ * block count; with rows null only dims is filled. */
+int lpmx_amg_export(int level, int which, int64_t cap, int *rows, int *cols, double *vals, int *dims);
 /* HIP-event timing of the J2 return-map kernel (k_j2_particle, one launch per force
  * evaluation) while on; switching on resets the totals. */
 void lpmx_profile_force(int on);

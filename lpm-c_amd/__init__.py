@@ -152,6 +152,7 @@ def lib():
         "lpmx_profile_amg_read": [C.POINTER(C.c_double), C.POINTER(C.c_int64)],
         "lpmx_amg_smoother_bytes": [C.c_int, C.POINTER(C.c_double)],
         "lpmx_amg_info": [_ip, C.POINTER(C.c_double), C.POINTER(C.c_int64)],
+        "lpmx_amg_export": [C.c_int, C.c_int, C.c_int64, _ip, _ip, _dp, _ip],
         "lpmx_profile_force_read": [C.POINTER(C.c_double), C.POINTER(C.c_int64)],
         "lpmx_j2_particle_bytes": [C.POINTER(C.c_double)],
     }

@@ -1,20 +1,23 @@
 // Smoothed-aggregation algebraic multigrid as the CG preconditioner (no counterpart
 // in the reference, whose solverCG is unpreconditioned).
 //
-// Setup (host, from the current constraint-eliminated tangent):
-//   - aggregates: greedy in index order over a graph (a root and its free
+// Setup, from the current constraint-eliminated tangent:
+//   - aggregates (host): greedy in index order over a graph (a root and its free
 //     neighbours), then leftovers join a neighbouring aggregate; the graph is the
 //     bond graph on the finest level and, on each coarser level, the adjacency of the
 //     aggregates of the level above (a and b adjacent when a member of a has a
 //     neighbour in b), so every level coarsens by about the size of a neighbourhood
 //     and the Galerkin stencils stay compact;
-//   - tentative prolongator: the rigid-body modes of each aggregate (3 translations
-//     and 3 rotations in 3D, 2 + 1 in 2D) about its centroid, zero on constrained
-//     DOFs, orthonormalised per aggregate (modified Gram-Schmidt; a dependent mode
-//     gets a zero column); the triangular factor carries the modes to the next level;
-//   - smoothed prolongator P = (I - w D^-1 A) P0, w = 4 / (3 lambda_max(D^-1 A));
-//   - Galerkin coarse matrix P^T A P, down to at most ~1200 coarse DOFs, which are
-//     inverted densely.
+//   - tentative prolongator (device): the rigid-body modes of each aggregate (3
+//     translations and 3 rotations in 3D, 2 + 1 in 2D) about its centroid, zero on
+//     constrained DOFs, orthonormalised per aggregate (modified Gram-Schmidt; a
+//     dependent mode gets a zero column); the triangular factor carries the modes to
+//     the next level;
+//   - smoothed prolongator P = (I - w D^-1 A) P0, w = 4 / (3 lambda_max(D^-1 A))
+//     (device: power iteration, block sparse product);
+//   - Galerkin coarse matrix P^T A P (device), down to at most ~1200 coarse DOFs,
+//     which are inverted densely (host).
+// The device kernels of the set-up are in amg_setup.hip.
 // The hierarchy is built on the first preconditioned solve and kept while the
 // pattern and the constraint set stay the same: the smoother of the finest level
 // always uses the current tangent and its block-Jacobi inverse, the coarse levels
@@ -22,9 +25,9 @@
 // Solve (device): one V-cycle per CG iteration with a Chebyshev polynomial smoother
 // (block-Jacobi scaled, interval [1.15 lambda_max / ratio, 1.15 lambda_max]) of the
 // same degree before and after the coarse correction, so the preconditioner is
-// symmetric positive definite.  Every level is stored like the fine tangent: sliced
-// ELL blocks, 64 rows per slice, slot-major within a slice, so one kernel serves
-// the level matrices, P and R = P^T in FP64 and FP32.  Everything is per row, in a
+// symmetric positive definite.  Levels with many rows are stored like the fine
+// tangent (sliced ELL blocks, 64 rows per slice), levels with few, long rows as block
+// CSR read by one wavefront per row, in FP64 and FP32.  Everything is per row, in a
 // fixed order: results are deterministic.
 #include <math.h>
 #include <stdio.h>
@@ -33,166 +36,14 @@
 
 #include <algorithm>
 #include <chrono>
-#include <thread>
 #include <vector>
 
+#include "amg_setup.h"
 #include "common.h"
 
 // ------------------------------------------------------------------ host side ----
 namespace
 {
-struct HB // block CSR, blocks row-major br x bc
-{
-    int nr = 0, nc = 0, br = 0, bc = 0;
-    std::vector<int64_t> ptr;
-    std::vector<int> idx;
-    std::vector<double> val;
-    int64_t nnzb() const { return ptr.empty() ? 0 : ptr[nr]; }
-};
-
-int host_threads(void)
-{
-    unsigned h = std::thread::hardware_concurrency();
-    int t = h ? (int)h : 4;
-    const char *e = getenv("OMP_NUM_THREADS");
-    if (e && atoi(e) > 0)
-        t = std::min(t, atoi(e));
-    return std::max(1, std::min(t, 32));
-}
-
-template <typename F>
-void parallel_for(int n, F f)
-{
-    const int T = std::min(host_threads(), std::max(1, n / 256));
-    if (T <= 1)
-    {
-        f(0, n, 0);
-        return;
-    }
-    std::vector<std::thread> th;
-    for (int t = 0; t < T; t++)
-    {
-        const int a = (int)((int64_t)n * t / T), b = (int)((int64_t)n * (t + 1) / T);
-        th.emplace_back([=, &f] { f(a, b, t); });
-    }
-    for (auto &x : th)
-        x.join();
-}
-
-// C = A B (A: br x bk blocks, B: bk x bc blocks); per row the columns ascend.
-HB spgemm(const HB &A, const HB &B)
-{
-    HB C;
-    C.nr = A.nr, C.nc = B.nc, C.br = A.br, C.bc = B.bc;
-    const int br = A.br, bk = A.bc, bc = B.bc, bs = br * bc;
-    const int T = std::min(host_threads(), std::max(1, A.nr / 256));
-    std::vector<std::vector<int>> ri(T), rn(T);      // per thread: row lengths, columns
-    std::vector<std::vector<double>> rv(T);
-    std::vector<int> first(T + 1);
-    for (int t = 0; t <= T; t++)
-        first[t] = (int)((int64_t)A.nr * t / T);
-    std::vector<std::thread> th;
-    for (int t = 0; t < T; t++)
-        th.emplace_back([&, t] {
-            std::vector<int> mark(B.nc, -1), cols;
-            std::vector<double> acc;
-            for (int i = first[t]; i < first[t + 1]; i++)
-            {
-                cols.clear();
-                for (int64_t ka = A.ptr[i]; ka < A.ptr[i + 1]; ka++)
-                {
-                    const int j = A.idx[ka];
-                    for (int64_t kb = B.ptr[j]; kb < B.ptr[j + 1]; kb++)
-                    {
-                        const int c = B.idx[kb];
-                        if (mark[c] < 0)
-                        {
-                            mark[c] = (int)cols.size();
-                            cols.push_back(c);
-                        }
-                    }
-                }
-                std::sort(cols.begin(), cols.end());
-                for (size_t q = 0; q < cols.size(); q++)
-                    mark[cols[q]] = (int)q;
-                acc.assign(cols.size() * bs, 0.0);
-                for (int64_t ka = A.ptr[i]; ka < A.ptr[i + 1]; ka++)
-                {
-                    const int j = A.idx[ka];
-                    const double *a = &A.val[ka * br * bk];
-                    for (int64_t kb = B.ptr[j]; kb < B.ptr[j + 1]; kb++)
-                    {
-                        const double *b = &B.val[kb * bk * bc];
-                        double *cblk = &acc[(size_t)mark[B.idx[kb]] * bs];
-                        for (int r = 0; r < br; r++)
-                            for (int s = 0; s < bk; s++)
-                            {
-                                const double ars = a[r * bk + s];
-                                if (ars == 0.0)
-                                    continue;
-                                for (int c = 0; c < bc; c++)
-                                    cblk[r * bc + c] += ars * b[s * bc + c];
-                            }
-                    }
-                }
-                rn[t].push_back((int)cols.size());
-                ri[t].insert(ri[t].end(), cols.begin(), cols.end());
-                rv[t].insert(rv[t].end(), acc.begin(), acc.end());
-                for (int c : cols)
-                    mark[c] = -1;
-            }
-        });
-    for (auto &x : th)
-        x.join();
-    C.ptr.assign(C.nr + 1, 0);
-    int row = 0;
-    for (int t = 0; t < T; t++)
-        for (int len : rn[t])
-        {
-            C.ptr[row + 1] = C.ptr[row] + len;
-            row++;
-        }
-    C.idx.reserve(C.ptr[C.nr]);
-    C.val.reserve(C.ptr[C.nr] * bs);
-    for (int t = 0; t < T; t++)
-    {
-        C.idx.insert(C.idx.end(), ri[t].begin(), ri[t].end());
-        C.val.insert(C.val.end(), rv[t].begin(), rv[t].end());
-    }
-    return C;
-}
-
-HB transpose(const HB &A)
-{
-    HB T;
-    T.nr = A.nc, T.nc = A.nr, T.br = A.bc, T.bc = A.br;
-    const int bs = A.br * A.bc;
-    T.ptr.assign(T.nr + 1, 0);
-    for (int64_t k = 0; k < A.nnzb(); k++)
-        T.ptr[A.idx[k] + 1]++;
-    for (int i = 0; i < T.nr; i++)
-        T.ptr[i + 1] += T.ptr[i];
-    // the place of every block in T (ascending rows of A: the columns of T ascend),
-    // then the copies in parallel
-    std::vector<int64_t> pos(T.ptr.begin(), T.ptr.end() - 1), dst(A.nnzb());
-    for (int64_t k = 0; k < A.nnzb(); k++)
-        dst[k] = pos[A.idx[k]]++;
-    T.idx.resize(A.nnzb());
-    T.val.resize(A.nnzb() * bs);
-    parallel_for(A.nr, [&](int a, int b, int) {
-        for (int i = a; i < b; i++)
-            for (int64_t k = A.ptr[i]; k < A.ptr[i + 1]; k++)
-            {
-                const int64_t d = dst[k];
-                T.idx[d] = i;
-                for (int r = 0; r < A.br; r++)
-                    for (int c = 0; c < A.bc; c++)
-                        T.val[d * bs + c * A.br + r] = A.val[k * bs + r * A.bc + c];
-            }
-    });
-    return T;
-}
-
 // In-place inverse of a small dense matrix (Gauss-Jordan, partial pivoting); a
 // singular matrix gets the pseudo-inverse of its nonzero diagonal part.
 void small_inverse(int d, double *a)
@@ -232,85 +83,6 @@ void small_inverse(int d, double *a)
             }
     }
     memcpy(a, inv.data(), sizeof(double) * d * d);
-}
-
-std::vector<double> diag_inverses(const HB &A)
-{
-    const int d = A.br, bs = d * d;
-    std::vector<double> Dinv((size_t)A.nr * bs, 0.0);
-    parallel_for(A.nr, [&](int a, int b, int) {
-        for (int i = a; i < b; i++)
-        {
-            double *o = &Dinv[(size_t)i * bs];
-            for (int64_t k = A.ptr[i]; k < A.ptr[i + 1]; k++)
-                if (A.idx[k] == i)
-                    memcpy(o, &A.val[k * bs], sizeof(double) * bs);
-            small_inverse(d, o);
-        }
-    });
-    return Dinv;
-}
-
-// y = A x
-void host_mv(const HB &A, const std::vector<double> &x, std::vector<double> &y)
-{
-    y.assign((size_t)A.nr * A.br, 0.0);
-    parallel_for(A.nr, [&](int a, int b, int) {
-        for (int i = a; i < b; i++)
-            for (int64_t k = A.ptr[i]; k < A.ptr[i + 1]; k++)
-            {
-                const double *blk = &A.val[k * A.br * A.bc];
-                const double *xv = &x[(size_t)A.idx[k] * A.bc];
-                for (int r = 0; r < A.br; r++)
-                {
-                    double s = 0;
-                    for (int c = 0; c < A.bc; c++)
-                        s += blk[r * A.bc + c] * xv[c];
-                    y[(size_t)i * A.br + r] += s;
-                }
-            }
-    });
-}
-
-// lambda_max(D^-1 A) by power iteration (A symmetric, D^-1 A similar to symmetric).
-double lambda_max(const HB &A, const std::vector<double> &Dinv, int iters)
-{
-    const int d = A.br;
-    const size_t m = (size_t)A.nr * d;
-    std::vector<double> x(m), y, z(m);
-    uint64_t s = 0x9E3779B97F4A7C15ull;
-    for (size_t i = 0; i < m; i++)
-    {
-        s ^= s << 13, s ^= s >> 7, s ^= s << 17;
-        x[i] = (double)(s >> 11) * (1.0 / 9007199254740992.0) - 0.5;
-    }
-    double lam = 0.0;
-    for (int it = 0; it < iters; it++)
-    {
-        double nx = 0;
-        for (double v : x)
-            nx += v * v;
-        nx = sqrt(nx);
-        if (!(nx > 0))
-            return 1.0;
-        for (double &v : x)
-            v /= nx;
-        host_mv(A, x, y);
-        for (int i = 0; i < A.nr; i++)
-            for (int r = 0; r < d; r++)
-            {
-                double t = 0;
-                for (int c = 0; c < d; c++)
-                    t += Dinv[(size_t)i * d * d + r * d + c] * y[(size_t)i * d + c];
-                z[(size_t)i * d + r] = t;
-            }
-        double nz = 0;
-        for (double v : z)
-            nz += v * v;
-        lam = sqrt(nz);
-        x.swap(z);
-    }
-    return lam;
 }
 
 // Greedy aggregation over an adjacency list (ascending neighbours, self excluded).
@@ -354,112 +126,6 @@ std::vector<int> aggregate(int n, const std::vector<int64_t> &aptr, const std::v
         }
     *nagg = na;
     return agg;
-}
-
-// Tentative prolongator: B (fine null space, [nr*d][nm] row-major) restricted to each
-// aggregate and orthonormalised; Bc receives the coarse null space [na*nm][nm].
-HB tentative(const std::vector<int> &agg, int na, int d, int nm, const std::vector<double> &B,
-             std::vector<double> &Bc)
-{
-    const int n = (int)agg.size();
-    std::vector<std::vector<int>> members(na);
-    for (int i = 0; i < n; i++)
-        members[agg[i]].push_back(i);
-    HB P;
-    P.nr = n, P.nc = na, P.br = d, P.bc = nm;
-    P.ptr.resize(n + 1);
-    for (int i = 0; i <= n; i++)
-        P.ptr[i] = i;
-    P.idx.assign(agg.begin(), agg.end());
-    P.val.assign((size_t)n * d * nm, 0.0);
-    Bc.assign((size_t)na * nm * nm, 0.0);
-    parallel_for(na, [&](int a0, int a1, int) {
-        std::vector<double> Q;
-        for (int a = a0; a < a1; a++)
-        {
-            const auto &mem = members[a];
-            const int rows = (int)mem.size() * d;
-            Q.assign((size_t)rows * nm, 0.0);
-            for (size_t q = 0; q < mem.size(); q++)
-                for (int r = 0; r < d; r++)
-                    for (int c = 0; c < nm; c++)
-                        Q[(q * d + r) * nm + c] = B[((size_t)mem[q] * d + r) * nm + c];
-            double *R = &Bc[(size_t)a * nm * nm];
-            double ref = 0.0;
-            for (int c = 0; c < nm; c++)
-            {
-                double s = 0;
-                for (int r = 0; r < rows; r++)
-                    s += Q[(size_t)r * nm + c] * Q[(size_t)r * nm + c];
-                ref = std::max(ref, sqrt(s));
-            }
-            for (int c = 0; c < nm; c++)
-            {
-                for (int p = 0; p < c; p++)
-                {
-                    double s = 0;
-                    for (int r = 0; r < rows; r++)
-                        s += Q[(size_t)r * nm + p] * Q[(size_t)r * nm + c];
-                    R[p * nm + c] = s;
-                    for (int r = 0; r < rows; r++)
-                        Q[(size_t)r * nm + c] -= s * Q[(size_t)r * nm + p];
-                }
-                double s = 0;
-                for (int r = 0; r < rows; r++)
-                    s += Q[(size_t)r * nm + c] * Q[(size_t)r * nm + c];
-                s = sqrt(s);
-                if (s > 1e-10 * ref && ref > 0)
-                {
-                    R[c * nm + c] = s;
-                    for (int r = 0; r < rows; r++)
-                        Q[(size_t)r * nm + c] /= s;
-                }
-                else
-                {
-                    R[c * nm + c] = 0.0;
-                    for (int r = 0; r < rows; r++)
-                        Q[(size_t)r * nm + c] = 0.0;
-                }
-            }
-            for (size_t q = 0; q < mem.size(); q++)
-                for (int r = 0; r < d; r++)
-                    for (int c = 0; c < nm; c++)
-                        P.val[((size_t)mem[q] * d + r) * nm + c] = Q[(q * d + r) * nm + c];
-        }
-    });
-    return P;
-}
-
-// P = P0 - w D^-1 (A P0)
-HB smooth_prolongator(const HB &A, const std::vector<double> &Dinv, const HB &P0, double w)
-{
-    HB AP = spgemm(A, P0);
-    const int d = A.br, nm = P0.bc;
-    // the result has AP's pattern (which contains P0's: A has its diagonal)
-    parallel_for(AP.nr, [&](int a, int b, int) {
-        std::vector<double> t(d * nm);
-        for (int i = a; i < b; i++)
-        {
-            const double *Di = &Dinv[(size_t)i * d * d];
-            for (int64_t k = AP.ptr[i]; k < AP.ptr[i + 1]; k++)
-            {
-                double *blk = &AP.val[k * d * nm];
-                for (int r = 0; r < d; r++)
-                    for (int c = 0; c < nm; c++)
-                    {
-                        double s = 0;
-                        for (int q = 0; q < d; q++)
-                            s += Di[r * d + q] * blk[q * nm + c];
-                        t[r * nm + c] = -w * s;
-                    }
-                if (AP.idx[k] == P0.idx[i])
-                    for (int e = 0; e < d * nm; e++)
-                        t[e] += P0.val[(size_t)i * d * nm + e];
-                memcpy(blk, t.data(), sizeof(double) * d * nm);
-            }
-        }
-    });
-    return AP;
 }
 
 // Graph of the aggregates: a and b are adjacent when a member of a has a neighbour in
@@ -591,89 +257,6 @@ void lpmx_amg_free(void)
     AM.degree = deg, AM.cdegree = cdeg, AM.ratio = ratio, AM.post = post;
 }
 
-template <typename T>
-static int upload(T *dst, const std::vector<T> &src)
-{
-    RETCHK(h2d(dst, src.data(), sizeof(T) * src.size()));
-    return LPMX_OK;
-}
-
-static int upload_vals(const std::vector<double> &val, Mat &B)
-{
-    std::vector<float> v32(val.size());
-    const int64_t nv = (int64_t)val.size(), chunk = 1 << 16;
-    parallel_for((int)((nv + chunk - 1) / chunk), [&](int a, int b, int) {
-        for (int64_t q = (int64_t)a * chunk; q < std::min(nv, (int64_t)b * chunk); q++)
-            v32[q] = (float)val[q];
-    });
-    RETCHK(amg_alloc(&B.val, (int64_t)val.size()));
-    RETCHK(amg_alloc(&B.val32, (int64_t)v32.size()));
-    RETCHK(upload(B.val, val));
-    RETCHK(upload(B.val32, v32));
-    return LPMX_OK;
-}
-
-static int upload_mat(const HB &H, Mat &B)
-{
-    B.nr = H.nr, B.nc = H.nc, B.br = H.br, B.bc = H.bc, B.nnzb = H.nnzb();
-    const int bs = H.br * H.bc;
-    if (H.nr < SLICE_MIN_ROWS)
-    {
-        B.layout = LAYOUT_WROW;
-        std::vector<double> val((size_t)B.nnzb * bs);
-        parallel_for(H.nr, [&](int a, int b, int) {
-            for (int i = a; i < b; i++)
-            {
-                const int64_t k0 = H.ptr[i], len = H.ptr[i + 1] - k0;
-                for (int64_t j = 0; j < len; j++)
-                    for (int e = 0; e < bs; e++)
-                        val[(size_t)(k0 * bs + e * len + j)] = H.val[(size_t)((k0 + j) * bs + e)];
-            }
-        });
-        RETCHK(amg_alloc(&B.rptr, (int64_t)H.nr + 1));
-        RETCHK(amg_alloc(&B.col, B.nnzb));
-        RETCHK(upload(B.rptr, H.ptr));
-        RETCHK(upload(B.col, H.idx));
-        return upload_vals(val, B);
-    }
-    B.layout = LAYOUT_SLICE;
-    B.nslices = (H.nr + 63) / 64;
-    std::vector<int> sptr(B.nslices + 1, 0);
-    for (int s = 0; s < B.nslices; s++)
-    {
-        int w = 0;
-        for (int l = 0; l < 64 && s * 64 + l < H.nr; l++)
-        {
-            const int i = s * 64 + l;
-            w = std::max<int>(w, (int)(H.ptr[i + 1] - H.ptr[i]));
-        }
-        sptr[s + 1] = sptr[s] + w;
-    }
-    B.nslots = sptr[B.nslices];
-    std::vector<int> col((size_t)B.nslots * 64, 0);
-    std::vector<double> val((size_t)B.nslots * bs * 64, 0.0);
-    parallel_for(H.nr, [&](int a, int b, int) { // rows write disjoint lanes
-        for (int i = a; i < b; i++)
-        {
-            const int s = i / 64, l = i % 64;
-            int k = 0;
-            for (int64_t q = H.ptr[i]; q < H.ptr[i + 1]; q++, k++)
-            {
-                const int64_t slot = sptr[s] + k;
-                col[slot * 64 + l] = H.idx[q];
-                for (int e = 0; e < bs; e++)
-                    val[(slot * bs + e) * 64 + l] = H.val[q * bs + e];
-            }
-            for (; sptr[s] + k < sptr[s + 1]; k++) // padding: a zero block at a valid column
-                col[(int64_t)(sptr[s] + k) * 64 + l] = H.nr == H.nc ? i : 0;
-        }
-    });
-    RETCHK(amg_alloc(&B.sptr, B.nslices + 1));
-    RETCHK(amg_alloc(&B.col, (int64_t)col.size()));
-    RETCHK(upload(B.sptr, sptr));
-    RETCHK(upload(B.col, col));
-    return upload_vals(val, B);
-}
 
 static int alloc_vectors(Level &v)
 {
@@ -740,14 +323,6 @@ static int mask_signature(uint64_t *sig)
     return LPMX_OK;
 }
 
-static int read_masks(std::vector<int> &db, std::vector<int> &fx)
-{
-    const int64_t m = (int64_t)g.n * g.dim;
-    db.resize(m), fx.resize(m);
-    RETCHK(d2h(db.data(), g.dispBC, sizeof(int) * m));
-    RETCHK(d2h(fx.data(), g.fix, sizeof(int) * m));
-    return LPMX_OK;
-}
 
 // Whether the hierarchy fits the current model and constraint set.
 bool lpmx_amg_current(void)
@@ -763,255 +338,6 @@ bool lpmx_amg_current(void)
 double lpmx_amg_setup_seconds(void) { return AM.t_setup; }
 int lpmx_amg_levels(void) { return AM.ready ? AM.nlev + 1 : 0; }
 
-// Build the hierarchy from the current (constraint-eliminated) tangent.
-int lpmx_amg_build(void)
-{
-    RETCHK(lpmx_require_pattern());
-    const auto t0 = std::chrono::steady_clock::now();
-    lpmx_amg_free();
-    if (const char *e = getenv("LPMX_AMG_DEGREE"))
-        AM.degree = std::max(1, std::min(6, atoi(e)));
-    if (const char *e = getenv("LPMX_AMG_CDEGREE"))
-        AM.cdegree = std::max(1, std::min(6, atoi(e)));
-    if (const char *e = getenv("LPMX_AMG_POST"))
-        AM.post = atoi(e) != 0;
-    if (const char *e = getenv("LPMX_AMG_RATIO"))
-        AM.ratio = std::max(2.0, atof(e));
-    const int n = g.n, D = g.dim, D2 = D * D, nm = D == 3 ? 6 : 3;
-    const bool progress = getenv("LPMX_PROGRESS") != nullptr;
-    auto phase = [&](const char *what) {
-        if (progress)
-            fprintf(stderr, "[lpmx] AMG setup: %s at %.2f s\n", what,
-                    std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
-    };
-    // the fine tangent in block CSR
-    std::vector<int> sptr(g.nslices + 1), col((size_t)g.nslots * 64), rowlen(n), nbr((size_t)n * g.nb), nb0(n);
-    std::vector<double> K((size_t)g.nslots * D2 * 64), x((size_t)3 * n);
-    RETCHK(d2h(sptr.data(), g.sptr, sizeof(int) * sptr.size()));
-    RETCHK(d2h(col.data(), g.col, sizeof(int) * col.size()));
-    RETCHK(d2h(K.data(), g.K, sizeof(double) * K.size()));
-    RETCHK(d2h(rowlen.data(), g.nb_conn, sizeof(int) * n));
-    RETCHK(d2h(nbr.data(), g.nbr, sizeof(int) * nbr.size()));
-    RETCHK(d2h(nb0.data(), g.nb0, sizeof(int) * n));
-    RETCHK(d2h(x.data(), g.x, sizeof(double) * x.size()));
-    std::vector<int> db, fx;
-    RETCHK(read_masks(db, fx));
-    HB A;
-    A.nr = A.nc = n, A.br = A.bc = D;
-    A.ptr.assign(n + 1, 0);
-    for (int i = 0; i < n; i++)
-        A.ptr[i + 1] = A.ptr[i] + rowlen[i];
-    A.idx.resize(A.ptr[n]);
-    A.val.resize(A.ptr[n] * D2);
-    parallel_for(n, [&](int a, int b, int) {
-        std::vector<std::pair<int, int>> ord;
-        for (int i = a; i < b; i++)
-        {
-            const int s = i / 64, l = i % 64;
-            ord.clear();
-            for (int k = 0; k < rowlen[i]; k++)
-                ord.push_back({col[(size_t)(sptr[s] + k) * 64 + l], k});
-            std::sort(ord.begin(), ord.end());
-            for (int q = 0; q < rowlen[i]; q++)
-            {
-                const int64_t slot = sptr[s] + ord[q].second, o = A.ptr[i] + q;
-                A.idx[o] = ord[q].first;
-                for (int e = 0; e < D2; e++)
-                    A.val[o * D2 + e] = K[(slot * D2 + e) * 64 + l];
-            }
-        }
-    });
-    std::vector<double>().swap(K);
-    // fine null space: rigid-body modes, zero on constrained DOFs (centred later)
-    std::vector<double> B((size_t)n * D * nm, 0.0);
-    // bond graph (ascending, self excluded)
-    std::vector<int64_t> aptr(n + 1, 0);
-    std::vector<int> adj;
-    {
-        // per row the sorted distinct partners (at most nb), then the prefix and the fill
-        const int w = g.nb;
-        std::vector<int> rows((size_t)n * w), cnt(n);
-        parallel_for(n, [&](int a, int b, int) {
-            for (int i = a; i < b; i++)
-            {
-                int *o = &rows[(size_t)i * w];
-                int m = 0;
-                for (int j = 0; j < nb0[i]; j++)
-                {
-                    const int q = nbr[(size_t)j * n + i];
-                    if (q >= 0 && q != i)
-                        o[m++] = q;
-                }
-                std::sort(o, o + m);
-                cnt[i] = (int)(std::unique(o, o + m) - o);
-            }
-        });
-        for (int i = 0; i < n; i++)
-            aptr[i + 1] = aptr[i] + cnt[i];
-        adj.resize(aptr[n]);
-        parallel_for(n, [&](int a, int b, int) {
-            for (int i = a; i < b; i++)
-                std::copy(&rows[(size_t)i * w], &rows[(size_t)i * w] + cnt[i], adj.begin() + aptr[i]);
-        });
-    }
-    int na = 0;
-    phase("fine matrix copied");
-    std::vector<int> agg = aggregate(n, aptr, adj, &na);
-    {
-        std::vector<double> cx((size_t)na * 3, 0.0);
-        std::vector<int> cnt(na, 0);
-        for (int i = 0; i < n; i++)
-        {
-            for (int c = 0; c < 3; c++)
-                cx[(size_t)agg[i] * 3 + c] += x[(size_t)3 * i + c];
-            cnt[agg[i]]++;
-        }
-        for (int a = 0; a < na; a++)
-            for (int c = 0; c < 3; c++)
-                cx[(size_t)a * 3 + c] /= cnt[a];
-        for (int i = 0; i < n; i++)
-        {
-            const double px = x[3 * i] - cx[(size_t)agg[i] * 3], py = x[3 * i + 1] - cx[(size_t)agg[i] * 3 + 1],
-                         pz = x[3 * i + 2] - cx[(size_t)agg[i] * 3 + 2];
-            double *b = &B[(size_t)i * D * nm];
-            if (D == 3)
-            {
-                const double m6[3][6] = {{1, 0, 0, 0, pz, -py}, {0, 1, 0, -pz, 0, px}, {0, 0, 1, py, -px, 0}};
-                for (int r = 0; r < 3; r++)
-                    for (int c = 0; c < 6; c++)
-                        b[r * 6 + c] = m6[r][c];
-            }
-            else
-            {
-                const double m3[2][3] = {{1, 0, -py}, {0, 1, px}};
-                for (int r = 0; r < 2; r++)
-                    for (int c = 0; c < 3; c++)
-                        b[r * 3 + c] = m3[r][c];
-            }
-            for (int r = 0; r < D; r++)
-                if (!(db[(size_t)i * D + r] && fx[(size_t)i * D + r]))
-                    for (int c = 0; c < nm; c++)
-                        b[r * nm + c] = 0.0;
-        }
-    }
-    // levels
-    std::vector<HB> As, Ps, Rs;
-    std::vector<std::vector<double>> Dinvs;
-    std::vector<double> lm;
-    As.push_back(std::move(A));
-    const int max_levels = 8, dense_max = 1200;
-    for (int lev = 0; lev < max_levels; lev++)
-    {
-        HB &Al = As[lev];
-        std::vector<double> Dinv = diag_inverses(Al);
-        const double lmax = lambda_max(Al, Dinv, 20);
-        lm.push_back(lmax);
-        phase("diagonal inverses and spectral bound");
-        if ((int64_t)Al.nr * Al.br <= dense_max || lev == max_levels - 1)
-        {
-            Dinvs.push_back(std::move(Dinv));
-            break;
-        }
-        if (lev > 0)
-            agg = aggregate(Al.nr, aptr, adj, &na);
-        if (na >= Al.nr) // no coarsening possible
-        {
-            Dinvs.push_back(std::move(Dinv));
-            break;
-        }
-        std::vector<double> Bc;
-        HB P0 = tentative(agg, na, Al.br, nm, B, Bc);
-        phase("aggregates and tentative prolongator");
-        HB P = smooth_prolongator(Al, Dinv, P0, 4.0 / (3.0 * lmax));
-        phase("smoothed prolongator");
-        HB R = transpose(P);
-        HB AP = spgemm(Al, P);
-        phase("A P");
-        HB Ac = spgemm(R, AP);
-        Dinvs.push_back(std::move(Dinv));
-        Ps.push_back(std::move(P));
-        Rs.push_back(std::move(R));
-        As.push_back(std::move(Ac));
-        B.swap(Bc);
-        quotient_graph(agg, na, aptr, adj);
-        if (progress)
-            fprintf(stderr, "[lpmx] AMG setup: level %d -> %d block rows of %d\n", lev, As.back().nr, As.back().br);
-        phase("Galerkin product done");
-    }
-    // upload: levels 0 .. nlev-1 smoothed, level nlev dense
-    AM.nlev = (int)As.size() - 1;
-    phase("levels built");
-    AM.L.resize(AM.nlev + 1);
-    for (int lev = 0; lev <= AM.nlev; lev++)
-    {
-        Level &v = AM.L[lev];
-        v.n = As[lev].nr, v.d = As[lev].br;
-        v.lmax = lm[lev];
-        if (lev == AM.nlev)
-            break;
-        if (lev > 0)
-        {
-            RETCHK(upload_mat(As[lev], v.A));
-            const int64_t bs = (int64_t)v.d * v.d;
-            std::vector<double> mp((size_t)bs * v.n);
-            std::vector<float> mp32((size_t)bs * v.n);
-            for (int i = 0; i < v.n; i++)
-                for (int e = 0; e < bs; e++)
-                {
-                    mp[(size_t)e * v.n + i] = Dinvs[lev][(size_t)i * bs + e];
-                    mp32[(size_t)e * v.n + i] = (float)mp[(size_t)e * v.n + i];
-                }
-            RETCHK(amg_alloc(&v.Minv, bs * v.n));
-            RETCHK(amg_alloc(&v.Minv32, bs * v.n));
-            RETCHK(h2d(v.Minv, mp.data(), sizeof(double) * mp.size()));
-            RETCHK(h2d(v.Minv32, mp32.data(), sizeof(float) * mp32.size()));
-        }
-        RETCHK(upload_mat(Ps[lev], v.P));
-        RETCHK(upload_mat(Rs[lev], v.R));
-        RETCHK(alloc_vectors(v));
-    }
-    phase("levels uploaded");
-    // dense coarsest level
-    {
-        const HB &Ac = As[AM.nlev];
-        const int m = Ac.nr * Ac.br;
-        if (m > 6000)
-        {
-            lpmx_amg_free();
-            return lpmx_fail(LPMX_ERR_STATE, "AMG coarsening stalled at %d DOFs", m);
-        }
-        std::vector<double> M((size_t)m * m, 0.0);
-        for (int i = 0; i < Ac.nr; i++)
-            for (int64_t k = Ac.ptr[i]; k < Ac.ptr[i + 1]; k++)
-                for (int r = 0; r < Ac.br; r++)
-                    for (int c = 0; c < Ac.bc; c++)
-                        M[(size_t)(i * Ac.br + r) * m + Ac.idx[k] * Ac.bc + c] = Ac.val[k * Ac.br * Ac.bc + r * Ac.bc + c];
-        small_inverse(m, M.data());
-        std::vector<float> M32(M.size());
-        for (size_t q = 0; q < M.size(); q++)
-            M32[q] = (float)M[q];
-        AM.nc_dense = m;
-        RETCHK(amg_alloc(&AM.Ainv, (int64_t)M.size()));
-        RETCHK(amg_alloc(&AM.Ainv32, (int64_t)M.size()));
-        RETCHK(h2d(AM.Ainv, M.data(), sizeof(double) * M.size()));
-        RETCHK(h2d(AM.Ainv32, M32.data(), sizeof(float) * M32.size()));
-        Level &v = AM.L[AM.nlev];
-        RETCHK(alloc_vectors(v));
-    }
-    AM.n_model = n;
-    RETCHK(mask_signature(&AM.bc_hash));
-    AM.ready = true;
-    AM.t_setup = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-    if (getenv("LPMX_PROGRESS"))
-    {
-        fprintf(stderr, "[lpmx] AMG setup %.2f s:", AM.t_setup);
-        for (int lev = 0; lev <= AM.nlev; lev++)
-            fprintf(stderr, " L%d %d x %d (%lld blocks, lmax %.3f)", lev, AM.L[lev].n, AM.L[lev].d,
-                    (long long)As[lev].nnzb(), AM.L[lev].lmax);
-        fprintf(stderr, "\n");
-    }
-    return LPMX_OK;
-}
 
 // ------------------------------------------------------------------ kernels ------
 // y = A x (MODE 0), y = b - A x (1), y += A x (2), y -= A x (3) for the lane's row.
@@ -1453,5 +779,392 @@ int lpmx_amg_apply_f32(const float *r, float *z, const int *done)
         return lpmx_fail(LPMX_ERR_STATE, "AMG hierarchy not built");
     vcycle<float>(0, r, z, done);
     LAUNCHCHK();
+    return LPMX_OK;
+}
+
+// ------------------------------------------------------------------- set-up -----
+int lpmx_bj_inverse(void);
+
+namespace
+{
+// Device temporaries of the set-up, released on every return.
+struct Scratch
+{
+    std::vector<void *> p;
+    template <typename T>
+    int alloc(T **q, int64_t count)
+    {
+        HIPCHK(hipMalloc((void **)q, sizeof(T) * (size_t)std::max<int64_t>(count, 1)));
+        p.push_back(*q);
+        return LPMX_OK;
+    }
+    void release(void *q)
+    {
+        for (void *&x : p)
+            if (q && x == q)
+            {
+                hipFree(x);
+                x = nullptr;
+            }
+    }
+    ~Scratch()
+    {
+        for (void *q : p)
+            if (q)
+                hipFree(q);
+    }
+};
+
+struct OwnedBsr
+{
+    DBsr m;
+    ~OwnedBsr() { amgd_free(m); }
+};
+} // namespace
+
+// A level operator from block CSR into its layout, FP64 and FP32 values.
+static int to_mat(const DBsr &H, Mat &B)
+{
+    B.nr = H.nr, B.nc = H.nc, B.br = H.br, B.bc = H.bc, B.nnzb = H.nnzb;
+    const int64_t bs = (int64_t)H.br * H.bc;
+    int64_t nval = 0;
+    if (H.nr < SLICE_MIN_ROWS)
+    {
+        B.layout = LAYOUT_WROW;
+        nval = (int64_t)H.nnzb * bs;
+        RETCHK(amg_alloc(&B.rptr, (int64_t)H.nr + 1));
+        RETCHK(amg_alloc(&B.col, (int64_t)H.nnzb));
+        RETCHK(amg_alloc(&B.val, nval));
+        RETCHK(amgd_rptr64(H, B.rptr));
+        HIPCHK(hipMemcpyAsync(B.col, H.idx, sizeof(int) * (size_t)H.nnzb, hipMemcpyDeviceToDevice, g_stream));
+        RETCHK(amgd_wrow_vals(H, B.val));
+    }
+    else
+    {
+        B.layout = LAYOUT_SLICE;
+        B.nslices = (H.nr + 63) / 64;
+        RETCHK(amg_alloc(&B.sptr, (int64_t)B.nslices + 1));
+        int ns = 0;
+        RETCHK(amgd_slice_widths(H, B.sptr, &ns));
+        B.nslots = ns;
+        nval = B.nslots * bs * 64;
+        RETCHK(amg_alloc(&B.col, B.nslots * 64));
+        RETCHK(amg_alloc(&B.val, nval));
+        RETCHK(amgd_slice_fill(H, B.sptr, B.nslots, B.col, B.val));
+    }
+    RETCHK(amg_alloc(&B.val32, nval));
+    return amgd_to_f32(B.val, B.val32, nval);
+}
+
+// lambda_max(D^-1 A) of level l: 20 power-iteration steps from a fixed pseudo-random
+// start (xorshift64); D^-1 A is similar to a symmetric matrix.
+static int level_lambda_max(int l, const double *Minv, double *lam)
+{
+    const Level &v = AM.L[l];
+    const int64_t m = (int64_t)v.n * v.d;
+    std::vector<double> x0(m);
+    uint64_t s = 0x9E3779B97F4A7C15ull;
+    for (int64_t i = 0; i < m; i++)
+    {
+        s ^= s << 13, s ^= s >> 7, s ^= s << 17;
+        x0[i] = (double)(s >> 11) * (1.0 / 9007199254740992.0) - 0.5;
+    }
+    Scratch S;
+    double *x = nullptr, *y = nullptr, *z = nullptr;
+    RETCHK(S.alloc(&x, m));
+    RETCHK(S.alloc(&y, m));
+    RETCHK(S.alloc(&z, m));
+    RETCHK(h2d(x, x0.data(), sizeof(double) * m));
+    const Mat A = level_matrix(l);
+    double val = 0.0;
+    for (int it = 0; it < 20; it++)
+    {
+        double nx2 = 0.0;
+        RETCHK(lpmx_dev_sumsq(x, m, &nx2));
+        const double nx = sqrt(nx2);
+        if (!(nx > 0))
+        {
+            val = 1.0;
+            break;
+        }
+        RETCHK(amgd_scale(x, m, nx));
+        launch_mv<double, 0>(A, x, y, nullptr, nullptr);
+        LAUNCHCHK();
+        RETCHK(amgd_block_diag_mv(v.n, v.d, Minv, y, z));
+        double nz2 = 0.0;
+        RETCHK(lpmx_dev_sumsq(z, m, &nz2));
+        val = sqrt(nz2);
+        std::swap(x, z);
+    }
+    HIPCHK(hipStreamSynchronize(g_stream));
+    *lam = val;
+    return LPMX_OK;
+}
+
+// The coarsest level as a dense row-major m x m matrix: from its block CSR, or (a
+// one-level hierarchy) from the model's tangent.
+static int dense_level(const DBsr *H, int m, std::vector<double> &M)
+{
+    M.assign((size_t)m * m, 0.0);
+    if (H)
+    {
+        const int bs = H->br * H->bc;
+        std::vector<int> ptr(H->nr + 1), idx(H->nnzb);
+        std::vector<double> val((size_t)H->nnzb * bs);
+        RETCHK(d2h(ptr.data(), H->ptr, sizeof(int) * ptr.size()));
+        RETCHK(d2h(idx.data(), H->idx, sizeof(int) * idx.size()));
+        RETCHK(d2h(val.data(), H->val, sizeof(double) * val.size()));
+        for (int i = 0; i < H->nr; i++)
+            for (int k = ptr[i]; k < ptr[i + 1]; k++)
+                for (int r = 0; r < H->br; r++)
+                    for (int c = 0; c < H->bc; c++)
+                        M[(size_t)(i * H->br + r) * m + idx[k] * H->bc + c] = val[(size_t)k * bs + r * H->bc + c];
+        return LPMX_OK;
+    }
+    const int n = g.n, D = g.dim, D2 = D * D;
+    std::vector<int> sptr(g.nslices + 1), col((size_t)g.nslots * 64), len(n);
+    std::vector<double> K((size_t)g.nslots * D2 * 64);
+    RETCHK(d2h(sptr.data(), g.sptr, sizeof(int) * sptr.size()));
+    RETCHK(d2h(col.data(), g.col, sizeof(int) * col.size()));
+    RETCHK(d2h(K.data(), g.K, sizeof(double) * K.size()));
+    RETCHK(d2h(len.data(), g.nb_conn, sizeof(int) * n));
+    for (int i = 0; i < n; i++)
+    {
+        const int s = i / 64, l = i % 64;
+        for (int k = 0; k < len[i]; k++)
+        {
+            const int64_t slot = sptr[s] + k;
+            const int c = col[slot * 64 + l];
+            for (int r = 0; r < D; r++)
+                for (int q = 0; q < D; q++)
+                    M[(size_t)(i * D + r) * m + c * D + q] = K[(slot * D2 + r * D + q) * 64 + l];
+        }
+    }
+    return LPMX_OK;
+}
+
+// Build the hierarchy from the current (constraint-eliminated) tangent.
+int lpmx_amg_build(void)
+{
+    RETCHK(lpmx_require_pattern());
+    RETCHK(lpmx_bj_inverse());
+    const auto t0 = std::chrono::steady_clock::now();
+    lpmx_amg_free();
+    if (const char *e = getenv("LPMX_AMG_DEGREE"))
+        AM.degree = std::max(1, std::min(6, atoi(e)));
+    if (const char *e = getenv("LPMX_AMG_CDEGREE"))
+        AM.cdegree = std::max(1, std::min(6, atoi(e)));
+    if (const char *e = getenv("LPMX_AMG_POST"))
+        AM.post = atoi(e) != 0;
+    if (const char *e = getenv("LPMX_AMG_RATIO"))
+        AM.ratio = std::max(2.0, atof(e));
+    const int n = g.n, D = g.dim, nm = D == 3 ? 6 : 3;
+    const bool progress = getenv("LPMX_PROGRESS") != nullptr;
+    auto phase = [&](const char *what) {
+        if (progress)
+            fprintf(stderr, "[lpmx] AMG setup: %s at %.3f s\n", what,
+                    std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
+    };
+    // the bond graph: the aggregation graph of the finest level
+    std::vector<int64_t> aptr;
+    std::vector<int> adj;
+    {
+        int *dp = nullptr, *da = nullptr, nadj = 0;
+        RETCHK(amgd_bond_graph(&dp, &da, &nadj));
+        std::vector<int> p32(n + 1);
+        adj.resize(nadj);
+        int rc = d2h(p32.data(), dp, sizeof(int) * p32.size());
+        if (rc == LPMX_OK)
+            rc = d2h(adj.data(), da, sizeof(int) * adj.size());
+        hipFree(dp);
+        hipFree(da);
+        RETCHK(rc);
+        aptr.assign(p32.begin(), p32.end());
+    }
+    phase("bond graph");
+    const int max_levels = 8, dense_max = 1200;
+    std::vector<int64_t> nnzb;
+    OwnedBsr Acur;            // the operator of the current level (levels > 0)
+    Scratch S;
+    double *Bnull = nullptr;  // null space of the current level, [rows*d][nm]
+    AM.L.reserve(max_levels);
+    int lev = 0;
+    for (;; lev++)
+    {
+        AM.L.emplace_back();
+        Level &v = AM.L[lev];
+        v.n = lev ? Acur.m.nr : n;
+        v.d = lev ? Acur.m.br : D;
+        nnzb.push_back(lev ? (int64_t)Acur.m.nnzb : g.nnzb);
+        if ((int64_t)v.n * v.d <= dense_max || lev == max_levels - 1)
+            break;
+        const double *Minv = g.Minv; // the finest level smooths with the model's inverses
+        if (lev > 0)
+        {
+            RETCHK(to_mat(Acur.m, v.A));
+            const int64_t bs = (int64_t)v.d * v.d;
+            RETCHK(amg_alloc(&v.Minv, bs * v.n));
+            RETCHK(amg_alloc(&v.Minv32, bs * v.n));
+            RETCHK(amgd_diag_inverses(Acur.m, v.Minv, v.Minv32));
+            Minv = v.Minv;
+        }
+        RETCHK(level_lambda_max(lev, Minv, &v.lmax));
+        phase("diagonal inverses and spectral bound");
+        int na = 0;
+        const std::vector<int> agg = aggregate(v.n, aptr, adj, &na);
+        if (na >= v.n) // no coarsening possible: this level is solved densely
+            break;
+        // members of every aggregate, ascending
+        std::vector<int> mptr(na + 1, 0), mem(v.n);
+        for (int i = 0; i < v.n; i++)
+            mptr[agg[i] + 1]++;
+        for (int a = 0; a < na; a++)
+            mptr[a + 1] += mptr[a];
+        {
+            std::vector<int> pos(mptr.begin(), mptr.end() - 1);
+            for (int i = 0; i < v.n; i++)
+                mem[pos[agg[i]]++] = i;
+        }
+        Scratch T;
+        int *dagg = nullptr, *dmptr = nullptr, *dmem = nullptr;
+        RETCHK(T.alloc(&dagg, v.n));
+        RETCHK(T.alloc(&dmptr, (int64_t)na + 1));
+        RETCHK(T.alloc(&dmem, v.n));
+        RETCHK(h2d(dagg, agg.data(), sizeof(int) * agg.size()));
+        RETCHK(h2d(dmptr, mptr.data(), sizeof(int) * mptr.size()));
+        RETCHK(h2d(dmem, mem.data(), sizeof(int) * mem.size()));
+        if (lev == 0)
+        {
+            RETCHK(S.alloc(&Bnull, (int64_t)n * D * nm));
+            RETCHK(amgd_null_space(n, D, nm, na, dagg, dmptr, dmem, Bnull));
+        }
+        double *Bc = nullptr;
+        RETCHK(S.alloc(&Bc, (int64_t)na * nm * nm));
+        OwnedBsr P0, P, R, AP, Ac;
+        RETCHK(amgd_tentative(v.n, na, v.d, nm, dagg, dmptr, dmem, Bnull, &P0.m, Bc));
+        HIPCHK(hipStreamSynchronize(g_stream));
+        S.release(Bnull);
+        Bnull = Bc;
+        phase("aggregates and tentative prolongator");
+        const AmgLeft Al = lev ? amgd_left(Acur.m) : amgd_left_tangent();
+        RETCHK(amgd_spgemm(Al, P0.m, &P.m));
+        RETCHK(amgd_smooth(P.m, Minv, dagg, P0.m, 4.0 / (3.0 * v.lmax)));
+        HIPCHK(hipStreamSynchronize(g_stream));
+        amgd_free(P0.m);
+        phase("smoothed prolongator");
+        RETCHK(amgd_transpose(P.m, &R.m));
+        RETCHK(amgd_spgemm(Al, P.m, &AP.m));
+        HIPCHK(hipStreamSynchronize(g_stream));
+        phase("A P");
+        RETCHK(amgd_spgemm(amgd_left(R.m), AP.m, &Ac.m));
+        RETCHK(to_mat(P.m, v.P));
+        RETCHK(to_mat(R.m, v.R));
+        RETCHK(alloc_vectors(v));
+        HIPCHK(hipStreamSynchronize(g_stream));
+        amgd_free(Acur.m);
+        Acur.m = Ac.m;
+        Ac.m = DBsr();
+        quotient_graph(agg, na, aptr, adj);
+        if (progress)
+            fprintf(stderr, "[lpmx] AMG setup: level %d -> %d block rows of %d\n", lev, Acur.m.nr, Acur.m.br);
+        phase("Galerkin product done");
+    }
+    AM.nlev = lev;
+    phase("levels built");
+    // dense coarsest level
+    {
+        Level &v = AM.L[lev];
+        const int m = v.n * v.d;
+        if (m > 6000)
+        {
+            lpmx_amg_free();
+            return lpmx_fail(LPMX_ERR_STATE, "AMG coarsening stalled at %d DOFs", m);
+        }
+        std::vector<double> M;
+        RETCHK(dense_level(lev ? &Acur.m : nullptr, m, M));
+        small_inverse(m, M.data());
+        std::vector<float> M32(M.size());
+        for (size_t q = 0; q < M.size(); q++)
+            M32[q] = (float)M[q];
+        AM.nc_dense = m;
+        RETCHK(amg_alloc(&AM.Ainv, (int64_t)M.size()));
+        RETCHK(amg_alloc(&AM.Ainv32, (int64_t)M.size()));
+        RETCHK(h2d(AM.Ainv, M.data(), sizeof(double) * M.size()));
+        RETCHK(h2d(AM.Ainv32, M32.data(), sizeof(float) * M32.size()));
+        RETCHK(alloc_vectors(v));
+    }
+    AM.n_model = n;
+    RETCHK(mask_signature(&AM.bc_hash));
+    AM.ready = true;
+    AM.t_setup = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (progress)
+    {
+        fprintf(stderr, "[lpmx] AMG setup %.3f s:", AM.t_setup);
+        for (int l = 0; l <= AM.nlev; l++)
+            fprintf(stderr, " L%d %d x %d (%lld blocks, lmax %.3f)", l, AM.L[l].n, AM.L[l].d, (long long)nnzb[l],
+                    AM.L[l].lmax);
+        fprintf(stderr, "\n");
+    }
+    return LPMX_OK;
+}
+
+// Test access to the stored blocks of a level (block COO, rows ascending).
+extern "C" int lpmx_amg_export(int level, int which, int64_t cap, int *rows, int *cols, double *vals, int *dims)
+{
+    if (!AM.ready)
+        return lpmx_fail(LPMX_ERR_STATE, "AMG hierarchy not built");
+    if (level < 0 || level >= AM.nlev || which < 0 || which > 2 || (which == 0 && level == 0) || !dims)
+        return lpmx_fail(LPMX_ERR_ARG, "no AMG matrix %d on level %d (levels 0..%d)", which, level, AM.nlev - 1);
+    const Mat &B = which == 0 ? AM.L[level].A : which == 1 ? AM.L[level].P : AM.L[level].R;
+    const int bs = B.br * B.bc;
+    std::vector<int> ri, ci;
+    std::vector<double> vv;
+    if (B.layout == LAYOUT_WROW)
+    {
+        std::vector<int64_t> rptr(B.nr + 1);
+        std::vector<int> col(B.nnzb);
+        std::vector<double> val((size_t)B.nnzb * bs);
+        RETCHK(d2h(rptr.data(), B.rptr, sizeof(int64_t) * rptr.size()));
+        RETCHK(d2h(col.data(), B.col, sizeof(int) * col.size()));
+        RETCHK(d2h(val.data(), B.val, sizeof(double) * val.size()));
+        for (int i = 0; i < B.nr; i++)
+        {
+            const int64_t k0 = rptr[i], len = rptr[i + 1] - k0;
+            for (int64_t j = 0; j < len; j++)
+            {
+                ri.push_back(i), ci.push_back(col[k0 + j]);
+                for (int e = 0; e < bs; e++)
+                    vv.push_back(val[(size_t)(k0 * bs + e * len + j)]);
+            }
+        }
+    }
+    else
+    {
+        std::vector<int> sptr(B.nslices + 1), col((size_t)B.nslots * 64);
+        std::vector<double> val((size_t)B.nslots * bs * 64);
+        RETCHK(d2h(sptr.data(), B.sptr, sizeof(int) * sptr.size()));
+        RETCHK(d2h(col.data(), B.col, sizeof(int) * col.size()));
+        RETCHK(d2h(val.data(), B.val, sizeof(double) * val.size()));
+        for (int i = 0; i < B.nr; i++)
+        {
+            const int s = i / 64, l = i % 64;
+            for (int64_t slot = sptr[s]; slot < sptr[s + 1]; slot++)
+            {
+                ri.push_back(i), ci.push_back(col[slot * 64 + l]);
+                for (int e = 0; e < bs; e++)
+                    vv.push_back(val[(slot * bs + e) * 64 + l]);
+            }
+        }
+    }
+    const int64_t count = (int64_t)ri.size();
+    dims[0] = B.nr, dims[1] = B.nc, dims[2] = B.br, dims[3] = B.bc, dims[4] = (int)count;
+    if (!rows)
+        return LPMX_OK;
+    if (cap < count || !cols || !vals)
+        return lpmx_fail(LPMX_ERR_ARG, "export buffers hold %lld blocks, %lld needed", (long long)cap,
+                         (long long)count);
+    memcpy(rows, ri.data(), sizeof(int) * count);
+    memcpy(cols, ci.data(), sizeof(int) * count);
+    memcpy(vals, vv.data(), sizeof(double) * vv.size());
     return LPMX_OK;
 }
