@@ -31,9 +31,10 @@ J = sum_n (R_n - T_n)^2 against the reaction curve T of a perturbed parameter se
 the yield stress, H and the 216,000-entry modulus field by the discrete adjoint
 through the whole history with a checkpoint every 5 steps.  Its line reports the
 forward time, the time of the reverse sweep (recomputation included) and their
-ratio, and, with --adjoint-fd, central differences (h = 1e-6 relative) of J along
-any of E, H, the yield stress and a random direction of the modulus field (each the
-logarithmic derivative d J / d log theta, compared with the adjoint's).
+ratio, and, with --adjoint-fd, central differences (relative step --adjoint-fd-h,
+default 1e-6) of J along any of E, H, the yield stress and a random direction of the
+modulus field (each the logarithmic derivative d J / d log theta, compared with the
+adjoint's) under `fd_check`.
 
 `--workload brittle` runs the 2D elastic-brittle shear: hexagonal 1000 x 1000 =
 1,000,000 particles, E 70 GPa, nu 0.25 (plane strain), per-bond critical stretch from
@@ -621,11 +622,8 @@ def adjoint_workload(P, a, emit=True):
         "dJ_demod_norm": float(np.linalg.norm(ge)),
         "plastic_particles": int((P.field("PL_FLAG", n0) > 0).sum()),
     }
-    if emit:
-        print(json.dumps(line), flush=True)
-    return line
     if a.adjoint_fd:
-        h = 1e-6
+        h = a.adjoint_fd_h
         rng = np.random.default_rng(3)
         d1 = rng.standard_normal(n0)
 
@@ -651,7 +649,10 @@ def adjoint_workload(P, a, emit=True):
                 continue
             fd[name] = {"fd": f, "adjoint": adj, "rel_err": abs(f - adj) / max(abs(adj), 1e-300)}
             log(f"adjoint: {name}: FD {f:.10e} adjoint {adj:.10e}")
-        print(json.dumps({"fd_check_h_rel_1e-6": fd}), flush=True)
+        line["fd_check"] = {"h_rel": h, "directions": fd}
+    if emit:
+        print(json.dumps(line), flush=True)
+    return line
 
 
 def main():
@@ -697,6 +698,8 @@ def main():
     ap.add_argument("--adjoint-fd", default="",
                     help="adjoint workload: central-difference checks along these of E,H,sigmay,emod_dir "
                          "(comma-separated; two forward histories each)")
+    ap.add_argument("--adjoint-fd-h", type=float, default=1e-6,
+                    help="adjoint workload: relative step of the central differences")
     ap.add_argument("--adjoint-target", default="",
                     help="adjoint workload: the target reaction curve (comma-separated, one value per step) "
                          "as printed by --adjoint-target-only, instead of computing it in the same run")

@@ -105,6 +105,8 @@ int lpmx_law5(void);
 int lpmx_damage_variant(int law, int *count);
 void lpmx_laws_free(void);
 int lpmx_require_pattern(void);
+// changes with every neighbour-list or pattern build (for caches of pattern data)
+int lpmx_pattern_version(void);
 static inline int nblocks(int64_t n, int bs = LPMX_BLOCK) { return (int)((n + bs - 1) / bs); }
 
 // -------------------------------------------------------- deterministic sums -----

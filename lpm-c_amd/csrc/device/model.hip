@@ -15,6 +15,8 @@ hipStream_t g_stream = nullptr;
 
 static char g_err[512] = "";
 static bool g_have_model = false, g_have_pattern = false, g_have_nbr = false;
+static int g_pattern_version = 0; // bumped by every neighbour or pattern build
+int lpmx_pattern_version(void) { return g_pattern_version; }
 static double g_lo[3], g_hi[3];
 static double *g_scratch = nullptr;
 static int64_t g_scratch_bytes = 0;
@@ -832,6 +834,7 @@ extern "C" int lpmx_neighbor_build(void)
         return lpmx_fail(LPMX_ERR_CAPACITY, "a neighbour list exceeds nneighbors/nneighbors1/nneighbors2");
     g_have_nbr = true;
     g_have_pattern = false;
+    g_pattern_version++;
     return LPMX_OK;
 }
 
@@ -1064,6 +1067,7 @@ extern "C" int lpmx_connectivity_build(int64_t *len_K)
     if (len_K)
         *len_K = total;
     g_have_pattern = true;
+    g_pattern_version++;
     return LPMX_OK;
 }
 

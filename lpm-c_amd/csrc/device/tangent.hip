@@ -5,7 +5,13 @@
 
 #include "common.h"
 
-static double *t_S = nullptr;   // per-particle shell sums of the elastic law, TS_NPLANE planes of n
+static double *t_S = nullptr;   // per-particle shell sums of the elastic law, [n][TS_NPLANE]
+static void *t_rec = nullptr;   // bond records, [n][nb] TBond then [n][nb] TBondOwn
+static int64_t t_rec_bytes = 0;
+static uint16_t *t_pstart = nullptr; // two-hop paths per conn slot: starts [n][LPMX_MAXCONN + 1]
+static uint16_t *t_plist = nullptr;  // and the paths (a << 5 | t) of every row, [n][nb * nb]
+static int t_paths_version = -1;     // pattern version the path lists belong to
+static int64_t t_paths_cap = 0;      // entries of the one allocation holding both
 static int *t_kstart = nullptr; // K_pointer[i][1] (0-based starts), n+1
 static double t_nd = 0.0;       // diagonal value of the constrained DOFs
 static bool t_nd_valid = false; // computed since the last tangent assembly
@@ -13,17 +19,33 @@ static bool t_nd_valid = false; // computed since the last tangent assembly
 // import, elimination): the block-Jacobi inverse is formed once per version
 static int t_kver = 0, t_bj_kver = -1;
 
-// Planes of t_S: A_s = sum dL, B_s = sum Tv dL (s = 0, 1), V_s = sum db n and
-// W_s = sum db Tv n (3 components each) over the particle's shell-s bonds.
+// Entries of a row of t_S: A_s = sum dL, B_s = sum Tv dL (s = 0, 1), V_s = sum db n
+// and W_s = sum db Tv n (3 components each) over the particle's shell-s bonds.
 #define TS_A 0
 #define TS_B 2
 #define TS_V 4
 #define TS_W 10
 #define TS_NPLANE 16
 
-// Shell sums of every particle under the elastic law (constitutive.c:241-261):
-// dL = (|x_i - x_j| - L0 - dLp[0]) * damage_broken, n = (x_i - x_j)/|x_i - x_j|.
-__global__ void k_tangent_sums(DV v, double *S)
+// A bond (p, t) as the row kernel reads it: partner l (-1 for an empty slot), shell s,
+// damage factors and Tv of the bond and of its reverse (l, rev), the unit vector
+// (x_p - x_l) / |x_p - x_l|: one 64-byte line.  TBondOwn: Kn and dL of the bond and
+// of its reverse, and the distance (read for the row particle's own bonds only).
+struct TBond
+{
+    int l, s;
+    double db, tv, dbr, tvr, n[3];
+};
+struct TBondOwn
+{
+    double kn, dl, knr, dlr, d, pad;
+};
+
+// Shell sums and bond records of every particle under the elastic law
+// (constitutive.c:241-261): dL = (|x_i - x_j| - L0 - dLp[0]) * damage_broken,
+// n = (x_i - x_j)/|x_i - x_j|.  The records turn the slot planes into rows, so a
+// neighbour's bonds are one contiguous read for the row kernel.
+__global__ void k_tangent_sums(DV v, double *__restrict__ S, TBond *__restrict__ rec, TBondOwn *__restrict__ own)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= v.n)
@@ -31,7 +53,10 @@ __global__ void k_tangent_sums(DV v, double *S)
     const int64_t n = v.n;
     double a[2] = {0, 0}, bb[2] = {0, 0}, V[2][3] = {{0, 0, 0}, {0, 0, 0}}, W[2][3] = {{0, 0, 0}, {0, 0, 0}};
     const double xi = v.x[3 * i], yi = v.x[3 * i + 1], zi = v.x[3 * i + 2];
-    for (int e = 0; e < v.nb0[i]; e++)
+    TBond *ri = rec + (int64_t)i * v.nb;
+    TBondOwn *oi = own + (int64_t)i * v.nb;
+    const int nbi = v.nb0[i];
+    for (int e = 0; e < nbi; e++)
     {
         const int64_t b = (int64_t)e * n + i;
         const int j = v.nbr[b], s = v.nsign[b];
@@ -49,13 +74,25 @@ __global__ void k_tangent_sums(DV v, double *S)
                 for (int c = 0; c < 3; c++)
                     V[q][c] += db * nv[c], W[q][c] += db * tv * nv[c];
             }
+        const int64_t br = (int64_t)v.rev[b] * n + j;
+        const double dbr = v.db[br];
+        TBond r;
+        r.l = j, r.s = s, r.db = db, r.tv = tv, r.dbr = dbr, r.tvr = v.Tv[br];
+        r.n[0] = nv[0], r.n[1] = nv[1], r.n[2] = nv[2];
+        ri[e] = r;
+        TBondOwn o;
+        o.kn = v.Kn[b], o.dl = dl, o.knr = v.Kn[br], o.dlr = (dis - v.L0[br] - v.dLp0[br]) * dbr, o.d = dis, o.pad = 0;
+        oi[e] = o;
     }
+    for (int e = nbi; e < v.nb; e++)
+        ri[e].l = -1;
+    double *Si = S + (int64_t)i * TS_NPLANE;
     for (int q = 0; q < 2; q++)
     {
-        S[(TS_A + q) * n + i] = a[q];
-        S[(TS_B + q) * n + i] = bb[q];
+        Si[TS_A + q] = a[q];
+        Si[TS_B + q] = bb[q];
         for (int c = 0; c < 3; c++)
-            S[(TS_V + 3 * q + c) * n + i] = V[q][c], S[(TS_W + 3 * q + c) * n + i] = W[q][c];
+            Si[TS_V + 3 * q + c] = V[q][c], Si[TS_W + 3 * q + c] = W[q][c];
     }
 }
 
@@ -76,7 +113,75 @@ __device__ __forceinline__ int conn_find(const int *sc, int nc, int q)
     return -1;
 }
 
-// Row i of the symmetrised tangent, written straight into the sliced-ELL K and K32:
+// Two-hop paths of every row, grouped by the conn slot they land on: path (a, t) is
+// i -> j = nbr(i, a) -> l = nbr(j, t) with the same shell on both bonds and l != i;
+// it adds to K(i, l).  Per slot the paths keep the order of (a, t); the lists
+// depend on the neighbour lists and the pattern only, so they are built once per
+// pattern.  One thread per row.
+__global__ void k_tangent_paths(DV v, uint16_t *__restrict__ pstart, uint16_t *__restrict__ plist)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= v.n)
+        return;
+    const int64_t n = v.n;
+    const int nc = v.nb_conn[i], cap = v.nb * v.nb;
+    uint16_t cnt[LPMX_MAXCONN + 1];
+    for (int k = 0; k <= nc; k++)
+        cnt[k] = 0;
+    uint16_t *ps = pstart + (int64_t)i * (LPMX_MAXCONN + 1);
+    uint16_t *pl = plist + (int64_t)i * cap;
+    for (int pass = 0; pass < 2; pass++)
+    {
+        for (int a = 0; a < v.nb0[i]; a++)
+        {
+            const int64_t b = (int64_t)a * n + i;
+            const int j = v.nbr[b], s = v.nsign[b];
+            for (int t = 0; t < v.nb0[j]; t++)
+            {
+                const int64_t bt = (int64_t)t * n + j;
+                const int l = v.nbr[bt];
+                if (v.nsign[bt] != s || l == i || l < 0)
+                    continue;
+                int lo = 0, hi = nc - 1, k = -1;
+                while (lo <= hi)
+                {
+                    const int mid = (lo + hi) >> 1, c = v.conn[(int64_t)mid * n + i];
+                    if (c == l)
+                    {
+                        k = mid;
+                        break;
+                    }
+                    if (c < l)
+                        lo = mid + 1;
+                    else
+                        hi = mid - 1;
+                }
+                if (k < 0)
+                    continue;
+                if (pass == 0)
+                    cnt[k]++;
+                else
+                    pl[cnt[k]++] = (uint16_t)(a << 5 | t);
+            }
+        }
+        if (pass == 0)
+        {
+            int acc = 0;
+            for (int k = 0; k < nc; k++)
+            {
+                const int c = cnt[k];
+                ps[k] = (uint16_t)acc;
+                cnt[k] = (uint16_t)acc;
+                acc += c;
+            }
+            ps[nc] = (uint16_t)acc;
+        }
+    }
+}
+
+#define TR_ROWS 8 // rows per work-group: one wavefront each, 8 consecutive lanes of a slice
+
+// Rows of the symmetrised tangent, written straight into the sliced-ELL K and K32:
 // K(i,m) = 0.5 (J_im + J_mi^T) for m != i, the upper triangle of J_ii mirrored for
 // m = i (stiffness.c:441-483), J = dPin/dx the Jacobian of the elastic law.  With
 // n = n_ij, g_ij = db_ij n_ij, F_ij = db_ij [2Kn dL_ij + (B_i + B_j)/2 + Tv_ij (A_i + A_j)/2]
@@ -89,144 +194,139 @@ __device__ __forceinline__ int conn_find(const int *sc, int nc, int q)
 // path i -> j -> l contributes to K(i,l) through J_il and, reversed, through J_li:
 // 0.5 (alpha + beta) n_ij n_jl^T with alpha = -db_ij (Tv_jl + Tv_ij) db_jl / 2 and
 // beta = -db_lj (Tv_ji + Tv_lj) db_ji / 2; for l = i it is J_ii's alpha term.
-// One 64-lane work-group per row.  Lane a handles bond a; every bond's off-diagonal
-// block lands on a distinct conn slot, and the two-hop rounds (bond a of i, lane t =
-// bond t of j) land on distinct slots within a round and are added in bond order:
-// the sums are deterministic.  Rows are mapped so that each XCD assembles a
-// contiguous range, so the partial cache lines of a slice meet in one L2.
+// One wavefront per row, TR_ROWS rows per work-group.  Lane a handles bond a; every
+// bond's off-diagonal block lands on a distinct conn slot, and the two-hop rounds
+// (bond a of i, lane t = bond t of j, read from j's contiguous records) land on
+// distinct slots within a round and are added in bond order: the sums are
+// deterministic.  The rows of a work-group are 8 consecutive lanes of one slice, so
+// the work-group stores every slot entry of its rows as one 64-byte segment.  Row
+// groups are mapped so that each XCD assembles a contiguous range of slices.
 template <int D>
-__global__ __launch_bounds__(64) void k_tangent_rows(DV v, const double *S)
+__global__ __launch_bounds__(64 * TR_ROWS) void k_tangent_rows(DV v, const double *__restrict__ S,
+                                                               const TBond *__restrict__ rec,
+                                                               const TBondOwn *__restrict__ own,
+                                                               const uint16_t *__restrict__ pstart,
+                                                               const uint16_t *__restrict__ plist)
 {
     constexpr int D2 = D * D;
-    __shared__ double acc[LPMX_MAXCONN][D2];
-    __shared__ double dg[LPMX_MAXNB][D2];
-    __shared__ double bn[LPMX_MAXNB][3], bx[LPMX_MAXNB][3], bdb[LPMX_MAXNB], btv[LPMX_MAXNB], bdbr[LPMX_MAXNB],
-        btvr[LPMX_MAXNB];
-    __shared__ int sc[LPMX_MAXCONN], bj[LPMX_MAXNB], bs[LPMX_MAXNB];
-    const int i = xcd_block(), lane = threadIdx.x;
+    __shared__ double acc[TR_ROWS][LPMX_MAXCONN][D2];
+    __shared__ double dg[TR_ROWS][LPMX_MAXNB][D2];
+    __shared__ double bn[TR_ROWS][LPMX_MAXNB][3], bdb[TR_ROWS][LPMX_MAXNB], btv[TR_ROWS][LPMX_MAXNB],
+        bdbr[TR_ROWS][LPMX_MAXNB], btvr[TR_ROWS][LPMX_MAXNB];
+    __shared__ int sc[TR_ROWS][LPMX_MAXCONN], bj[TR_ROWS][LPMX_MAXNB], bs[TR_ROWS][LPMX_MAXNB];
+    __shared__ int rnc[TR_ROWS], rpii[TR_ROWS];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int row0 = xcd_block() * TR_ROWS, i = row0 + w;
     const int64_t n = v.n;
-    const int nc = v.nb_conn[i], nbi = v.nb0[i];
-    if (lane < nc)
-        sc[lane] = v.conn[(int64_t)lane * n + i];
-    for (int e = lane; e < LPMX_MAXCONN * D2; e += 64)
-        (&acc[0][0])[e] = 0.0;
-    __syncthreads();
-    const double xi[3] = {v.x[3 * i], v.x[3 * i + 1], v.x[3 * i + 2]};
-    // ---- bonds of i, one per lane
-    if (lane < nbi)
+    const bool live = i < v.n;
+    const int si = row0 / LPMX_SLICE, li0 = row0 % LPMX_SLICE;
+    const int64_t slot0 = v.sptr[si];
+    const int width = v.sptr[si + 1] - (int)slot0;
+    const int nc = live ? v.nb_conn[i] : 0, nbi = live ? v.nb0[i] : 0;
+    // the rows' columns, 8 consecutive lanes of the slice per slot
+    for (int t = threadIdx.x; t < width * TR_ROWS; t += 64 * TR_ROWS)
     {
-        const int64_t b = (int64_t)lane * n + i;
-        const int j = v.nbr[b], s = v.nsign[b];
-        const double db = v.db[b], tv = v.Tv[b], kn = v.Kn[b];
-        const int64_t br = (int64_t)v.rev[b] * n + j;
-        const double dbr = v.db[br], tvr = v.Tv[br], knr = v.Kn[br];
-        const double xj[3] = {v.x[3 * j], v.x[3 * j + 1], v.x[3 * j + 2]};
-        double nv[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
-        const double d = sqrt(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]);
-        for (int c = 0; c < 3; c++)
-            nv[c] /= d;
-        const double Ai = S[(TS_A + s) * n + i], Bi = S[(TS_B + s) * n + i];
-        const double Aj = S[(TS_A + s) * n + j], Bj = S[(TS_B + s) * n + j];
-        const double dl = (d - v.L0[b] - v.dLp0[b]) * db, dlr = (d - v.L0[br] - v.dLp0[br]) * dbr;
+        const int r8 = t % TR_ROWS, k = t / TR_ROWS;
+        sc[r8][k] = v.col[(slot0 + k) * LPMX_SLICE + li0 + r8];
+    }
+    if (lane == 0)
+        rnc[w] = nc, rpii[w] = live ? v.dslot[i] : 0;
+    for (int e = lane; e < LPMX_MAXCONN * D2; e += 64)
+        (&acc[w][0][0])[e] = 0.0;
+    __syncthreads();
+    // ---- bonds of i: D lanes per bond, one row r of the bond's blocks each
+    for (int u = lane; u < D * nbi; u += 64)
+    {
+        const int ab = u / D, r = u % D;
+        const TBond rb = rec[(int64_t)i * v.nb + ab];
+        const TBondOwn ob = own[(int64_t)i * v.nb + ab];
+        const int j = rb.l, s = rb.s;
+        const double db = rb.db, tv = rb.tv, kn = ob.kn, dbr = rb.dbr, tvr = rb.tvr, knr = ob.knr;
+        const double nv[3] = {rb.n[0], rb.n[1], rb.n[2]};
+        const double d = ob.d;
+        const double *Si = S + (int64_t)i * TS_NPLANE, *Sj = S + (int64_t)j * TS_NPLANE;
+        const double Ai = Si[TS_A + s], Bi = Si[TS_B + s];
+        const double Aj = Sj[TS_A + s], Bj = Sj[TS_B + s];
+        const double dl = ob.dl, dlr = ob.dlr;
         const double F = db * (2.0 * kn * dl + 0.5 * (Bi + Bj) + 0.5 * tv * (Ai + Aj));
         const double Fr = dbr * (2.0 * knr * dlr + 0.5 * (Bj + Bi) + 0.5 * tvr * (Aj + Ai));
-        double Vi[3], Wi[3], Vj[3], Wj[3];
+        double Vj[3], Wj[3];
         for (int c = 0; c < 3; c++)
-        {
-            Vi[c] = S[(TS_V + 3 * s + c) * n + i], Wi[c] = S[(TS_W + 3 * s + c) * n + i];
-            Vj[c] = S[(TS_V + 3 * s + c) * n + j], Wj[c] = S[(TS_W + 3 * s + c) * n + j];
-        }
+            Vj[c] = Sj[TS_V + 3 * s + c], Wj[c] = Sj[TS_W + 3 * s + c];
+        const double Vir = Si[TS_V + 3 * s + r], Wir = Si[TS_W + 3 * s + r];
         const double fd = F / d, fdr = Fr / d, k2 = 2.0 * kn * db * db, k2r = 2.0 * knr * dbr * dbr;
         const double w2 = 0.5 * db * dbr * (tvr + tv);
-        const int pj = conn_find(sc, nc, j);
-        for (int r = 0; r < D; r++)
-            for (int c = 0; c < D; c++)
-            {
-                const double nn = nv[r] * nv[c], del = r == c ? 1.0 : 0.0;
-                // J_ij of this bond, and J_ji's transpose (n_ji = -n)
-                const double jij = -(fd * (del - nn) + k2 * nn) + 0.5 * db * nv[r] * (Wj[c] + tv * Vj[c]) -
-                                   0.5 * (tv * Vi[r] + Wi[r]) * db * nv[c];
-                const double jjiT = -(fdr * (del - nn) + k2r * nn) - 0.5 * dbr * (Wi[r] + tvr * Vi[r]) * nv[c] +
-                                    0.5 * dbr * nv[r] * (tvr * Vj[c] + Wj[c]);
-                if (pj >= 0)
-                    acc[pj][r * D + c] = 0.5 * (jij + jjiT);
-                dg[lane][r * D + c] =
-                    fd * (del - nn) + k2 * nn + 0.5 * db * nv[r] * (Wi[c] + tv * Vi[c]) + w2 * nn;
-            }
-        bj[lane] = j, bs[lane] = s;
-        for (int c = 0; c < 3; c++)
-            bn[lane][c] = nv[c], bx[lane][c] = xj[c];
-        bdb[lane] = db, btv[lane] = tv, bdbr[lane] = dbr, btvr[lane] = tvr;
+        const int pj = conn_find(sc[w], nc, j);
+        for (int c = 0; c < D; c++)
+        {
+            const double nn = nv[r] * nv[c], del = r == c ? 1.0 : 0.0;
+            const double Vic = Si[TS_V + 3 * s + c], Wic = Si[TS_W + 3 * s + c];
+            // J_ij of this bond, and J_ji's transpose (n_ji = -n)
+            const double jij = -(fd * (del - nn) + k2 * nn) + 0.5 * db * nv[r] * (Wj[c] + tv * Vj[c]) -
+                               0.5 * (tv * Vir + Wir) * db * nv[c];
+            const double jjiT = -(fdr * (del - nn) + k2r * nn) - 0.5 * dbr * (Wir + tvr * Vir) * nv[c] +
+                                0.5 * dbr * nv[r] * (tvr * Vj[c] + Wj[c]);
+            if (pj >= 0)
+                acc[w][pj][r * D + c] = 0.5 * (jij + jjiT);
+            dg[w][ab][r * D + c] = fd * (del - nn) + k2 * nn + 0.5 * db * nv[r] * (Wic + tv * Vic) + w2 * nn;
+        }
+        if (r == 0)
+        {
+            bj[w][ab] = j, bs[w][ab] = s;
+            for (int c = 0; c < 3; c++)
+                bn[w][ab][c] = nv[c];
+            bdb[w][ab] = db, btv[w][ab] = tv, bdbr[w][ab] = dbr, btvr[w][ab] = tvr;
+        }
     }
     __syncthreads();
-    const int pii = v.dslot[i];
-    if (lane < D2)
+    const int pii = rpii[w];
+    if (lane < D2 && live)
     {
         double t = 0.0;
         for (int a = 0; a < nbi; a++)
-            t += dg[a][lane];
-        acc[pii][lane] += t;
+            t += dg[w][a][lane];
+        acc[w][pii][lane] += t;
     }
     __syncthreads();
-    // ---- two-hop paths i -> j -> l: groups of nb lanes take consecutive bonds of i
-    const int nbm = v.nb, ng = 64 / nbm, grp = lane / nbm, t = lane % nbm;
-    for (int a0 = 0; a0 < nbi; a0 += ng)
+    // ---- two-hop paths i -> j -> l: lane k sums the paths that land on slot k, in
+    // their listed order, from j's contiguous records
+    if (lane < nc)
     {
-        const int a = a0 + grp;
-        double gam = 0.0, njl[3] = {0, 0, 0};
-        int pl = -1;
-        if (grp < ng && a < nbi)
+        double sum[D2];
+        for (int e = 0; e < D2; e++)
+            sum[e] = 0.0;
+        const uint16_t *ps = pstart + (int64_t)i * (LPMX_MAXCONN + 1);
+        const uint16_t *pl = plist + (int64_t)i * v.nb * v.nb;
+        const int p1 = ps[lane + 1];
+        for (int p = ps[lane]; p < p1; p++)
         {
-            const int j = bj[a], s = bs[a];
-            if (t < v.nb0[j])
-            {
-                const int64_t bt = (int64_t)t * n + j;
-                const int l = v.nbr[bt];
-                if (v.nsign[bt] == s && l != i)
-                {
-                    const double dbjl = v.db[bt], tvjl = v.Tv[bt];
-                    const int64_t br = (int64_t)v.rev[bt] * n + l;
-                    const double dblj = v.db[br], tvlj = v.Tv[br];
-                    double u[3] = {bx[a][0] - v.x[3 * l], bx[a][1] - v.x[3 * l + 1], bx[a][2] - v.x[3 * l + 2]};
-                    const double djl = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
-                    for (int c = 0; c < 3; c++)
-                        njl[c] = u[c] / djl;
-                    const double alpha = -0.5 * bdb[a] * (tvjl + btv[a]) * dbjl;
-                    const double beta = -0.5 * dblj * (btvr[a] + tvlj) * bdbr[a];
-                    gam = 0.5 * (alpha + beta);
-                    pl = conn_find(sc, nc, l);
-                }
-            }
+            const int code = pl[p], a = code >> 5;
+            const TBond *rt = rec + (int64_t)bj[w][a] * v.nb + (code & 31);
+            const double alpha = -0.5 * bdb[w][a] * (rt->tv + btv[w][a]) * rt->db;
+            const double beta = -0.5 * rt->dbr * (btvr[w][a] + rt->tvr) * bdbr[w][a];
+            const double gam = 0.5 * (alpha + beta);
+            for (int r = 0; r < D; r++)
+                for (int c = 0; c < D; c++)
+                    sum[r * D + c] += gam * bn[w][a][r] * rt->n[c];
         }
-        // groups add in bond order; within a group the slots are distinct
-        for (int q = 0; q < ng; q++)
-        {
-            if (grp == q && pl >= 0)
-            {
-                const int a = a0 + q;
-                for (int r = 0; r < D; r++)
-                    for (int c = 0; c < D; c++)
-                        acc[pl][r * D + c] += gam * bn[a][r] * njl[c];
-            }
-            __syncthreads();
-        }
+        for (int e = 0; e < D2; e++)
+            acc[w][lane][e] += sum[e];
     }
-    // ---- the row's blocks in slice order; padding slots hold zero blocks
-    const int si = i / LPMX_SLICE, li = i % LPMX_SLICE;
-    const int w = v.sptr[si + 1] - v.sptr[si];
-    for (int k = lane; k < w; k += 64)
+    __syncthreads();
+    // ---- the rows' blocks in slice order, 8 lanes of the slice per store; padding
+    // slots hold zero blocks
+    for (int u = threadIdx.x; u < width * D2 * TR_ROWS; u += 64 * TR_ROWS)
     {
-        const int64_t slot = v.sptr[si] + k;
-        for (int r = 0; r < D; r++)
-            for (int c = 0; c < D; c++)
-            {
-                double val = 0.0;
-                if (k < nc)
-                    val = (k == pii && r > c) ? acc[k][c * D + r] : acc[k][r * D + c];
-                const int64_t o = (slot * D2 + r * D + c) * LPMX_SLICE + li;
-                v.K[o] = val;
-                v.K32[o] = (float)val;
-            }
+        const int r8 = u % TR_ROWS, ke = u / TR_ROWS, k = ke / D2, e = ke % D2;
+        if (row0 + r8 >= v.n)
+            continue;
+        const int r = e / D, c = e % D;
+        double val = 0.0;
+        if (k < rnc[r8])
+            val = (k == rpii[r8] && r > c) ? acc[r8][k][c * D + r] : acc[r8][k][e];
+        const int64_t o = ((slot0 + k) * D2 + e) * LPMX_SLICE + li0 + r8;
+        v.K[o] = val;
+        v.K32[o] = (float)val;
     }
 }
 
@@ -234,6 +334,34 @@ static int ensure_tangent_buffers(void)
 {
     if (!t_S)
         HIPCHK(hipMalloc(&t_S, sizeof(double) * TS_NPLANE * (int64_t)g.n));
+    const int64_t need = ((int64_t)sizeof(TBond) + (int64_t)sizeof(TBondOwn)) * g.nb * (int64_t)g.n;
+    if (need > t_rec_bytes)
+    {
+        if (t_rec)
+            hipFree(t_rec);
+        t_rec = nullptr;
+        t_rec_bytes = 0;
+        HIPCHK(hipMalloc(&t_rec, need));
+        t_rec_bytes = need;
+    }
+    const int64_t pneed = (int64_t)(LPMX_MAXCONN + 1 + g.nb * g.nb) * g.n;
+    if (pneed > t_paths_cap)
+    {
+        if (t_pstart)
+            hipFree(t_pstart);
+        t_pstart = t_plist = nullptr;
+        t_paths_cap = 0;
+        HIPCHK(hipMalloc(&t_pstart, sizeof(uint16_t) * pneed));
+        t_paths_cap = pneed;
+        t_paths_version = -1;
+    }
+    t_plist = t_pstart + (int64_t)(LPMX_MAXCONN + 1) * g.n; // the paths follow this model's starts
+    if (t_paths_version != lpmx_pattern_version())
+    {
+        hipLaunchKernelGGL(k_tangent_paths, dim3(nblocks(g.n)), dim3(LPMX_BLOCK), 0, g_stream, g, t_pstart, t_plist);
+        LAUNCHCHK();
+        t_paths_version = lpmx_pattern_version();
+    }
     return LPMX_OK;
 }
 
@@ -241,9 +369,18 @@ void lpmx_tangent_free(void)
 {
     if (t_S)
         hipFree(t_S);
+    if (t_rec)
+        hipFree(t_rec);
+    if (t_pstart)
+        hipFree(t_pstart);
     if (t_kstart)
         hipFree(t_kstart);
     t_S = nullptr;
+    t_rec = nullptr;
+    t_rec_bytes = 0;
+    t_pstart = t_plist = nullptr;
+    t_paths_cap = 0;
+    t_paths_version = -1;
     t_kstart = nullptr;
     t_nd_valid = false;
 }
@@ -261,12 +398,15 @@ extern "C" int lpmx_tangent(void)
     t_nd_valid = false;
     t_epoch++;
     t_kver++;
-    hipLaunchKernelGGL(k_tangent_sums, dim3(nblocks(g.n)), dim3(LPMX_BLOCK), 0, g_stream, g, t_S);
+    TBond *rec = (TBond *)t_rec;
+    TBondOwn *own = (TBondOwn *)(rec + (int64_t)g.nb * g.n);
+    hipLaunchKernelGGL(k_tangent_sums, dim3(nblocks(g.n)), dim3(LPMX_BLOCK), 0, g_stream, g, t_S, rec, own);
     LAUNCHCHK();
+    const dim3 G(nblocks(g.n, TR_ROWS)), T(64 * TR_ROWS);
     if (g.dim == 3)
-        hipLaunchKernelGGL(k_tangent_rows<3>, dim3(g.n), dim3(64), 0, g_stream, g, t_S);
+        hipLaunchKernelGGL(k_tangent_rows<3>, G, T, 0, g_stream, g, t_S, rec, own, t_pstart, t_plist);
     else
-        hipLaunchKernelGGL(k_tangent_rows<2>, dim3(g.n), dim3(64), 0, g_stream, g, t_S);
+        hipLaunchKernelGGL(k_tangent_rows<2>, G, T, 0, g_stream, g, t_S, rec, own, t_pstart, t_plist);
     LAUNCHCHK();
     return LPMX_OK;
 }

@@ -115,6 +115,45 @@ def test_bond_constants_and_tangent(oracle):
     assert np.abs(y - yo).max() <= 1e-5 * np.abs(yo).max()
 
 
+@pytest.mark.parametrize("kind", ["sc", "hex"])
+def test_tangent_with_broken_bonds(oracle, kind):
+    """The tangent rows (bond records, eight rows per work-group, the last slice
+    partly filled) with a share of broken bonds (damage_broken 0 on one side of a
+    bond, or both): against the oracle's analytic tangent to 1e-12, SC and the 2D
+    hexagonal lattice."""
+    L = P.lib()
+    if kind == "sc":
+        _, n, x, t, m = sc_pair(oracle, 7)
+        d = 3
+    else:
+        n, x, t, c = P.build_hex_model([0.0, 12.0, 0.0, 10.0, 0.0, 1.0], critical=1.0)
+        m = oracle_model(oracle, x, 1, t, [c] * 3, plmode=6, sigmay=1e30, critical_bstrain=1.0, nbreak=20)
+        d = 2
+    nb = m.nbr.shape[1]
+    rng = np.random.default_rng(11)
+    xd = x + 2e-3 * rng.standard_normal(x.shape) * (np.arange(3) < d)
+    dlp = 1e-4 * rng.standard_normal((n, nb)) * (m.nbr >= 0)
+    broken = np.where((rng.random((n, nb)) < 0.1) & (m.nbr >= 0), 0.0, 1.0)
+    push("XYZ", xd)
+    push("DLP0", dlp)
+    push("DAMAGE_BROKEN", broken)
+    P.check(L.lpmx_tangent(), "tangent")
+    nK = m.s.nK
+    IK = np.zeros(d * n + 1, np.int32)
+    JK = np.zeros(nK, np.int32)
+    K = np.zeros(nK)
+    P.check(L.lpmx_export_upper_csr(P.ptr(IK), P.ptr(JK), P.ptr(K)), "export")
+    m.x[:] = xd
+    m.dLp[:, :, 0] = dlp
+    m.dmg_broken[:] = broken
+    oracle.lib().orc_tangent_analytic(m.p)
+    np.testing.assert_array_equal(IK, m.IK)
+    np.testing.assert_array_equal(JK, m.JK)
+    err = np.abs(K - m.K).max() / np.abs(m.K).max()
+    print(f"{kind}: {n} particles, {int((broken == 0).sum())} broken bond sides, tangent rel diff {err:.2e}")
+    assert err <= 1e-12
+
+
 @pytest.mark.parametrize("mode", [6, 0, 4])
 def test_constitutive_laws_bit_exact(oracle, mode):
     L = P.lib()
