@@ -107,7 +107,49 @@ __global__ void k_vjp_force(DV v, VjpIO io, AdjWork a)
     }
 }
 
-// Per particle: second pass, return map and trial pass in reverse.
+// Bond (i, j) of the forward recomputation: distance, unit vector (x_i - x_q)/d and
+// first-pass stretch, in k_j2_particle's expression order.
+struct VjpBond
+{
+    double dis, c[3], dl;
+};
+
+__device__ __forceinline__ VjpBond vjp_bond(const DV &v, int i, int64_t b)
+{
+    VjpBond o;
+    const int q = v.nbr[b];
+    const double dx = v.x[3 * i] - v.x[3 * q], dy = v.x[3 * i + 1] - v.x[3 * q + 1],
+                 dz = v.x[3 * i + 2] - v.x[3 * q + 2];
+    o.dis = sqrt(dx * dx + dy * dy + dz * dz);
+    double dl = o.dis - v.L0[b];
+    dl -= v.dLp0[b];
+    dl *= v.db[b];
+    o.dl = dl;
+    o.c[0] = (v.x[3 * i] - v.x[3 * q]) / o.dis;
+    o.c[1] = (v.x[3 * i + 1] - v.x[3 * q + 1]) / o.dis;
+    o.c[2] = (v.x[3 * i + 2] - v.x[3 * q + 2]) / o.dis;
+    return o;
+}
+
+// Position adjoint of bond b from the adjoints of its distance (disb) and unit
+// vector (cb): d(dis)/dx_i = c, dc/dx_i = (I - c c^T)/dis.  Linear in (disb, cb), so
+// each reverse pass adds its own part.
+__device__ __forceinline__ void vjp_geom(const DV &v, const AdjWork &a, int64_t b, int64_t NB, const VjpBond &g,
+                                         double disb, const double *cb)
+{
+    const double dot = g.c[0] * cb[0] + g.c[1] * cb[1] + g.c[2] * cb[2];
+    a.gb[b] += disb * g.c[0] + (cb[0] - dot * g.c[0]) / g.dis;
+    a.gb[NB + b] += disb * g.c[1] + (cb[1] - dot * g.c[1]) / g.dis;
+    a.gb[2 * NB + b] += disb * g.c[2] + (cb[2] - dot * g.c[2]) / g.dis;
+}
+
+// Per particle: second pass, return map and trial pass in reverse.  Nothing per bond
+// is held across passes: the geometry, stretches and trial forces are recomputed
+// from the cached positions in the forward's expressions (so the same values and
+// the same yield decisions), and each pass adds its part of the position adjoint to
+// a.gb, so no per-bond array lives in scratch memory.  (Keeping the geometry in
+// LDS or in bond planes of HBM instead measured slower: 893 and 639 against 547 us
+// at 216k particles.)
 __global__ void k_vjp_particle(DV v, VjpIO io, AdjWork a)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -115,57 +157,48 @@ __global__ void k_vjp_particle(DV v, VjpIO io, AdjWork a)
         return;
     const int n = v.n, nbi = v.nb0[i];
     const int64_t NB = (int64_t)n * v.nb;
-    // adjoints of the second-pass stretches and sums from both ends' forces
-    double dL2b[LPMX_MAXNB], A2b[2] = {0, 0}, B2b[2] = {0, 0};
+    // sums of the adjoints of the second-pass stretches, from both ends' forces
+    double A2b[2] = {0, 0}, B2b[2] = {0, 0};
     for (int j = 0; j < nbi; j++)
     {
         const int64_t b = BI(j, i);
         const int q = v.nbr[b], s = v.nsign[b], r = v.rev[b];
         const double t = a.Fb[b] * v.dw[b];
-        dL2b[j] = r >= 0 ? t * v.Kn[b] : 0.0;
         A2b[s] += t * 0.5 * v.Tv[b];
         B2b[s] += t * 0.5;
         if (r >= 0)
         {
             const int64_t br = BI(r, q);
             const double tr = a.Fb[br] * v.dw[br];
-            dL2b[j] += tr * v.Kn[br];
             A2b[s] += tr * 0.5 * v.Tv[br];
             B2b[s] += tr * 0.5;
         }
     }
     // forward recomputation (k_j2_particle's order)
-    double dis[LPMX_MAXNB], cs[LPMX_MAXNB][3], dL1[LPMX_MAXNB], Ftr[LPMX_MAXNB];
     double a0 = 0, a1 = 0, b0 = 0, b1 = 0;
     for (int j = 0; j < nbi; j++)
     {
         const int64_t b = BI(j, i);
-        const int q = v.nbr[b];
-        const double dx = v.x[3 * i] - v.x[3 * q], dy = v.x[3 * i + 1] - v.x[3 * q + 1],
-                     dz = v.x[3 * i + 2] - v.x[3 * q + 2];
-        dis[j] = sqrt(dx * dx + dy * dy + dz * dz);
-        double dl = dis[j] - v.L0[b];
-        dl -= v.dLp0[b];
-        dl *= v.db[b];
-        dL1[j] = dl;
+        const VjpBond g = vjp_bond(v, i, b);
         if (v.nsign[b] == 0)
-            a0 += dl, b0 += v.Tv[b] * dl;
+            a0 += g.dl, b0 += v.Tv[b] * g.dl;
         else
-            a1 += dl, b1 += v.Tv[b] * dl;
-        cs[j][0] = (v.x[3 * i] - v.x[3 * q]) / dis[j];
-        cs[j][1] = (v.x[3 * i + 1] - v.x[3 * q + 1]) / dis[j];
-        cs[j][2] = (v.x[3 * i + 2] - v.x[3 * q + 2]) / dis[j];
+            a1 += g.dl, b1 += v.Tv[b] * g.dl;
     }
+    // trial force of bond b from its first-pass stretch
+    auto ftrial = [&](int64_t b, double dl) {
+        const int s = v.nsign[b];
+        double Fij = 2.0 * v.Kn[b] * dl + (s == 0 ? b0 : b1) + v.Tv[b] * (s == 0 ? a0 : a1);
+        return Fij * v.dw[b];
+    };
     double sl[6] = {0, 0, 0, 0, 0, 0};
     for (int j = 0; j < nbi; j++)
     {
         const int64_t b = BI(j, i);
-        const int s = v.nsign[b];
-        double Fij = 2.0 * v.Kn[b] * dL1[j] + (s == 0 ? b0 : b1) + v.Tv[b] * (s == 0 ? a0 : a1);
-        Fij *= v.dw[b];
-        Ftr[j] = Fij;
+        const VjpBond g = vjp_bond(v, i, b);
+        const double Fij = ftrial(b, g.dl);
         const double of = vjp_opp(v, i, j);
-        const double cx = cs[j][0], cy = cs[j][1], cz = cs[j][2], L0 = v.L0[b];
+        const double cx = g.c[0], cy = g.c[1], cz = g.c[2], L0 = v.L0[b];
         sl[0] += of / v.volume * L0 * Fij * cx * cx;
         sl[1] += of / v.volume * L0 * Fij * cy * cy;
         sl[2] += of / v.volume * L0 * Fij * cz * cz;
@@ -191,27 +224,36 @@ __global__ void k_vjp_particle(DV v, VjpIO io, AdjWork a)
         for (int c = 0; c < 6; c++)
             dep[c] = dlam * 1.5 * sl[c] / seq;
     // second pass in reverse (dL2 of the evaluation is in v.dL)
-    double depb[6] = {0, 0, 0, 0, 0, 0}, disb[LPMX_MAXNB], csb[LPMX_MAXNB][3];
+    double depb[6] = {0, 0, 0, 0, 0, 0};
     for (int j = 0; j < nbi; j++)
     {
         const int64_t b = BI(j, i);
-        const int s = v.nsign[b];
+        const int q = v.nbr[b], s = v.nsign[b], r = v.rev[b];
+        const double t = a.Fb[b] * v.dw[b];
+        double dL2b = r >= 0 ? t * v.Kn[b] : 0.0;
+        if (r >= 0)
+        {
+            const int64_t br = BI(r, q);
+            const double tr = a.Fb[br] * v.dw[br];
+            dL2b += tr * v.Kn[br];
+        }
+        const VjpBond g = vjp_bond(v, i, b);
         const double db = v.db[b];
-        const double tot = dL2b[j] + A2b[s] + B2b[s] * v.Tv[b];
+        const double tot = dL2b + A2b[s] + B2b[s] * v.Tv[b];
         a.Tvb[b] += B2b[s] * v.dL[b];
-        disb[j] = tot * db;
         const double xpb = -tot * db + io.dLp2b[b] * db;
         io.dLp0b[b] = xpb;
         double cc[6], ccb[6];
-        cc6(cs[j][0], cs[j][1], cs[j][2], cc);
+        cc6(g.c[0], g.c[1], g.c[2], cc);
         for (int c = 0; c < 6; c++)
         {
             const double wc = c < 3 ? 1.0 : 2.0;
             depb[c] += xpb * v.L0[b] * db * wc * cc[c];
             ccb[c] = xpb * v.L0[b] * db * wc * dep[c];
         }
-        csb[j][0] = csb[j][1] = csb[j][2] = 0.0;
-        cc6_bar(cs[j][0], cs[j][1], cs[j][2], ccb, csb[j]);
+        double cb[3] = {0.0, 0.0, 0.0};
+        cc6_bar(g.c[0], g.c[1], g.c[2], ccb, cb);
+        vjp_geom(v, a, b, NB, g, tot * db, cb);
     }
     for (int j = nbi; j < v.nb; j++)
         io.dLp0b[BI(j, i)] = io.dLp2b[BI(j, i)] * v.db[BI(j, i)];
@@ -256,39 +298,47 @@ __global__ void k_vjp_particle(DV v, VjpIO io, AdjWork a)
         beta0b[c] -= sdevb[c];
         slb[c] = c < 3 ? sdevb[c] - mb : sdevb[c];
     }
-    // trial stress, trial force, first pass, geometry
-    double A1b[2] = {0, 0}, B1b[2] = {0, 0}, dL1b[LPMX_MAXNB];
-    for (int j = 0; j < nbi; j++)
-    {
-        const int64_t b = BI(j, i);
-        const int s = v.nsign[b];
+    // trial stress, trial force: adjoint of the trial force of bond b, and the part of
+    // its unit-vector adjoint that comes through the trial stress
+    auto ftrial_bar = [&](int j, int64_t b, const VjpBond &g, double *cb) {
         double cc[6], ccb[6];
-        cc6(cs[j][0], cs[j][1], cs[j][2], cc);
+        cc6(g.c[0], g.c[1], g.c[2], cc);
         const double w = vjp_opp(v, i, j) / v.volume * v.L0[b];
+        const double Ftr = ftrial(b, g.dl);
         double Ftrb = 0.0;
         for (int c = 0; c < 6; c++)
-            Ftrb += slb[c] * w * cc[c], ccb[c] = slb[c] * w * Ftr[j];
-        cc6_bar(cs[j][0], cs[j][1], cs[j][2], ccb, csb[j]);
-        const double t = Ftrb * v.dw[b];
-        dL1b[j] = t * 2.0 * v.Kn[b];
-        B1b[s] += t;
-        A1b[s] += t * v.Tv[b];
-        a.Knb[b] += t * 2.0 * dL1[j];
-        a.Tvb[b] += t * (s == 0 ? a0 : a1);
-    }
+            Ftrb += slb[c] * w * cc[c], ccb[c] = slb[c] * w * Ftr;
+        cb[0] = cb[1] = cb[2] = 0.0;
+        cc6_bar(g.c[0], g.c[1], g.c[2], ccb, cb);
+        return Ftrb * v.dw[b];
+    };
+    double A1b[2] = {0, 0}, B1b[2] = {0, 0};
     for (int j = 0; j < nbi; j++)
     {
         const int64_t b = BI(j, i);
         const int s = v.nsign[b];
-        const double tot = dL1b[j] + A1b[s] + B1b[s] * v.Tv[b];
-        a.Tvb[b] += B1b[s] * dL1[j];
-        disb[j] += tot * v.db[b];
+        const VjpBond g = vjp_bond(v, i, b);
+        double cb[3];
+        const double t = ftrial_bar(j, b, g, cb);
+        B1b[s] += t;
+        A1b[s] += t * v.Tv[b];
+        a.Knb[b] += t * 2.0 * g.dl;
+        a.Tvb[b] += t * (s == 0 ? a0 : a1);
+        vjp_geom(v, a, b, NB, g, 0.0, cb);
+    }
+    // first pass and geometry
+    for (int j = 0; j < nbi; j++)
+    {
+        const int64_t b = BI(j, i);
+        const int s = v.nsign[b];
+        const VjpBond g = vjp_bond(v, i, b);
+        double cb[3];
+        const double dL1b = ftrial_bar(j, b, g, cb) * 2.0 * v.Kn[b];
+        const double tot = dL1b + A1b[s] + B1b[s] * v.Tv[b];
+        a.Tvb[b] += B1b[s] * g.dl;
         io.dLp0b[b] -= tot * v.db[b];
-        const double cx = cs[j][0], cy = cs[j][1], cz = cs[j][2];
-        const double dot = cx * csb[j][0] + cy * csb[j][1] + cz * csb[j][2];
-        a.gb[b] += disb[j] * cx + (csb[j][0] - dot * cx) / dis[j];
-        a.gb[NB + b] += disb[j] * cy + (csb[j][1] - dot * cy) / dis[j];
-        a.gb[2 * NB + b] += disb[j] * cz + (csb[j][2] - dot * cz) / dis[j];
+        const double zero[3] = {0.0, 0.0, 0.0};
+        vjp_geom(v, a, b, NB, g, tot * v.db[b], zero);
     }
     for (int c = 0; c < 6; c++)
         io.beta0b[(int64_t)c * n + i] = beta0b[c];

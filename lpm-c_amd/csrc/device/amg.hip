@@ -828,7 +828,11 @@ static int to_mat(const DBsr &H, Mat &B)
     B.nr = H.nr, B.nc = H.nc, B.br = H.br, B.bc = H.bc, B.nnzb = H.nnzb;
     const int64_t bs = (int64_t)H.br * H.bc;
     int64_t nval = 0;
-    if (H.nr < SLICE_MIN_ROWS)
+    // one thread per row (SLICE) for many rows, or for a fair number of short rows (a
+    // wavefront on a row of a few blocks leaves most lanes idle); a wavefront per row
+    // (WROW) for long rows
+    const bool slice = H.nr >= SLICE_MIN_ROWS || (H.nr >= 65536 && (int64_t)H.nnzb <= 48LL * H.nr);
+    if (!slice)
     {
         B.layout = LAYOUT_WROW;
         nval = (int64_t)H.nnzb * bs;

@@ -149,34 +149,6 @@ __device__ __forceinline__ T block_sum(T v, T *sh)
     return t;
 }
 
-// True in the work-group that finishes last; its partials[] reads see every
-// other group's writes.  The counter is reset by the caller (last group).
-__device__ __forceinline__ bool last_group(unsigned *counter)
-{
-    __shared__ bool am_last;
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0)
-        am_last = atomicAdd(counter, 1u) == gridDim.x - 1;
-    __syncthreads();
-    return am_last;
-}
-
-__device__ __forceinline__ double coherent_load(const double *p)
-{
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Sum of parts[0..np) by one work-group in a fixed order (strided per thread,
-// then block_sum), readable coherently after last_group().
-__device__ __forceinline__ double ordered_total(const double *parts, int np, double *sh)
-{
-    double s = 0.0;
-    for (int k = threadIdx.x; k < np; k += blockDim.x)
-        s += coherent_load(parts + k);
-    return block_sum(s, sh);
-}
-
 // ------------------------------------------------------ host-side reductions -----
 // Deterministic device sums of up to 4 per-element values produced by a functor;
 // used for norms and small outputs outside the CG inner loop.
@@ -188,7 +160,6 @@ int lpmx_dev_scan_int(const int *in, int *out_excl, int count, int *total_host);
 
 // scratch: a device buffer reused by host-side helpers (grown on demand)
 double *lpmx_scratch(int64_t bytes);
-unsigned *lpmx_counters(void); // 64 zero-initialised counters for last_group()
 
 // solver workspace release (called from lpmx_model_destroy)
 void lpmx_solver_free(void);

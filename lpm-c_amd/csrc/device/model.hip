@@ -20,7 +20,6 @@ int lpmx_pattern_version(void) { return g_pattern_version; }
 static double g_lo[3], g_hi[3];
 static double *g_scratch = nullptr;
 static int64_t g_scratch_bytes = 0;
-static unsigned *g_counters = nullptr;
 
 extern "C" const char *lpmx_last_error(void) { return g_err; }
 
@@ -68,14 +67,10 @@ static int init_device(void)
     if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
         return lpmx_fail(LPMX_ERR_NOGPU, "device %d is %s, this library is built for gfx950", dev, prop.gcnArchName);
     HIPCHK(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
-    HIPCHK(hipMalloc(&g_counters, 64 * sizeof(unsigned)));
-    HIPCHK(hipMemsetAsync(g_counters, 0, 64 * sizeof(unsigned), g_stream));
     return LPMX_OK;
 }
 
 int lpmx_device_init(void) { return init_device(); }
-
-unsigned *lpmx_counters(void) { return g_counters; }
 
 double *lpmx_scratch(int64_t bytes)
 {
