@@ -330,6 +330,40 @@ __global__ __launch_bounds__(64 * TR_ROWS) void k_tangent_rows(DV v, const doubl
     }
 }
 
+// The reference's tangent is its upper CSR (stiffness.c:441-483); the solvers read
+// both triangles here, so every block below the diagonal is made the exact
+// transpose of its mirror above it (the row kernel's lower blocks agree with those
+// only to round-off).  One thread per row; the upper blocks are only read.
+template <int D>
+__global__ void k_tangent_mirror(DV v)
+{
+    constexpr int D2 = D * D;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= v.n)
+        return;
+    const int s = i / LPMX_SLICE, l = i % LPMX_SLICE;
+    const int nc = v.nb_conn[i];
+    for (int k = 0; k < nc; k++)
+    {
+        const int64_t e0 = (int64_t)(v.sptr[s] + k) * LPMX_SLICE + l;
+        if (v.col[e0] >= i)
+            continue;
+        const int64_t tp = v.tpos[e0];
+        if (tp < 0)
+            continue;
+        const int64_t st = tp / LPMX_SLICE, lt = tp % LPMX_SLICE;
+        const int64_t slot = v.sptr[s] + k;
+        for (int r = 0; r < D; r++)
+            for (int c = 0; c < D; c++)
+            {
+                const int64_t o = (slot * D2 + r * D + c) * LPMX_SLICE + l;
+                const int64_t t = (st * D2 + c * D + r) * LPMX_SLICE + lt;
+                v.K[o] = v.K[t];
+                v.K32[o] = v.K32[t];
+            }
+    }
+}
+
 static int ensure_tangent_buffers(void)
 {
     if (!t_S)
@@ -407,6 +441,11 @@ extern "C" int lpmx_tangent(void)
         hipLaunchKernelGGL(k_tangent_rows<3>, G, T, 0, g_stream, g, t_S, rec, own, t_pstart, t_plist);
     else
         hipLaunchKernelGGL(k_tangent_rows<2>, G, T, 0, g_stream, g, t_S, rec, own, t_pstart, t_plist);
+    LAUNCHCHK();
+    if (g.dim == 3)
+        hipLaunchKernelGGL(k_tangent_mirror<3>, dim3(nblocks(g.n)), dim3(LPMX_BLOCK), 0, g_stream, g);
+    else
+        hipLaunchKernelGGL(k_tangent_mirror<2>, dim3(nblocks(g.n)), dim3(LPMX_BLOCK), 0, g_stream, g);
     LAUNCHCHK();
     return LPMX_OK;
 }

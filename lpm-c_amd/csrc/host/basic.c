@@ -2,7 +2,9 @@
  * Host helpers of the drop-in API: allocation, copies, counts, the type setters,
  * the two MKL routines drivers call directly, and the output-only strain and bond
  * re-ordering (reference src/lpm_basic.c).  None of these is on the load-step hot
- * path; they keep drivers linking and running unchanged.
+ * path; they keep drivers linking and running unchanged.  Those that read or change
+ * a model array tell the drop-in layer first (lpm_dropin_host_access), so that in
+ * its resident mode they see the device's current values.
  */
 #include <math.h>
 #include <stdio.h>
@@ -10,6 +12,7 @@
 #include <string.h>
 
 #include "lpm.h"
+#include "dropin.h"
 
 /* ---------------------------------------------------------------- allocation --- */
 int *allocInt1D(int num, int a)
@@ -111,23 +114,31 @@ void freeDouble3D(double ***arr, int row, int column)
 
 void copyInt2D(int **target, int **source, int row, int col)
 {
+    lpm_dropin_host_access(source, 0);
+    lpm_dropin_host_access(target, 1);
     for (int i = 0; i < row; i++)
         memcpy(target[i], source[i], sizeof(int) * col);
 }
 
 void copyDouble1D(double *target, double *source, int row)
 {
+    lpm_dropin_host_access(source, 0);
+    lpm_dropin_host_access(target, 1);
     memcpy(target, source, sizeof(double) * row);
 }
 
 void copyDouble2D(double **target, double **source, int row, int col)
 {
+    lpm_dropin_host_access(source, 0);
+    lpm_dropin_host_access(target, 1);
     for (int i = 0; i < row; i++)
         memcpy(target[i], source[i], sizeof(double) * col);
 }
 
 void copyDouble3D(double ***target, double ***source, int row, int col, int deep)
 {
+    lpm_dropin_host_access(source, 0);
+    lpm_dropin_host_access(target, 1);
     for (int i = 0; i < row; i++)
         for (int j = 0; j < col; j++)
             memcpy(target[i][j], source[i][j], sizeof(double) * deep);
@@ -136,6 +147,7 @@ void copyDouble3D(double ***target, double ***source, int row, int col, int deep
 /* ------------------------------------------------------------------- counting --- */
 int findMaxDouble(double *arr, int len)
 {
+    lpm_dropin_host_access(arr, 0);
     int t = 0;
     for (int i = 0; i < len; i++)
         if (arr[i] > arr[t])
@@ -145,6 +157,7 @@ int findMaxDouble(double *arr, int len)
 
 int findMaxInt(int *arr, int len)
 {
+    lpm_dropin_host_access(arr, 0);
     int t = 0;
     for (int i = 0; i < len; i++)
         if (arr[i] > arr[t])
@@ -154,6 +167,7 @@ int findMaxInt(int *arr, int len)
 
 int countEqual(int *arr, int len, int b)
 {
+    lpm_dropin_host_access(arr, 0);
     int c = 0;
     for (int i = 0; i < len; i++)
         c += arr[i] == b;
@@ -162,6 +176,7 @@ int countEqual(int *arr, int len, int b)
 
 int countNEqual(int *arr, int len, int b)
 {
+    lpm_dropin_host_access(arr, 0);
     int c = 0;
     for (int i = 0; i < len; i++)
         c += arr[i] != b;
@@ -170,6 +185,7 @@ int countNEqual(int *arr, int len, int b)
 
 int countLarger(int *arr, int len, int b)
 {
+    lpm_dropin_host_access(arr, 0);
     int c = 0;
     for (int i = 0; i < len; i++)
         c += arr[i] > b;
@@ -178,6 +194,7 @@ int countLarger(int *arr, int len, int b)
 
 int findNElement(int *in, int *out, int len, int a)
 {
+    lpm_dropin_host_access(in, 0);
     int k = 0;
     for (int i = 0; i < len; i++)
         if (in[i] != a)
@@ -187,6 +204,7 @@ int findNElement(int *in, int *out, int len, int a)
 
 double sumDouble(double *arr, int len)
 {
+    lpm_dropin_host_access(arr, 0);
     double s = 0.0;
     for (int i = 0; i < len; i++)
         s += arr[i];
@@ -195,6 +213,7 @@ double sumDouble(double *arr, int len)
 
 int sumInt(int *arr, int len)
 {
+    lpm_dropin_host_access(arr, 0);
     int s = 0;
     for (int i = 0; i < len; i++)
         s += arr[i];
@@ -205,6 +224,7 @@ int sumInt(int *arr, int len)
  * the ascending order whatever the algorithm). */
 void sortInt(int *arr, int len)
 {
+    lpm_dropin_host_access(arr, 1);
     for (int r = len / 2; r >= 1; r /= 2)
         for (int i = r; i < len; i++)
         {
@@ -235,6 +255,7 @@ int getLine(char *line, int max, FILE *fp)
 
 void printDouble(double *target, int m, int n)
 {
+    lpm_dropin_host_access(target, 0);
     for (int i = m - 1; i < n; i++)
         printf("%8.8e \n", target[i]);
     printf("\n");
@@ -242,6 +263,7 @@ void printDouble(double *target, int m, int n)
 
 void printInt(int *target, int n)
 {
+    lpm_dropin_host_access(target, 0);
     for (int i = 0; i < n; i++)
         printf("%d ", target[i]);
     printf("\n");
@@ -250,6 +272,8 @@ void printInt(int *target, int n)
 /* --------------------------------------------------------------- type setters --- */
 void setTypeRect(double r0, double r1, double r2, double r3, double r4, double r5, int t)
 {
+    lpm_dropin_host_access(xyz, 0);
+    lpm_dropin_host_access(type, 1);
     for (int i = 0; i < nparticle; i++)
         if (xyz[i][0] >= r0 && xyz[i][0] <= r1 && xyz[i][1] >= r2 && xyz[i][1] <= r3 && xyz[i][2] >= r4 &&
             xyz[i][2] <= r5)
@@ -258,6 +282,8 @@ void setTypeRect(double r0, double r1, double r2, double r3, double r4, double r
 
 void setTypeCircle(double x, double y, double r, int t)
 {
+    lpm_dropin_host_access(xyz, 0);
+    lpm_dropin_host_access(type, 1);
     for (int i = 0; i < nparticle; i++)
     {
         double dx = xyz[i][0] - x, dy = xyz[i][1] - y;
@@ -268,6 +294,8 @@ void setTypeCircle(double x, double y, double r, int t)
 
 void setTypeFullNeighbor(int k)
 {
+    lpm_dropin_host_access(nb_initial, 0);
+    lpm_dropin_host_access(type, 1);
     for (int i = 0; i < nparticle; i++)
         if (nb_initial[i] == nneighbors)
             type[i] = k;
@@ -277,6 +305,7 @@ void setTypeFullNeighbor(int k)
 /* Euclidean norm with the usual scaling against overflow. */
 double cblas_dnrm2(const int n, const double *x, const int incx)
 {
+    lpm_dropin_host_access(x, 0);
     double scale = 0.0, ssq = 1.0;
     for (int i = 0; i < n; i++)
     {
@@ -303,6 +332,7 @@ static void reorder_bonds(double **dst, int stretch)
 {
     if (!bond_vector || !dst)
         return;
+    lpm_dropin_sync_host();
     for (int i = 0; i < nparticle; i++)
         for (int k = 0; k < nneighbors; k++)
             for (int j = 0; j < nb_initial[i]; j++)

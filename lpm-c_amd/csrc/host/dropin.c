@@ -12,6 +12,15 @@
  *
  * Fatal conditions print and exit(1), as the reference does; a CG that reaches its
  * cap prints and carries on with the displacement it has, as the reference does.
+ *
+ * Resident mode (LPMX_DROPIN_RESIDENT=1 in the environment when the device model is
+ * made): the device copy of every field is the current one.  A call no longer pulls
+ * what it writes but marks the host copy stale, a push skips a field the device
+ * already holds, and library code that reads a host array (the writers, copyDouble*,
+ * countNEqual, cblas_dnrm2, the damage file lines) first brings back the stale fields
+ * it reads.  The tangent K_global stays on the device the same way.  It is for
+ * drivers that touch the model's arrays only through library calls, as the
+ * reference's drivers do; the device computes the same values in both modes.
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -27,6 +36,12 @@ static double *ext_modulus = NULL; /* [nparticle] or NULL */
 static double *ext_crit = NULL;    /* [nparticle * nneighbors] or NULL */
 static lpmx_solver ext_solver;
 static int ext_solver_set = 0;
+/* resident mode: per field, the host copy is behind the device's (h_stale), the
+ * device copy is current (d_fresh), and the host array the device copy came from */
+static int resident = 0;
+static unsigned char h_stale[LPMX_F_COUNT], d_fresh[LPMX_F_COUNT];
+static const void *d_from[LPMX_F_COUNT];
+static int k_h_stale = 0, k_d_fresh = 0, k_have_pattern = 0; /* the same for K_global */
 
 void lpm_dropin_fatal(const char *where, int rc)
 {
@@ -177,6 +192,9 @@ void lpm_dropin_push(int f)
     struct hb b;
     if (!bind(f, &b))
         return;
+    /* resident: nothing to send while the device holds this array's current values */
+    if (resident && d_from[f] == b.p && (d_fresh[f] || h_stale[f]))
+        return;
     const long n = nparticle, w = b.width, tot = n * w;
     int rc;
     if (b.kind == K_I1 || b.kind == K_I2)
@@ -203,9 +221,26 @@ void lpm_dropin_push(int f)
         rc = lpmx_put(f, t, tot);
     }
     CHK("push", rc);
+    d_fresh[f] = 1, h_stale[f] = 0, d_from[f] = b.p;
 }
 
+static void pull_now(int f);
+
 void lpm_dropin_pull(int f)
+{
+    struct hb b;
+    if (!bind(f, &b))
+        return;
+    if (resident)
+    {
+        h_stale[f] = 1, d_fresh[f] = 1, d_from[f] = b.p;
+        return;
+    }
+    pull_now(f);
+}
+
+/* the device copy of field f into the host array */
+static void pull_now(int f)
 {
     struct hb b;
     if (!bind(f, &b))
@@ -234,6 +269,74 @@ void lpm_dropin_pull(int f)
             for (long i = 0; i < n; i++)
                 ((double **)b.p)[i][b.slot] = t[i];
     }
+    h_stale[f] = 0, d_fresh[f] = 1, d_from[f] = b.p;
+}
+
+/* Host code is about to read (write = 0) or change (write = 1) the array at p: bring
+ * back the fields stored there whose host copy is stale; after a change the device
+ * copy is behind. */
+void lpm_dropin_host_access(const void *p, int write)
+{
+    if (!resident || !p)
+        return;
+    if (p == (const void *)K_global)
+    {
+        if (k_h_stale)
+            CHK("K export", lpmx_export_upper_csr(NULL, NULL, K_global));
+        k_h_stale = 0;
+        if (write)
+            k_d_fresh = 0;
+        return;
+    }
+    for (int f = 0; f < LPMX_F_COUNT; f++)
+    {
+        struct hb b;
+        if (!bind(f, &b) || b.p != p)
+            continue;
+        if (h_stale[f])
+            pull_now(f);
+        if (write)
+            d_fresh[f] = 0;
+    }
+}
+
+/* Every stale host copy brought back (before host code that reads many arrays). */
+void lpm_dropin_sync_host(void)
+{
+    if (!resident)
+        return;
+    for (int f = 0; f < LPMX_F_COUNT; f++)
+        if (h_stale[f])
+            pull_now(f);
+    if (k_h_stale)
+        CHK("K export", lpmx_export_upper_csr(NULL, NULL, K_global));
+    k_h_stale = 0;
+}
+
+static void host_read(int f)
+{
+    if (resident && h_stale[f])
+        pull_now(f);
+}
+
+/* K_global to the device unless it holds the current values, and back unless the
+ * host copy may wait (resident) */
+static void k_import(void)
+{
+    if (resident && k_d_fresh)
+        return;
+    CHK("K import", lpmx_import_upper_csr(K_global));
+    k_d_fresh = 1;
+}
+
+static void k_export(void)
+{
+    if (resident)
+    {
+        k_h_stale = 1, k_d_fresh = 1;
+        return;
+    }
+    CHK("K export", lpmx_export_upper_csr(NULL, NULL, K_global));
 }
 
 static void push_list(const int *f, int k)
@@ -260,6 +363,13 @@ static void pull_list(const int *f, int k)
         const int fl_[] = {__VA_ARGS__};                                       \
         pull_list(fl_, (int)(sizeof(fl_) / sizeof(int)));                      \
     } while (0)
+#define PULL_NOW(...)                                                          \
+    do                                                                         \
+    {                                                                          \
+        const int fl_[] = {__VA_ARGS__};                                       \
+        for (int k_ = 0; k_ < (int)(sizeof(fl_) / sizeof(int)); k_++)           \
+            pull_now(fl_[k_]);                                                 \
+    } while (0)
 
 /* ------------------------------------------------------------ device model --- */
 static void push_params(void)
@@ -277,6 +387,12 @@ static void create_model(void)
     for (long i = 0; i < nparticle; i++)
         memcpy(x + 3 * i, xyz[i], sizeof(double) * 3);
     CHK("lpmx_model_create", lpmx_model_create(&d, x));
+    const char *rm = getenv("LPMX_DROPIN_RESIDENT");
+    resident = rm && atoi(rm) != 0;
+    memset(h_stale, 0, sizeof(h_stale));
+    memset(d_fresh, 0, sizeof(d_fresh));
+    memset(d_from, 0, sizeof(d_from));
+    k_h_stale = k_d_fresh = k_have_pattern = 0;
     if (xyz_initial)
         lpm_dropin_push(LPMX_F_XYZ_INITIAL);
     d_have_model = 1;
@@ -303,8 +419,8 @@ void searchNormalNeighbor()
 {
     create_model();
     CHK("searchNormalNeighbor", lpmx_neighbor_build());
-    PULL(LPMX_F_NEIGHBORS, LPMX_F_NSIGN, LPMX_F_NB, LPMX_F_NB_INITIAL, LPMX_F_DISTANCE_INITIAL, LPMX_F_CSX_INITIAL,
-         LPMX_F_CSY_INITIAL, LPMX_F_CSZ_INITIAL);
+    PULL_NOW(LPMX_F_NEIGHBORS, LPMX_F_NSIGN, LPMX_F_NB, LPMX_F_NB_INITIAL, LPMX_F_DISTANCE_INITIAL,
+             LPMX_F_CSX_INITIAL, LPMX_F_CSY_INITIAL, LPMX_F_CSZ_INITIAL);
     /* the per-shell lists are the shell-ordered subsequences of neighbors[i] */
     for (int i = 0; i < nparticle; i++)
     {
@@ -329,7 +445,7 @@ int searchAFEMNeighbor()
     need_model("searchAFEMNeighbor");
     int64_t len = 0;
     CHK("searchAFEMNeighbor", lpmx_connectivity_build(&len));
-    lpm_dropin_pull(LPMX_F_NB_CONN);
+    pull_now(LPMX_F_NB_CONN);
     const long cm = nneighbors_AFEM + 1;
     int *t = itmp(nparticle * cm);
     CHK("conn", lpmx_get(LPMX_F_CONN, t, nparticle * cm));
@@ -398,7 +514,7 @@ void calcKnTv()
             ce[3 * t + c] = Ce[t][c];
     CHK("calcKnTv", lpmx_set_materials(ntype, type, ce, sigmay, ext_modulus));
     free(ce);
-    PULL(LPMX_F_KN, LPMX_F_TV);
+    PULL_NOW(LPMX_F_KN, LPMX_F_TV);
 }
 
 /* --------------------------------------------------------------------- tangent --- */
@@ -407,9 +523,18 @@ static void stiffness(int pm)
     need_model("calcStiffnessFiniteDifference");
     PUSH(LPMX_F_XYZ, LPMX_F_DLP0, LPMX_F_DAMAGE_BROKEN, LPMX_F_KN, LPMX_F_TV);
     CHK("tangent", lpmx_tangent());
-    CHK("tangent export", lpmx_export_upper_csr(IK, JK, K_global));
+    if (!resident || !k_have_pattern)
+    {
+        CHK("tangent export", lpmx_export_upper_csr(IK, JK, K_global));
+        k_have_pattern = 1, k_h_stale = 0, k_d_fresh = 1;
+    }
+    else
+        k_export();
     if (pm != 6) /* the reference only evaluates the elastic law (plmode 6) */
+    {
         memset(K_global, 0, sizeof(double) * K_pointer[nparticle][1]);
+        k_h_stale = 0, k_d_fresh = 0;
+    }
 }
 
 /* calcStiffness{3,2}DFiniteDifference (reference src/stiffness.c:271-516): the
@@ -450,10 +575,10 @@ void setForceBC(int nboundForce, struct forceBCPara *fBP)
 static void bc_stiffness(void)
 {
     need_model("setDispBC_stiffnessUpdate");
-    CHK("K import", lpmx_import_upper_csr(K_global));
+    k_import();
     PUSH(LPMX_F_DISPBC_INDEX, LPMX_F_FIX_INDEX, LPMX_F_RESIDUAL);
     CHK("setDispBC_stiffnessUpdate", lpmx_bc_eliminate());
-    CHK("K export", lpmx_export_upper_csr(NULL, NULL, K_global));
+    k_export();
     lpm_dropin_pull(LPMX_F_RESIDUAL);
 }
 
@@ -464,7 +589,7 @@ void setDispBC_stiffnessUpdate2D() { bc_stiffness(); }
 static void solve(double rtol2, double atol2, int announce)
 {
     need_model("solverCG");
-    CHK("K import", lpmx_import_upper_csr(K_global));
+    k_import();
     PUSH(LPMX_F_RESIDUAL, LPMX_F_XYZ);
     int prec = ext_solver_set ? ext_solver.precision : LPMX_FP64;
     double inner = ext_solver_set ? ext_solver.inner_rtol2 : 1e-2;
@@ -721,6 +846,7 @@ int updateDuctileDamagePwiseNonlocal(const char *dataName, int tstep)
     /* the broken-bond file lists newly broken bonds in (i, slot) order */
     const long nbn = (long)nparticle * nneighbors;
     unsigned char *was = malloc(nbn > 0 ? nbn : 1);
+    host_read(LPMX_F_DAMAGE_BROKEN);
     for (long i = 0; i < nparticle; i++)
         for (long j = 0; j < nneighbors; j++)
             was[i * nneighbors + j] = damage_broken[i][j] > EPS || damage_broken[i][j] < -EPS;
@@ -729,6 +855,9 @@ int updateDuctileDamagePwiseNonlocal(const char *dataName, int tstep)
     PULL(LPMX_F_DAMAGE_NONLOCAL0, LPMX_F_DAMAGE_BROKEN, LPMX_F_DAMAGE_D0, LPMX_F_DAMAGE_W);
     if (fp)
     {
+        host_read(LPMX_F_DAMAGE_BROKEN);
+        host_read(LPMX_F_NB_INITIAL);
+        host_read(LPMX_F_NEIGHBORS);
         for (long i = 0; i < nparticle; i++)
             for (long j = 0; j < nb_initial[i]; j++)
                 if (was[i * nneighbors + j] && damage_broken[i][j] == 0.0)
@@ -782,6 +911,8 @@ static int ductile_variant(const char *dataName, int tstep, int law, const char 
     const long nbn = (long)nparticle * nneighbors;
     unsigned char *was = malloc(nbn > 0 ? nbn : 1);
     unsigned char *failed_before = malloc(nparticle > 0 ? nparticle : 1);
+    host_read(LPMX_F_DAMAGE_BROKEN);
+    host_read(LPMX_F_DAMAGE_LOCAL0);
     for (long i = 0; i < nparticle; i++)
     {
         for (long j = 0; j < nneighbors; j++)
@@ -794,6 +925,10 @@ static int ductile_variant(const char *dataName, int tstep, int law, const char 
          LPMX_F_NB);
     if (fp)
     {
+        host_read(LPMX_F_DAMAGE_LOCAL0);
+        host_read(LPMX_F_DAMAGE_BROKEN);
+        host_read(LPMX_F_NB_INITIAL);
+        host_read(LPMX_F_NEIGHBORS);
         for (long i = 0; i < nparticle; i++)
         {
             if (law == LPMX_DMG_PWISE_LOCAL)

@@ -178,6 +178,7 @@ static int keep_in_box(const double *p, const void *arg)
 /* moveParticle (reference src/initialization.c:830-866). */
 void moveParticle(double *movexyz)
 {
+    lpm_dropin_host_access(xyz, 1);
     for (int i = 0; i < nparticle; i++)
         for (int j = 0; j < 3; j++)
             xyz[i][j] = xyz[i][j] + movexyz[j];

@@ -8,10 +8,12 @@
 #include <stdlib.h>
 
 #include "lpm.h"
+#include "dropin.h"
 
 /* writeDump (reference src/data_handler.c:42-84): LAMMPS dump of the particles. */
 void writeDump(const char *dataName, int step, char flag)
 {
+    lpm_dropin_sync_host();
     FILE *f = fopen(dataName, flag == 's' ? "w+" : "a+");
     if (!f)
         return;
@@ -54,6 +56,7 @@ void writeDump(const char *dataName, int step, char flag)
 /* writeRSS (reference src/data_handler.c:87-107). */
 void writeRSS(const char *dataName, int step)
 {
+    lpm_dropin_sync_host();
     FILE *f = fopen(dataName, "a+");
     if (!f)
         return;
@@ -73,6 +76,7 @@ void writeRSS(const char *dataName, int step)
 
 static void type_average(const char *dataName, double **a, int t1, int tstep)
 {
+    lpm_dropin_sync_host();
     FILE *f = fopen(dataName, "a+");
     if (!f)
         return;
@@ -98,6 +102,7 @@ void writeStrain(const char *dataName, int t1, int tstep) { type_average(dataNam
 
 void writeCab(const char *dataName, int ii)
 {
+    lpm_dropin_sync_host();
     FILE *f = fopen(dataName, "a+");
     if (!f)
         return;
@@ -114,6 +119,7 @@ void writeCab(const char *dataName, int ii)
 
 void writeInternalForce(const char *dataName, int step)
 {
+    lpm_dropin_sync_host();
     FILE *f = fopen(dataName, "a+");
     if (!f)
         return;
@@ -134,6 +140,7 @@ void writeInternalForce(const char *dataName, int step)
 
 void writeJact(const char *dataName, int step)
 {
+    lpm_dropin_sync_host();
     FILE *f = fopen(dataName, "a+");
     if (!f)
         return;
@@ -151,6 +158,7 @@ void writeJact(const char *dataName, int step)
 
 static void bond_rows(const char *dataName, int step, double **a, double **den)
 {
+    lpm_dropin_sync_host();
     FILE *f = fopen(dataName, "a+");
     if (!f)
         return;
@@ -170,6 +178,7 @@ void writeBondstretch(const char *dataName, int step) { bond_rows(dataName, step
 
 void writeBondforce(const char *dataName, int step)
 {
+    lpm_dropin_sync_host();
     FILE *f = fopen(dataName, "a+");
     if (!f)
         return;
@@ -187,6 +196,7 @@ void writeBondforce(const char *dataName, int step)
 /* writeDisp (reference src/data_handler.c:278-306): mean displacement of type t1. */
 void writeDisp(const char *dataName, char c, int t1, int tstep)
 {
+    lpm_dropin_sync_host();
     FILE *f = fopen(dataName, "a+");
     if (!f)
         return;
@@ -207,6 +217,7 @@ void writeDisp(const char *dataName, char c, int t1, int tstep)
  * projected on the axis. */
 void writeReaction(const char *dataName, char c, int t1, int tstep)
 {
+    lpm_dropin_sync_host();
     FILE *f = fopen(dataName, "a+");
     if (!f)
         return;
@@ -225,6 +236,7 @@ void writeReaction(const char *dataName, char c, int t1, int tstep)
  * of the body along c. */
 void writeForce(const char *dataname, char c, double p, int tstep)
 {
+    lpm_dropin_sync_host();
     FILE *f = fopen(dataname, "a+");
     if (!f)
         return;
@@ -243,6 +255,7 @@ void writeForce(const char *dataname, char c, double p, int tstep)
 
 void writeK_global(const char *dataName, int l)
 {
+    lpm_dropin_sync_host();
     FILE *f = fopen(dataName, "w+");
     if (!f)
         return;
@@ -253,6 +266,7 @@ void writeK_global(const char *dataName, int l)
 
 void writeDlambda(const char *dataName, int m, int n, int globalStep, int iterStep)
 {
+    lpm_dropin_sync_host();
     FILE *f = fopen(dataName, "a+");
     if (!f)
         return;
@@ -264,6 +278,7 @@ void writeDlambda(const char *dataName, int m, int n, int globalStep, int iterSt
 
 void writeDamage(const char *dataName, int step)
 {
+    lpm_dropin_sync_host();
     FILE *f = fopen(dataName, "a+");
     if (!f)
         return;
@@ -280,6 +295,7 @@ void writeDamage(const char *dataName, int step)
 
 void writeNeighbor(const char *dataName)
 {
+    lpm_dropin_sync_host();
     FILE *f = fopen(dataName, "a+");
     if (!f)
         return;
