@@ -143,8 +143,20 @@ __device__ bool cp_lu_solve(const DV &v, int i, int ns, double *r)
         {
             const double l = D[CPI(r2 * ns + k, i)] / dkk;
             D[CPI(r2 * ns + k, i)] = l;
-            for (int j = k + 1; j < ns; j++)
-                D[CPI(r2 * ns + j, i)] -= l * D[CPI(k * ns + j, i)];
+            // four columns at a time, so their loads are in flight together (one
+            // memory round trip per four entries instead of per entry)
+            for (int j0 = k + 1; j0 < ns; j0 += 4)
+            {
+                double x[4], y[4];
+#pragma unroll
+                for (int t = 0; t < 4; t++)
+                    if (j0 + t < ns)
+                        x[t] = D[CPI(r2 * ns + j0 + t, i)], y[t] = D[CPI(k * ns + j0 + t, i)];
+#pragma unroll
+                for (int t = 0; t < 4; t++)
+                    if (j0 + t < ns)
+                        D[CPI(r2 * ns + j0 + t, i)] = x[t] - l * y[t];
+            }
         }
     }
     for (int k = 0; k < ns; k++)
@@ -318,7 +330,15 @@ __global__ void k_cp_particle(DV v, int first, int count)
                     rr[s] = r[s];
                 }
                 // the active submatrix of the slip Jacobian; the hardening sum over
-                // the active systems (the inactive ones add Jact * hd = 0)
+                // the active systems (the inactive ones add Jact * hd = 0).  By
+                // position c of system d = act[c] (act ascending, so d = m exactly when
+                // c = a and d = u exactly when c = b): hd(m, u, d) is t0[c] but at c = a
+                // and c = b.  t0 is a register array (unrolled, compile-time indices):
+                // the sum does not chase act[c] and gam[d] through scratch memory.
+                double t0[LPMX_MAXSLIP];
+#pragma unroll
+                for (int c = 0; c < LPMX_MAXSLIP; c++)
+                    t0[c] = c < na ? v.cp_q * hp * gam[act[c]] : 0.0;
                 for (int a = 0; a < na; a++)
                 {
                     const int m = act[a];
@@ -328,21 +348,15 @@ __global__ void k_cp_particle(DV v, int first, int count)
                     for (int b = 0; b < na; b++)
                     {
                         const int u = act[b];
+                        // hd at d = m (h + hp gam_m when m = u, else hp gam_m) and at
+                        // d = u != m (q (h + hp gam_u)); the same expressions as the
+                        // reference's four cases
+                        const double hm = m == u ? h + hp * gam[m] : hp * gam[m], hu = v.cp_q * (h + hp * gam[u]);
                         double hs = 0.0;
-                        for (int c = 0; c < na; c++)
-                        {
-                            const int d = act[c];
-                            double hd;
-                            if (m == d && u == d)
-                                hd = h + hp * gam[d];
-                            else if (m == d && u != d)
-                                hd = hp * gam[d];
-                            else if (m != d && u == d)
-                                hd = v.cp_q * (h + hp * gam[d]);
-                            else
-                                hd = v.cp_q * hp * gam[d];
-                            hs += hd;
-                        }
+#pragma unroll
+                        for (int c = 0; c < LPMX_MAXSLIP; c++)
+                            if (c < na)
+                                hs += c == a ? hm : (c == b ? hu : t0[c]);
                         const double t2 = hs * pw;
                         const double cab = v.cp_Cab[CPI(m * ns + u, i)];
                         v.cp_D[CPI(a * na + b, i)] = m == u ? cab + t1 + t2 : cab + t2;
