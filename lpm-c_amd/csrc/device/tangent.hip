@@ -333,34 +333,46 @@ __global__ __launch_bounds__(64 * TR_ROWS) void k_tangent_rows(DV v, const doubl
 // The reference's tangent is its upper CSR (stiffness.c:441-483); the solvers read
 // both triangles here, so every block below the diagonal is made the exact
 // transpose of its mirror above it (the row kernel's lower blocks agree with those
-// only to round-off).  One thread per row; the upper blocks are only read.
+// only to round-off).  One thread per row.  The lower blocks written and the upper
+// blocks read are disjoint, so K is passed as a read-only view of the upper blocks
+// and a write-only view of the lower ones: a block's D*D loads are in flight together
+// and need not wait for the previous block's stores.  K32 holds the FP32 rounding
+// of K at every write, so the lower FP32 block is rounded from the copied values.
 template <int D>
-__global__ void k_tangent_mirror(DV v)
+__global__ void k_tangent_mirror(int n, const int *__restrict__ nb_conn, const int *__restrict__ sptr,
+                                 const int *__restrict__ col, const int *__restrict__ tpos,
+                                 const double *__restrict__ Kup, double *__restrict__ Klo,
+                                 float *__restrict__ K32lo)
 {
     constexpr int D2 = D * D;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= v.n)
+    if (i >= n)
         return;
     const int s = i / LPMX_SLICE, l = i % LPMX_SLICE;
-    const int nc = v.nb_conn[i];
+    const int nc = nb_conn[i];
+    const int64_t s0 = sptr[s];
     for (int k = 0; k < nc; k++)
     {
-        const int64_t e0 = (int64_t)(v.sptr[s] + k) * LPMX_SLICE + l;
-        if (v.col[e0] >= i)
+        const int64_t e0 = (s0 + k) * LPMX_SLICE + l;
+        if (col[e0] >= i)
             continue;
-        const int64_t tp = v.tpos[e0];
+        const int64_t tp = tpos[e0];
         if (tp < 0)
             continue;
         const int64_t st = tp / LPMX_SLICE, lt = tp % LPMX_SLICE;
-        const int64_t slot = v.sptr[s] + k;
+        double blk[D2];
+#pragma unroll
         for (int r = 0; r < D; r++)
+#pragma unroll
             for (int c = 0; c < D; c++)
-            {
-                const int64_t o = (slot * D2 + r * D + c) * LPMX_SLICE + l;
-                const int64_t t = (st * D2 + c * D + r) * LPMX_SLICE + lt;
-                v.K[o] = v.K[t];
-                v.K32[o] = v.K32[t];
-            }
+                blk[r * D + c] = Kup[(st * D2 + c * D + r) * LPMX_SLICE + lt];
+#pragma unroll
+        for (int e = 0; e < D2; e++)
+        {
+            const int64_t o = ((s0 + k) * D2 + e) * LPMX_SLICE + l;
+            Klo[o] = blk[e];
+            K32lo[o] = (float)blk[e];
+        }
     }
 }
 
@@ -443,9 +455,11 @@ extern "C" int lpmx_tangent(void)
         hipLaunchKernelGGL(k_tangent_rows<2>, G, T, 0, g_stream, g, t_S, rec, own, t_pstart, t_plist);
     LAUNCHCHK();
     if (g.dim == 3)
-        hipLaunchKernelGGL(k_tangent_mirror<3>, dim3(nblocks(g.n)), dim3(LPMX_BLOCK), 0, g_stream, g);
+        hipLaunchKernelGGL(k_tangent_mirror<3>, dim3(nblocks(g.n)), dim3(LPMX_BLOCK), 0, g_stream, g.n, g.nb_conn,
+                           g.sptr, g.col, g.tpos, g.K, g.K, g.K32);
     else
-        hipLaunchKernelGGL(k_tangent_mirror<2>, dim3(nblocks(g.n)), dim3(LPMX_BLOCK), 0, g_stream, g);
+        hipLaunchKernelGGL(k_tangent_mirror<2>, dim3(nblocks(g.n)), dim3(LPMX_BLOCK), 0, g_stream, g.n, g.nb_conn,
+                           g.sptr, g.col, g.tpos, g.K, g.K, g.K32);
     LAUNCHCHK();
     return LPMX_OK;
 }
