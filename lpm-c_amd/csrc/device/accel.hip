@@ -362,6 +362,10 @@ static int aa_step(AaHist *a, double *x, int xs, const double *d, int *mixed)
 // --------------------------------------------------------- the Newton loop's -----
 // precision LPMX_MIXED keeps the differences in FP32
 int lpmx_accel_begin(int depth, int precision) { return aa_begin(&A, depth, precision == LPMX_MIXED); }
+// Keep the history's differences and drop only its last point: the next step records
+// a new point without a difference across it (a re-solve of the same load step after
+// bond breakage, whose map differs from the last one by the broken bonds).
+void lpmx_accel_rebase(void) { A.have_prev = 0; }
 void lpmx_accel_forget(void) { aa_forget(&A); }
 // Revert xyz to the plain iterate of the last solve and forget the history.
 int lpmx_accel_revert(void) { return aa_revert(&A, g.x, 3); }

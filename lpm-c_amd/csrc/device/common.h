@@ -185,6 +185,7 @@ int lpmx_cp_particles(void);
 
 // Anderson acceleration of the Newton loop (accel.hip)
 int lpmx_accel_begin(int depth, int precision);
+void lpmx_accel_rebase(void);
 int lpmx_accel_step(int *mixed);
 int lpmx_accel_revert(void);
 void lpmx_accel_forget(void);

@@ -40,6 +40,21 @@ static bool loop_progress(void)
     return on;
 }
 
+// A re-solve after bond breakage keeps the Anderson differences of the step's earlier
+// solves (lpmx_accel_rebase): the broken bonds change the map little, and on the 1M
+// brittle shear the re-solves of a cascade take half the iterations of a new history,
+// with the same bonds broken.  LPMX_AA_WARM=0 starts a new history instead.
+static bool aa_warm(void)
+{
+    static int on = -1;
+    if (on < 0)
+    {
+        const char *e = getenv("LPMX_AA_WARM");
+        on = !(e && atoi(e) == 0);
+    }
+    return on;
+}
+
 int lpmx_run_steps_impl(const lpmx_solver *s, int nsteps, const int *nbd, const lpmx_dispbc *dbc, const int *nbf,
                         const lpmx_forcebc *fbc, int maxbc, char out_axis, int out_type, lpmx_step_stats *out,
                         int skip_damage, int newton_only)
@@ -68,13 +83,17 @@ int lpmx_run_steps_impl(const lpmx_solver *s, int nsteps, const int *nbd, const 
         RETCHK(lpmx_set_disp_bc(nbd[st], dbc + (int64_t)st * maxbc));
         RETCHK(lpmx_set_force_bc(nbf[st], fbc + (int64_t)st * maxbc));
         RETCHK(lpmx_bond_force(4)); // predictor (lpmc_project.c:405)
+        bool resolve = false;
         for (;;)
         {
             double nres = 0, nrea = 0;
             RETCHK(lpmx_update_rr(&nres, &nrea, nullptr));
             const double tolm = nres > nrea ? nres : nrea;
             int ni = 0;
-            RETCHK(lpmx_accel_begin(s->accel_depth, s->precision));
+            if (resolve && aa_warm())
+                lpmx_accel_rebase();
+            else
+                RETCHK(lpmx_accel_begin(s->accel_depth, s->precision));
             double nprev = nres;
             double ring[RING];
             int since = 0;
@@ -165,6 +184,7 @@ int lpmx_run_steps_impl(const lpmx_solver *s, int nsteps, const int *nbd, const 
                 RETCHK(lpmx_tangent());
                 RETCHK(sync());
                 o.t_tangent += now_s() - tb;
+                resolve = true;
                 continue;
             }
             break;
