@@ -409,8 +409,12 @@ __global__ __launch_bounds__(256) void k_slice_mv(int nr, const int *__restrict_
 }
 
 // WROW layout, one wavefront per row (4 rows per work-group); the lanes' partial
-// sums are combined in a fixed butterfly order.
-template <int BR, int BC, typename TV, typename TX, int MODE>
+// sums are combined in a fixed butterfly order.  RND = 2 takes two 64-block rounds
+// per loop trip with their loads in flight together (the per-lane sums keep the
+// order of one round after the other, so the bits are those of RND = 1): faster on
+// long rows (level-1 operator, 182 blocks per row at 1M: 6-8%; restriction, 298:
+// 11%), slower on short ones, whose extra registers halve the waves per SIMD.
+template <int BR, int BC, typename TV, typename TX, int MODE, int RND>
 __global__ __launch_bounds__(256) void k_wrow_mv(int nr, const int64_t *__restrict__ rptr, const int *__restrict__ col,
                                                  const TV *__restrict__ val, const TX *__restrict__ x,
                                                  TX *__restrict__ y, const TX *__restrict__ b, const int *done)
@@ -427,19 +431,47 @@ __global__ __launch_bounds__(256) void k_wrow_mv(int nr, const int64_t *__restri
 #pragma unroll
     for (int r = 0; r < BR; r++)
         acc[r] = 0;
-    for (int j = lane; j < len; j += 64)
-    {
-        const int m = col[k0 + j];
-        TX xm[BC];
-#pragma unroll
-        for (int c = 0; c < BC; c++)
-            xm[c] = x[(int64_t)m * BC + c];
-#pragma unroll
-        for (int r = 0; r < BR; r++)
+    if (RND == 1)
+        for (int j = lane; j < len; j += 64)
+        {
+            const int m = col[k0 + j];
+            TX xm[BC];
 #pragma unroll
             for (int c = 0; c < BC; c++)
-                acc[r] += (TX)__builtin_nontemporal_load(v + (int64_t)(r * BC + c) * len + j) * xm[c];
-    }
+                xm[c] = x[(int64_t)m * BC + c];
+#pragma unroll
+            for (int r = 0; r < BR; r++)
+#pragma unroll
+                for (int c = 0; c < BC; c++)
+                    acc[r] += (TX)__builtin_nontemporal_load(v + (int64_t)(r * BC + c) * len + j) * xm[c];
+        }
+    else
+        for (int j = lane; j < len; j += 128)
+        {
+            const bool two = j + 64 < len;
+            const int m0 = col[k0 + j], m1 = two ? col[k0 + j + 64] : m0;
+            TX x0[BC], x1[BC], v0[BR * BC], v1[BR * BC];
+#pragma unroll
+            for (int c = 0; c < BC; c++)
+                x0[c] = x[(int64_t)m0 * BC + c], x1[c] = x[(int64_t)m1 * BC + c];
+#pragma unroll
+            for (int e = 0; e < BR * BC; e++)
+            {
+                v0[e] = (TX)__builtin_nontemporal_load(v + (int64_t)e * len + j);
+                v1[e] = two ? (TX)__builtin_nontemporal_load(v + (int64_t)e * len + j + 64) : (TX)0;
+            }
+#pragma unroll
+            for (int r = 0; r < BR; r++)
+#pragma unroll
+                for (int c = 0; c < BC; c++)
+                    acc[r] += v0[r * BC + c] * x0[c];
+            if (two)
+#pragma unroll
+                for (int r = 0; r < BR; r++)
+#pragma unroll
+                    for (int c = 0; c < BC; c++)
+                        acc[r] += v1[r * BC + c] * x1[c];
+        }
 #pragma unroll
     for (int r = 0; r < BR; r++)
         acc[r] = wave_sum(acc[r]);
@@ -535,11 +567,15 @@ static void launch_mv_t(const Mat &B, const TX *x, TX *y, const TX *b, const int
     if (B.layout == LAYOUT_WROW)
     {
         const dim3 G(nblocks(B.nr, 4)), T(256);
+        const bool lng = B.nnzb >= 96 * (int64_t)B.nr; // rows of 96 blocks and more on average
         if (sizeof(TX) == 8)
-            hipLaunchKernelGGL((k_wrow_mv<BR, BC, double, TX, MODE>), G, T, 0, g_stream, B.nr, B.rptr, B.col,
+            hipLaunchKernelGGL((k_wrow_mv<BR, BC, double, TX, MODE, 1>), G, T, 0, g_stream, B.nr, B.rptr, B.col,
                                (const double *)B.val, x, y, b, done);
+        else if (lng)
+            hipLaunchKernelGGL((k_wrow_mv<BR, BC, float, TX, MODE, 2>), G, T, 0, g_stream, B.nr, B.rptr, B.col,
+                               (const float *)B.val32, x, y, b, done);
         else
-            hipLaunchKernelGGL((k_wrow_mv<BR, BC, float, TX, MODE>), G, T, 0, g_stream, B.nr, B.rptr, B.col,
+            hipLaunchKernelGGL((k_wrow_mv<BR, BC, float, TX, MODE, 1>), G, T, 0, g_stream, B.nr, B.rptr, B.col,
                                (const float *)B.val32, x, y, b, done);
         return;
     }
