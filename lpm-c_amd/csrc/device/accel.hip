@@ -166,9 +166,12 @@ __global__ __launch_bounds__(256) void k_aa_dots_total(const double *parts, int 
         out[q] = s;
 }
 
-// x -= sum_j gamma_j dC_j
+// x -= sum_j gamma_j dC_j.  Every slot is read, as in k_aa_dots: the host points the
+// slots from cnt on at slot s, with gamma_j = +0 there, so their terms are +-0 and
+// leave the sum's bits as they are, and the AA_MAX loads of an element are issued
+// together.
 template <typename T>
-__global__ void k_aa_mix(int n, int dim, int xs, double *__restrict__ x, AaPtrsT<T> P, int cnt, AaGamma gam)
+__global__ void k_aa_mix(int n, int dim, int xs, double *__restrict__ x, AaPtrsT<T> P, AaGamma gam)
 {
     const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (t >= (int64_t)n * dim)
@@ -176,8 +179,7 @@ __global__ void k_aa_mix(int n, int dim, int xs, double *__restrict__ x, AaPtrsT
     double s = 0.0;
 #pragma unroll
     for (int j = 0; j < AA_MAX; j++)
-        if (j < cnt)
-            s += gam.g[j] * (double)P.dC[j][t];
+        s += gam.g[j] * (double)P.dC[j][t];
     const int64_t i = t / dim;
     const int c = (int)(t - i * dim);
     x[xs * i + c] -= s;
@@ -320,20 +322,18 @@ static int aa_step(AaHist *a, double *x, int xs, const double *d, int *mixed)
         a->cnt++;
     a->next = (a->next + 1) % a->depth;
     const int cnt = a->cnt;
+    // the unused slots read slot s (written just now, so finite and cached)
+    AaPtrs32 Q32 = a->P32;
+    AaPtrs Q64 = a->P;
+    for (int j = cnt; j < AA_MAX; j++)
+    {
+        Q32.dF[j] = a->P32.dF[s], Q32.dC[j] = a->P32.dC[s];
+        Q64.dF[j] = a->P.dF[s], Q64.dC[j] = a->P.dC[s];
+    }
     if (a->f32)
-    {
-        AaPtrs32 Q = a->P32; // the unused slots read slot s
-        for (int j = cnt; j < AA_MAX; j++)
-            Q.dF[j] = a->P32.dF[s];
-        hipLaunchKernelGGL(k_aa_dots<float>, dim3(AA_GRID), dim3(256), 0, g_stream, m, Q, cnt, s, d, a->parts);
-    }
+        hipLaunchKernelGGL(k_aa_dots<float>, dim3(AA_GRID), dim3(256), 0, g_stream, m, Q32, cnt, s, d, a->parts);
     else
-    {
-        AaPtrs Q = a->P; // the unused slots read slot s
-        for (int j = cnt; j < AA_MAX; j++)
-            Q.dF[j] = a->P.dF[s];
-        hipLaunchKernelGGL(k_aa_dots<double>, dim3(AA_GRID), dim3(256), 0, g_stream, m, Q, cnt, s, d, a->parts);
-    }
+        hipLaunchKernelGGL(k_aa_dots<double>, dim3(AA_GRID), dim3(256), 0, g_stream, m, Q64, cnt, s, d, a->parts);
     hipLaunchKernelGGL(k_aa_dots_total, dim3(2 * AA_MAX), dim3(256), 0, g_stream, a->parts, AA_GRID, cnt, a->out);
     LAUNCHCHK();
     double o[2 * AA_MAX];
@@ -351,9 +351,9 @@ static int aa_step(AaHist *a, double *x, int xs, const double *d, int *mixed)
         return LPMX_OK;
     }
     if (a->f32)
-        hipLaunchKernelGGL(k_aa_mix<float>, dim3(nb), dim3(LPMX_BLOCK), 0, g_stream, n, dim, xs, x, a->P32, cnt, gam);
+        hipLaunchKernelGGL(k_aa_mix<float>, dim3(nb), dim3(LPMX_BLOCK), 0, g_stream, n, dim, xs, x, Q32, gam);
     else
-        hipLaunchKernelGGL(k_aa_mix<double>, dim3(nb), dim3(LPMX_BLOCK), 0, g_stream, n, dim, xs, x, a->P, cnt, gam);
+        hipLaunchKernelGGL(k_aa_mix<double>, dim3(nb), dim3(LPMX_BLOCK), 0, g_stream, n, dim, xs, x, Q64, gam);
     LAUNCHCHK();
     *mixed = 1;
     return LPMX_OK;
