@@ -57,8 +57,11 @@ struct AaHist
 
 static AaHist A; // the Newton loop's history
 
-// work-groups of the dot pass: three per CU (256 CUs), one resident round of four-wave
-// groups at the kernel's 166 VGPRs (three waves per SIMD)
+// work-groups of the dot pass: three per CU (256 CUs).  At the kernel's 177 VGPRs
+// (two waves per SIMD) that is 1.5 resident rounds; two per CU (one round) took 0.082
+// against 0.0995 ms at 1M, but its other grouping of the partial sums moved the
+// iterates enough that one step of the 216k perturbed-parameter history at Newton
+// 1e-10 stalled at its cap, so the grouping stays.
 static const int AA_GRID = 768;
 
 static void aa_release(AaHist *a)
