@@ -49,8 +49,9 @@ int amgd_diag_inverses(const DBsr &A, double *Minv, float *Minv32);
 // Layouts of a level operator: block-wise WROW values, sliced-ELL widths and fill.
 int amgd_rptr64(const DBsr &A, int64_t *rptr);
 int amgd_wrow_vals(const DBsr &A, double *val);
-int amgd_slice_widths(const DBsr &A, int *sptr, int *nslots);
-int amgd_slice_fill(const DBsr &A, const int *sptr, int64_t nslots, int *col, double *val);
+int amgd_slice_widths(const DBsr &A, int *sptr, int *nslots, const int *perm = nullptr);
+int amgd_slice_fill(const DBsr &A, const int *sptr, int64_t nslots, int *col, double *val,
+                    const int *perm = nullptr);
 int amgd_to_f32(const double *src, float *dst, int64_t count);
 // y = Minv x with planar block inverses (d x d), the power iteration's step.
 int amgd_block_diag_mv(int n, int d, const double *Minv, const double *x, double *y);

@@ -830,7 +830,8 @@ int amgd_wrow_vals(const DBsr &A, double *val)
     return LPMX_OK;
 }
 
-__global__ void k_slice_width(int nr, int nslices, const int *__restrict__ ptr, int *__restrict__ w)
+__global__ void k_slice_width(int nr, int nslices, const int *__restrict__ ptr, const int *__restrict__ perm,
+                              int *__restrict__ w)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s > nslices)
@@ -838,17 +839,20 @@ __global__ void k_slice_width(int nr, int nslices, const int *__restrict__ ptr, 
     int m = 0;
     if (s < nslices)
         for (int l = 0; l < 64 && s * 64 + l < nr; l++)
-            m = max(m, ptr[s * 64 + l + 1] - ptr[s * 64 + l]);
+        {
+            const int i = perm ? perm[s * 64 + l] : s * 64 + l;
+            m = max(m, ptr[i + 1] - ptr[i]);
+        }
     w[s] = m;
 }
 
-int amgd_slice_widths(const DBsr &A, int *sptr, int *nslots)
+int amgd_slice_widths(const DBsr &A, int *sptr, int *nslots, const int *perm)
 {
     const int ns = (A.nr + 63) / 64;
     int *w = nullptr;
     RETCHK(dalloc(&w, (int64_t)ns + 1));
     hipLaunchKernelGGL(k_slice_width, dim3(nblocks((int64_t)ns + 1)), dim3(LPMX_BLOCK), 0, g_stream, A.nr, ns, A.ptr,
-                       w);
+                       perm, w);
     LAUNCHCHK();
     const int rc = lpmx_dev_scan_int(w, sptr, ns + 1, nslots);
     HIPCHK(hipStreamSynchronize(g_stream));
@@ -856,16 +860,18 @@ int amgd_slice_widths(const DBsr &A, int *sptr, int *nslots)
     return rc;
 }
 
-// Sliced ELL: row i's k-th block in slot sptr[i/64] + k, lane i%64; padding slots keep
-// a zero block at a valid column (the row itself for a square matrix).
+// Sliced ELL: the row at position p (row perm[p], or p without a map) has its k-th
+// block in slot sptr[p/64] + k, lane p%64; padding slots keep a zero block at a valid
+// column (the row itself for a square matrix).
 __global__ void k_slice_fill(int nr, int nc, int bs, const int *__restrict__ ptr, const int *__restrict__ idx,
-                             const double *__restrict__ src, const int *__restrict__ sptr, int *__restrict__ col,
-                             double *__restrict__ val)
+                             const double *__restrict__ src, const int *__restrict__ perm, const int *__restrict__ sptr,
+                             int *__restrict__ col, double *__restrict__ val)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nr)
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= nr)
         return;
-    const int s = i >> 6, l = i & 63;
+    const int i = perm ? perm[p] : p;
+    const int s = p >> 6, l = p & 63;
     const int p0 = ptr[i], len = ptr[i + 1] - p0, w = sptr[s + 1] - sptr[s];
     for (int k = 0; k < w; k++)
     {
@@ -881,13 +887,13 @@ __global__ void k_slice_fill(int nr, int nc, int bs, const int *__restrict__ ptr
     }
 }
 
-int amgd_slice_fill(const DBsr &A, const int *sptr, int64_t nslots, int *col, double *val)
+int amgd_slice_fill(const DBsr &A, const int *sptr, int64_t nslots, int *col, double *val, const int *perm)
 {
     const int bs = A.br * A.bc;
     HIPCHK(hipMemsetAsync(col, 0, sizeof(int) * nslots * 64, g_stream));
     HIPCHK(hipMemsetAsync(val, 0, sizeof(double) * nslots * bs * 64, g_stream));
     hipLaunchKernelGGL(k_slice_fill, dim3(nblocks(A.nr)), dim3(LPMX_BLOCK), 0, g_stream, A.nr, A.nc, bs, A.ptr, A.idx,
-                       A.val, sptr, col, val);
+                       A.val, perm, sptr, col, val);
     LAUNCHCHK();
     return LPMX_OK;
 }
