@@ -221,6 +221,14 @@ static struct Amg
 // rows from which one thread per row fills the device (256 CUs x 1024 threads)
 static const int SLICE_MIN_ROWS = 262144;
 
+// LPMX_AMG_SLICE_MIN=<rows> (a test hook): the sliced layout from that many rows on,
+// with the rows always sorted, so that small models exercise it
+static int slice_min_override(void)
+{
+    const char *e = getenv("LPMX_AMG_SLICE_MIN");
+    return e ? atoi(e) : 0;
+}
+
 // Copies between the host and the model's data, ordered on the library's stream (a
 // non-blocking stream: a copy on the null stream would not wait for the kernels
 // queued on it, e.g. the constraint elimination right before a solve).
@@ -890,7 +898,7 @@ static int slice_order(const DBsr &H, int **perm)
             mp = std::max(mp, ptr[p + 1] - ptr[p]), ms = std::max(ms, ptr[ord[p] + 1] - ptr[ord[p]]);
         plain += mp, sorted += ms;
     }
-    if (sorted > plain - plain / 20)
+    if (sorted > plain - plain / 20 && slice_min_override() <= 0)
         return LPMX_OK;
     RETCHK(amg_alloc(perm, (int64_t)H.nr));
     HIPCHK(hipMemcpyAsync(*perm, ord.data(), sizeof(int) * (size_t)H.nr, hipMemcpyHostToDevice, g_stream));
@@ -906,7 +914,9 @@ static int to_mat(const DBsr &H, Mat &B)
     // one thread per row (SLICE) for many rows, or for a fair number of short rows (a
     // wavefront on a row of a few blocks leaves most lanes idle); a wavefront per row
     // (WROW) for long rows
-    const bool slice = H.nr >= SLICE_MIN_ROWS || (H.nr >= 65536 && (int64_t)H.nnzb <= 48LL * H.nr);
+    const int smin = slice_min_override();
+    const bool slice = (smin > 0 && H.nr >= smin) || H.nr >= SLICE_MIN_ROWS ||
+                       (H.nr >= 65536 && (int64_t)H.nnzb <= 48LL * H.nr);
     if (!slice)
     {
         B.layout = LAYOUT_WROW;

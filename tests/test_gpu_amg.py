@@ -209,8 +209,12 @@ def build_constrained(case):
     return n, d, x
 
 
-@pytest.mark.parametrize("case", ["sc", "hex"])
-def test_amg_hierarchy_matches_definition(case):
+@pytest.mark.parametrize("case,layout", [("sc", "default"), ("hex", "default"), ("sc", "sliced")])
+def test_amg_hierarchy_matches_definition(case, layout, monkeypatch):
+    # "sliced": every level's matrices in the sliced layout with their rows sorted by
+    # length (at 1M only the prolongator is), which the export maps back to rows
+    if layout == "sliced":
+        monkeypatch.setenv("LPMX_AMG_SLICE_MIN", "1")
     n, d, x = build_constrained(case)
     nm = 6 if d == 3 else 3
     L = P.lib()
