@@ -25,7 +25,9 @@
  *                           test (interior stress = C:eps); radial-return closed form;
  *                           the predictor continues the elastic law between two states
  *   orc_bond_force_j2energy serial order = Jacobi form bit for bit; the elastic branch
- *                           is the elastic law (the plastic energy formula: unpinned)
+ *                           is the elastic law; a plastic return whose root lies in
+ *                           the bisection bracket lands on the hardened yield surface
+ *                           when its energy is recomputed from the updated stretches
  *   orc_bond_force_j2nonlinear the elastic branch is the elastic law; the Jacobi
  *                           multiplier is the root of the hardening law to TOLITER
  *   orc_damage_bwise_local  reverse particle order gives the same state; pairwise

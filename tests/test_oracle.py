@@ -448,6 +448,52 @@ def test_j2energy_elastic_branch_is_the_elastic_law(oracle):
     np.testing.assert_array_equal(m3.stress, m6.stress)
 
 
+def test_j2energy_plastic_return_lands_on_the_yield_surface(oracle):
+    """The plastic branch of the energy-based J2 law (constitutive.c:286-463) returns
+    to the hardened yield surface.  From the updated state alone (slot 2): the
+    particle's elastic stretches with its new plastic stretch, their deviatoric energy
+    U_d = sum_shells 1/2 Kn (dl - shell mean)^2, sigma = sqrt(6 c44 U_d / V) with
+    V = volume nb / nbmax, and the updated back stress and hardening satisfy
+    |sigma - beta_eq| = sigmay + (1 - xi) H alpha to the bisection's tolerance
+    (dlambda within TOLITER = 1e-4 of the root).  This ties the plastic stretch
+    increment (dlambda f_d / 4 Kn with f_d scaled by 1 / (1 + dlambda/2)), the
+    energy-stress relation and the alpha and beta_eq updates together without
+    restating the return equation; a factor of 2 off in any of them moves sigma by
+    about dlambda sigma / 2, over 1% at the particles checked.  The strain is small
+    enough that the root lies inside the bisection's bracket [0, 1] (past it the
+    return stops at dlambda = 1, short of the surface); the update ends with the
+    slot 2 -> 0 copy, so slot 2 is the updated state."""
+    m, _ = sc(oracle, 7.1, sigmay=200.0, J2_H=2000.0, J2_xi=0.3)
+    rng = np.random.default_rng(41)
+    live = m.nbr >= 0
+    m.dLp[:, :, 0] = 2e-5 * rng.standard_normal(live.shape) * live
+    m.alpha[:, 0] = 1e-3 * rng.uniform(size=m.n)
+    m.beta_eq[:, 0] = 20.0 * rng.standard_normal(m.n)
+    eps = 1.1e-3 * np.array([[1.0, 0.2, 0.0], [0.2, -0.5, 0.1], [0.0, 0.1, -0.5]])
+    m.x[:] = m.x0 + m.x0 @ eps.T + 3e-5 * rng.standard_normal(m.x.shape)
+    oracle.lib().orc_bond_force_j2energy(m.p, 1, 0)
+    s = m.s
+    H, xi, nbm = s.J2_H, s.J2_xi, s.nbmax
+    Ce = np.asarray(m.Ce).reshape(-1, 3)
+    sel = np.flatnonzero((m.pl_flag == 1) & (m.dlambda > 1e-2) & (m.dlambda < 0.99))
+    assert len(sel) > 100
+    worst = 0.0
+    for i in sel:
+        nb = int(m.nb[i])
+        assert nb == m.nb_initial[i]  # no broken bonds here
+        q = m.nbr[i, :nb]
+        dl = np.linalg.norm(m.x[i] - m.x[q], axis=1) - m.L0[i, :nb] - m.dLp[i, :nb, 2]
+        ud = 0.0
+        for shell in (0, 1):
+            k = m.nsign[i, :nb] == shell
+            if k.any():
+                ud += 0.5 * np.sum(m.Kn[i, :nb][k] * (dl[k] - dl[k].mean()) ** 2)
+        sig = np.sqrt(6.0 * Ce[m.type[i], 2] * ud / (s.volume * nb / nbm))
+        f = abs(sig - m.beta_eq[i, 2]) - (m.sigmay[i] + (1.0 - xi) * H * m.alpha[i, 2])
+        worst = max(worst, abs(f) / m.sigmay[i])
+    assert worst < 5e-4, worst
+
+
 # ---------------------------------------------------------------- tangent --------
 def test_analytic_jacobian_matches_central_differences(oracle):
     m, _ = sc(oracle, 3.2)
